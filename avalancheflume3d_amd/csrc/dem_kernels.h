@@ -1,0 +1,74 @@
+// dem_kernels.h -- device data layout and kernel interface of the gfx950 DEM step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dem_math.h"
+
+namespace avf {
+
+// Per-instance parameters, resident in device memory and read by every kernel.  time and
+// cc advance on the device, so a captured graph replays correctly.
+struct InstParams {
+  // system_par fields the ops read (cabecera.hu:34-66)
+  double dt, hdt, time, side_x, side_y;
+  double sint, cost;
+  double kappa_pp, gamma_pp, mu_pp, fr_pp;
+  double kappa_pw, gamma_pw, mu_pw, fr_pw;
+  double dy2, sinfi, cosfi, tanfi;
+  // geometry (cabecera.hu:14-19 as runtime values) and the deposit plane
+  double hop_height, hop_width, yfunnel, lfunnel, yhop_l, yhop_r;
+  double grav, drag, sphere_div;
+  double dep_sin, dep_cos, dep_x0, dep_x1, dep_z0;
+  // clump-cell regrid (cell_dim.cu / avalanche_hst.cu:311-333)
+  double rmax, gyr_max, cell_dist, regrid_ddt_steps;
+  double rr_min[3], cell_side[3], cs_inv[3], rr_max[3], length[3];
+  int64_t ncell[3];
+  // sphere-cell grid of this instance (counting-sort binning)
+  double sg_lo[3], sg_inv_h, sg_h, sg_reach, sg_rsph;
+  int32_t sg_n[3], pad0;
+  int64_t sg_base;
+  // storage range and bookkeeping
+  int64_t begin, count;
+  int64_t cc;
+  int32_t regrid_interval, inter_flag, phase, modules_fill;
+  int32_t modules_aval, pad1;
+};
+
+// Clump state fields (SoA, field-major: st[f * cap + c]).
+enum StateField {
+  S_RX, S_RY, S_RZ, S_VX, S_VY, S_VZ, S_WX, S_WY, S_WZ,
+  S_R00, S_R01, S_R02, S_R10, S_R11, S_R12, S_R20, S_R21, S_R22,
+  S_AX, S_AY, S_AZ, S_GX, S_GY, S_GZ, S_NFIELDS
+};
+// Static per-clump fields (radius changes only when a clump freezes).
+enum StaticField { C_RAD, C_RAD0, C_RHO, C_MASS, C_IX, C_IY, C_IZ, C_NFIELDS };
+// Force / torque of the last force stage.
+enum ForceField { F_FX, F_FY, F_FZ, F_TX, F_TY, F_TZ, F_NFIELDS };
+
+// 192-B record per clump written by the predict pass and gathered by neighbours:
+// rr[3], rad, vv[3], rho, ww[3], (orig | flags), sphere centres[4][3].
+constexpr int kStageDoubles = 24;
+enum StageSlot { G_RR = 0, G_RAD = 3, G_VV = 4, G_RHO = 7, G_WW = 8, G_ID = 11, G_SPH = 12 };
+
+constexpr uint32_t kInvalidKey = 0xffffffffu;
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+// Device-side diagnostics (avf_diag).
+struct DevDiag {
+  unsigned long long cell_below, cell_beyond, neigh_overflow, max_neighbours, max_cell, pad;
+};
+
+// Order-preserving int64 image of a double (for integer atomic min / max).
+__device__ __forceinline__ long long order_key(double v) {
+  long long b = __double_as_longlong(v);
+  return b >= 0 ? b : (b ^ 0x7fffffffffffffffll);
+}
+__device__ __forceinline__ double order_val(long long k) {
+  return __longlong_as_double(k >= 0 ? k : (k ^ 0x7fffffffffffffffll));
+}
+
+}  // namespace avf

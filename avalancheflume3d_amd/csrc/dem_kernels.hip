@@ -1,0 +1,1099 @@
+// dem_kernels.hip -- gfx950 kernels of the DEM step for tetrahedral 4-sphere clumps.
+//
+// One step (reference: avalanche_hst.cu:668-710, avalanche; :438-467, hopper filling):
+//   k_predict   init_Verlet.cu:11-152 fused with the sphere-centre update, the 192-B stage
+//               record and the sphere-cell keys (integer atomic ranks: counting sort);
+//   k_scan_*    exclusive scan of the sphere-cell counts;
+//   k_scatter   places sphere records (x, y, z, r) + id in cell order;
+//   k_force     boundary modules + force_part (27-cell sweep of the sphere grid, contacts
+//               folded in the reference's summation order) + get_accel + finish_Verlet;
+//   k_cellreduce / k_finalize   cell_dim + regrid every regrid_interval steps, cc++.
+// No float atomics anywhere: per-clump sums are sequential in the reference order.
+#include "dem_kernels.h"
+
+namespace avf {
+
+// ----------------------------------------------------------------------- helpers ----
+
+__device__ __forceinline__ int grid_index(double x, double lo, double inv_h, int n) {
+  double f = (x - lo) * inv_h;
+  if (!(f >= 0.0)) return 0;
+  if (f >= (double)n) return n - 1;
+  return (int)f;
+}
+
+__device__ __forceinline__ uint32_t sphere_key(const InstParams &p, d3 q) {
+  int ix = grid_index(q.x, p.sg_lo[0], p.sg_inv_h, p.sg_n[0]);
+  int iy = grid_index(q.y, p.sg_lo[1], p.sg_inv_h, p.sg_n[1]);
+  int iz = grid_index(q.z, p.sg_lo[2], p.sg_inv_h, p.sg_n[2]);
+  return (uint32_t)(p.sg_base + ix + (int64_t)p.sg_n[0] * (iy + (int64_t)p.sg_n[1] * iz));
+}
+
+// Sphere centres rr + (rad/div) R (+-1,+-1,+-1), init_Verlet.cu:130-142.
+__device__ __forceinline__ void sphere_centres(d3 rr, const double *R, double pos, d3 *s) {
+  s[0] = make_d3(rr.x + pos * (R[0] + R[1] + R[2]), rr.y + pos * (R[3] + R[4] + R[5]),
+                 rr.z + pos * (R[6] + R[7] + R[8]));
+  s[1] = make_d3(rr.x + pos * (R[0] - R[1] - R[2]), rr.y + pos * (R[3] - R[4] - R[5]),
+                 rr.z + pos * (R[6] - R[7] - R[8]));
+  s[2] = make_d3(rr.x + pos * (-R[0] - R[1] + R[2]), rr.y + pos * (-R[3] - R[4] + R[5]),
+                 rr.z + pos * (-R[6] - R[7] + R[8]));
+  s[3] = make_d3(rr.x + pos * (-R[0] + R[1] - R[2]), rr.y + pos * (-R[3] + R[4] - R[5]),
+                 rr.z + pos * (-R[6] + R[7] - R[8]));
+}
+
+__device__ __forceinline__ d3 pick(const d3 *s, int i) {
+  d3 r = s[0];
+  if (i == 1) r = s[1];
+  if (i == 2) r = s[2];
+  if (i == 3) r = s[3];
+  return r;
+}
+
+// --------------------------------------------------------------------- k_predict ----
+
+// mode 1: predict (init_Verlet) then stage; mode 0: stage the current state only (initial
+// force pass, avalanche_hst.cu:365-384, whose force stage sees the unchanged time).
+__global__ void k_predict(double *__restrict__ st, double *__restrict__ sta,
+                          double *__restrict__ stage, const int32_t *__restrict__ inst_of,
+                          const int32_t *__restrict__ orig, const InstParams *__restrict__ P,
+                          uint32_t *__restrict__ counts, uint32_t *__restrict__ skey,
+                          uint32_t *__restrict__ srank, int64_t n, int64_t cap, int mode) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const InstParams &p = P[inst_of[c]];
+  const double t_old = p.time;
+  const double t_new = mode ? t_old + p.dt : t_old;
+  const double dt = p.dt, hdt = p.hdt;
+
+  d3 rr = make_d3(st[S_RX * cap + c], st[S_RY * cap + c], st[S_RZ * cap + c]);
+  double R[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) R[k] = st[(S_R00 + k) * cap + c];
+  double rad = sta[C_RAD * cap + c];
+  const double rad0 = sta[C_RAD0 * cap + c];
+  d3 vv = make_d3(st[S_VX * cap + c], st[S_VY * cap + c], st[S_VZ * cap + c]);
+  d3 ww = make_d3(st[S_WX * cap + c], st[S_WY * cap + c], st[S_WZ * cap + c]);
+
+  if (mode) {
+    if (frozen(t_old, rr.x, p.side_x)) {
+      rad = 0.0;
+      sta[C_RAD * cap + c] = 0.0;
+    } else {
+      d3 aa = make_d3(st[S_AX * cap + c], st[S_AY * cap + c], st[S_AZ * cap + c]);
+      d3 ang = make_d3(st[S_GX * cap + c], st[S_GY * cap + c], st[S_GZ * cap + c]);
+      // init_Verlet.cu:46-53
+      rr.x += dt * (vv.x + hdt * aa.x);
+      rr.y += dt * (vv.y + hdt * aa.y);
+      rr.z += dt * (vv.z + hdt * aa.z);
+      vv.x += dt * aa.x;
+      vv.y += dt * aa.y;
+      vv.z += dt * aa.z;
+      // init_Verlet.cu:57-89: Rodrigues rotation by dOmega
+      d3 dO = make_d3(dt * (ww.x + hdt * ang.x), dt * (ww.y + hdt * ang.y),
+                      dt * (ww.z + hdt * ang.z));
+      double mag = sqrt(dO.x * dO.x + dO.y * dO.y + dO.z * dO.z);
+      double dR[9];
+      if (mag > 0.0) {
+        d3 nn = make_d3(dO.x / mag, dO.y / mag, dO.z / mag);
+        double cdO = cos(mag), sdO = sin(mag), om = 1.0 - cdO;
+        dR[0] = om * nn.x * nn.x + cdO;
+        dR[1] = om * nn.x * nn.y - sdO * nn.z;
+        dR[2] = om * nn.x * nn.z + sdO * nn.y;
+        dR[3] = om * nn.y * nn.x + sdO * nn.z;
+        dR[4] = om * nn.y * nn.y + cdO;
+        dR[5] = om * nn.y * nn.z - sdO * nn.x;
+        dR[6] = om * nn.z * nn.x - sdO * nn.y;
+        dR[7] = om * nn.z * nn.y + sdO * nn.x;
+        dR[8] = om * nn.z * nn.z + cdO;
+      } else {
+        dR[0] = 1.0; dR[1] = 0.0; dR[2] = 0.0;
+        dR[3] = 0.0; dR[4] = 1.0; dR[5] = 0.0;
+        dR[6] = 0.0; dR[7] = 0.0; dR[8] = 1.0;
+      }
+      // init_Verlet.cu:92-100: T = dR R (rows)
+      double T[9];
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+          T[3 * i + j] = dR[3 * i] * R[j] + dR[3 * i + 1] * R[3 + j] + dR[3 * i + 2] * R[6 + j];
+      // init_Verlet.cu:103-114: normalise each column
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        double norm = sqrt(T[j] * T[j] + T[3 + j] * T[3 + j] + T[6 + j] * T[6 + j]);
+        R[j] = T[j] / norm;
+        R[3 + j] = T[3 + j] / norm;
+        R[6 + j] = T[6 + j] / norm;
+      }
+      ww.x += dt * ang.x;
+      ww.y += dt * ang.y;
+      ww.z += dt * ang.z;
+      st[S_RX * cap + c] = rr.x;
+      st[S_RY * cap + c] = rr.y;
+      st[S_RZ * cap + c] = rr.z;
+      st[S_VX * cap + c] = vv.x;
+      st[S_VY * cap + c] = vv.y;
+      st[S_VZ * cap + c] = vv.z;
+      st[S_WX * cap + c] = ww.x;
+      st[S_WY * cap + c] = ww.y;
+      st[S_WZ * cap + c] = ww.z;
+#pragma unroll
+      for (int k = 0; k < 9; k++) st[(S_R00 + k) * cap + c] = R[k];
+    }
+  }
+
+  // Sphere centres always use the radius the clump had before it froze: a frozen clump's
+  // centres are the stale ones of the step it crossed side.x (rr, R unchanged since).
+  d3 sph[4];
+  sphere_centres(rr, R, rad0 / p.sphere_div, sph);
+
+  double *g = stage + c * kStageDoubles;
+  g[G_RR + 0] = rr.x;
+  g[G_RR + 1] = rr.y;
+  g[G_RR + 2] = rr.z;
+  g[G_RAD] = rad;
+  g[G_VV + 0] = vv.x;
+  g[G_VV + 1] = vv.y;
+  g[G_VV + 2] = vv.z;
+  g[G_RHO] = sta[C_RHO * cap + c];
+  g[G_WW + 0] = ww.x;
+  g[G_WW + 1] = ww.y;
+  g[G_WW + 2] = ww.z;
+  g[G_ID] = __longlong_as_double((long long)(uint32_t)orig[c]);
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    g[G_SPH + 3 * s + 0] = sph[s].x;
+    g[G_SPH + 3 * s + 1] = sph[s].y;
+    g[G_SPH + 3 * s + 2] = sph[s].z;
+  }
+
+  // Binning at the force-stage time: clumps past side.x are left out (cells.cu:65).
+  const bool out = frozen(t_new, rr.x, p.side_x);
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    uint32_t key = kInvalidKey, rank = 0;
+    if (!out) {
+      key = sphere_key(p, sph[s]);
+      rank = atomicAdd(&counts[key], 1u);
+    }
+    skey[4 * c + s] = key;
+    srank[4 * c + s] = rank;
+  }
+}
+
+// ------------------------------------------------------------------------- scan -----
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+// Exclusive block scan; *total receives the block sum.  lds needs kScanBlock/64 words.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total, uint32_t *lds) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  uint32_t x = wave_incl_scan(v);
+  __syncthreads();
+  if (lane == 63) lds[wid] = x;
+  __syncthreads();
+  if (wid == 0) {
+    uint32_t w = lane < nw ? lds[lane] : 0u;
+    w = wave_incl_scan(w);
+    if (lane < nw) lds[lane] = w;
+  }
+  __syncthreads();
+  uint32_t off = wid ? lds[wid - 1] : 0u;
+  *total = lds[nw - 1];
+  return off + x - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_a(const uint32_t *__restrict__ counts,
+                                                       uint32_t *__restrict__ bsum,
+                                                       const int64_t *__restrict__ ncells_p) {
+  __shared__ uint32_t lds[kScanBlock / 64];
+  const int64_t ncells = *ncells_p;
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  if (base >= ncells) return;
+  const int64_t i0 = base + (int64_t)threadIdx.x * kScanItems;
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++)
+    if (i0 + k < ncells) s += counts[i0 + k];
+  uint32_t tot;
+  block_excl_scan(s, &tot, lds);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// One block: exclusive scan of the tile sums; also advances time when requested (between
+// the predict pass that computed time + dt and the force pass that uses it).
+__global__ __launch_bounds__(1024) void k_scan_b(uint32_t *__restrict__ bsum,
+                                                 const int64_t *__restrict__ ncells_p,
+                                                 uint32_t *__restrict__ starts) {
+  __shared__ uint32_t lds[1024 / 64];
+  const int64_t ncells = *ncells_p;
+  const int64_t nb = (ncells + kScanTile - 1) / kScanTile;
+  const int64_t per = (nb + blockDim.x - 1) / blockDim.x;
+  const int64_t j0 = (int64_t)threadIdx.x * per;
+  uint32_t s = 0;
+  for (int64_t k = 0; k < per; k++)
+    if (j0 + k < nb) s += bsum[j0 + k];
+  uint32_t tot;
+  uint32_t off = block_excl_scan(s, &tot, lds);
+  for (int64_t k = 0; k < per; k++)
+    if (j0 + k < nb) {
+      uint32_t v = bsum[j0 + k];
+      bsum[j0 + k] = off;
+      off += v;
+    }
+  if (threadIdx.x == 0) starts[ncells] = tot;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_c(uint32_t *__restrict__ counts,
+                                                       const uint32_t *__restrict__ boff,
+                                                       uint32_t *__restrict__ starts,
+                                                       const int64_t *__restrict__ ncells_p) {
+  __shared__ uint32_t lds[kScanBlock / 64];
+  const int64_t ncells = *ncells_p;
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  if (base >= ncells) return;
+  const int64_t i0 = base + (int64_t)threadIdx.x * kScanItems;
+  uint32_t v[kScanItems], s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    v[k] = (i0 + k < ncells) ? counts[i0 + k] : 0u;
+    s += v[k];
+  }
+  uint32_t tot;
+  uint32_t off = block_excl_scan(s, &tot, lds) + boff[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++)
+    if (i0 + k < ncells) {
+      starts[i0 + k] = off;
+      off += v[k];
+      counts[i0 + k] = 0u;
+    }
+}
+
+// ----------------------------------------------------------------------- scatter ----
+
+__global__ void k_scatter(const uint32_t *__restrict__ skey, const uint32_t *__restrict__ srank,
+                          const uint32_t *__restrict__ starts, const double *__restrict__ stage,
+                          double4 *__restrict__ spos, uint32_t *__restrict__ sid,
+                          InstParams *__restrict__ P, int32_t ninst, int64_t nsph,
+                          int advance) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (advance && blockIdx.x == 0 && threadIdx.x < ninst) {
+    InstParams &p = P[threadIdx.x];
+    p.time = p.time + p.dt;  // avalanche_hst.cu:672 (sys_par.time += sys_par.dt)
+  }
+  if (i >= nsph) return;
+  uint32_t key = skey[i];
+  if (key == kInvalidKey) return;
+  uint32_t dst = starts[key] + srank[i];
+  const double *g = stage + (i >> 2) * kStageDoubles;
+  const int s = (int)(i & 3);
+  spos[dst] = make_double4(g[G_SPH + 3 * s], g[G_SPH + 3 * s + 1], g[G_SPH + 3 * s + 2],
+                           g[G_RAD]);
+  sid[dst] = (uint32_t)i;
+}
+
+// ------------------------------------------------------------------ boundaries -----
+
+// force_hop.cu:9-244 for one nw (0: x walls, 1: y walls, 2: bottom gate while time < 0).
+__device__ void hop_walls(const InstParams &p, int nw, double time, d3 rr, d3 vv, d3 ww,
+                          const d3 *sph, double ri, double mass, d3 &ff, d3 &tq) {
+  const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
+  const double xfront = p.hop_width, xrear = 0.0, yleft = p.yhop_l, yright = p.yhop_r;
+  const double zbottom = p.hop_height;
+  for (int ns = 0; ns < 4; ns++) {
+    d3 r = sph[ns], rrc = make_d3(0, 0, 0), nvec = make_d3(0, 0, 0);
+    double dist, comp = 0.0;
+    int touch = 0;
+    if (r.z < zbottom || r.x > p.hop_width) continue;
+    if (nw == 0) {
+      dist = r.x - xrear;
+      if (dist <= ri) {
+        touch = 1;
+        rrc = make_d3(xrear, r.y, r.z);
+        nvec = make_d3(-1.0, 0.0, 0.0);
+        comp = ri - dist;
+      }
+      dist = xfront - r.x;
+      if (dist <= ri) {
+        touch = 2;
+        rrc = make_d3(xfront, r.y, r.z);
+        nvec = make_d3(1.0, 0.0, 0.0);
+        comp = ri - dist;
+      }
+    } else if (nw == 1) {
+      dist = r.y - yleft;
+      if (dist <= ri) {
+        touch = 3;
+        rrc = make_d3(r.x, yleft, r.z);
+        nvec = make_d3(0.0, -1.0, 0.0);
+        comp = ri - dist;
+      }
+      dist = yright - r.y;
+      if (dist <= ri) {
+        touch = 4;
+        rrc = make_d3(r.x, yright, r.z);
+        nvec = make_d3(0.0, 1.0, 0.0);
+        comp = ri - dist;
+      }
+    } else {
+      dist = r.z - zbottom;
+      if (dist <= ri) {
+        touch = (time < 0.0) ? 5 : 0;
+        rrc = make_d3(r.x, r.y, zbottom);
+        nvec = make_d3(0.0, 0.0, -1.0);
+        comp = ri - dist;
+      }
+    }
+    if (touch)
+      wall_contact(rr, vv, ww, rrc, nvec, comp, km, gm, p.mu_pw, p.fr_pw, p.dt, mass, false, ff,
+                   tq);
+  }
+}
+
+// force_funnel.cu:9-213: converging side planes of the funnel (slope frame).
+__device__ void funnel_walls(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph, double ri,
+                             double mass, d3 &ff, d3 &tq) {
+  const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
+  const double sinfi = p.sinfi, cosfi = p.cosfi, tanfi = p.tanfi, cost = p.cost, sint = p.sint;
+  for (int ns = 0; ns < 4; ns++) {
+    d3 q = to_slope(sph[ns], cost, sint), rrc = make_d3(0, 0, 0), nvec = make_d3(0, 0, 0);
+    double dist, comp = 0.0;
+    int touch = 0;
+    if (q.x > p.lfunnel - ri * sinfi) continue;
+    dist = fabs(q.y - q.x * tanfi) * cosfi;
+    if (dist <= ri) {
+      touch = 1;
+      d3 rc = make_d3(q.x + dist * sinfi, q.y - dist * cosfi, q.z);
+      rrc = to_lab(rc, cost, sint);
+      nvec = to_lab(make_d3(sinfi, -cosfi, 0.0), cost, sint);
+      comp = ri - dist;
+    }
+    dist = fabs(p.yfunnel - q.y - q.x * tanfi) * cosfi;
+    if (dist <= ri) {
+      touch = 2;
+      d3 rc = make_d3(q.x + dist * sinfi, q.y + dist * cosfi, q.z);
+      rrc = to_lab(rc, cost, sint);
+      nvec = to_lab(make_d3(sinfi, cosfi, 0.0), cost, sint);
+      comp = ri - dist;
+    }
+    if (touch)
+      wall_contact(rr, vv, ww, rrc, nvec, comp, km, gm, p.mu_pw, p.fr_pw, p.dt, mass, false, ff,
+                   tq);
+  }
+}
+
+// force_funnel.cu:216-404: vertical corner edges at the funnel exit.
+__device__ void funnel_corners(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph,
+                               double ri, double mass, d3 &ff, d3 &tq) {
+  const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
+  const double yfun_l = p.dy2, yfun_r = p.side_y + p.dy2, hsy = p.dy2 + p.side_y / 2.0;
+  const double lf = p.lfunnel, cost = p.cost, sint = p.sint;
+  for (int ns = 0; ns < 4; ns++) {
+    d3 q = to_slope(sph[ns], cost, sint);
+    if (q.x < lf - ri * p.sinfi || q.x > lf + ri) continue;
+    if (q.y > yfun_r || q.y < yfun_l) continue;
+    double yc = (q.y <= hsy) ? yfun_l : yfun_r;
+    double sep = ri, sep2 = sep * sep;
+    d3 drr;
+    drr.x = lf - q.x;
+    double dist2 = drr.x * drr.x;
+    if (dist2 > sep2) continue;
+    drr.y = yc - q.y;
+    dist2 += drr.y * drr.y;
+    if (dist2 > sep2) continue;
+    double dist = sqrt(dist2), comp = sep - dist;
+    d3 nr = make_d3(drr.x / dist, drr.y / dist, 0.0);
+    d3 rc = make_d3(q.x + ri * nr.x, q.y + ri * nr.y, q.z);
+    d3 rrc = to_lab(rc, cost, sint), nvec = to_lab(nr, cost, sint);
+    wall_contact(rr, vv, ww, rrc, nvec, comp, km, gm, p.mu_pw, p.fr_pw, p.dt, mass, true, ff, tq);
+  }
+}
+
+// force_walls.cu:9-199: flume side walls beyond the funnel (caller skips frozen clumps).
+__device__ void side_walls(const InstParams &p, double time, d3 rr, d3 vv, d3 ww, const d3 *sph,
+                           double ri, double mass, d3 &ff, d3 &tq) {
+  const double km = 0.5 * p.kappa_pw * mass;
+  double gm = 0.5 * p.gamma_pw * mass, mu = p.mu_pw, cns_1 = p.fr_pw;
+  const double yleft = p.dy2, yright = p.side_y + p.dy2;
+  for (int ns = 0; ns < 4; ns++) {
+    d3 r = sph[ns], q = to_slope(r, p.cost, p.sint);
+    d3 rrc = make_d3(0, 0, 0), nvec = make_d3(0, 0, 0);
+    double dist, comp = 0.0;
+    int touch = 0;
+    if (q.x <= p.lfunnel) continue;
+    dist = r.y - yleft;
+    if (dist <= ri) {
+      touch = 1;
+      rrc = make_d3(r.x, yleft, r.z);
+      nvec = make_d3(0.0, -1.0, 0.0);
+      comp = ri - dist;
+    }
+    dist = yright - r.y;
+    if (dist <= ri) {
+      touch = 2;
+      rrc = make_d3(r.x, yright, r.z);
+      nvec = make_d3(0.0, 1.0, 0.0);
+      comp = ri - dist;
+    }
+    if (!p.inter_flag && time > 0.0) gm = mu = cns_1 = 0.0;
+    if (touch)
+      wall_contact(rr, vv, ww, rrc, nvec, comp, km, gm, mu, cns_1, p.dt, mass, false, ff, tq);
+  }
+}
+
+// force_flume.cu:10-182: inclined floor z = -x tan(theta).
+__device__ void flume_floor(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph, double ri,
+                            double mass, d3 &ff, d3 &tq) {
+  const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
+  const double sint = p.sint, cost = p.cost, tant = sint / cost;
+  for (int ns = 0; ns < 4; ns++) {
+    d3 r = sph[ns];
+    double rect = -tant * r.x - r.z, dist = fabs(cost * rect);
+    if (!(dist <= ri)) continue;
+    d3 nvec = make_d3(-sint, 0.0, -cost);
+    d3 rrc = make_d3(r.x - dist * sint, r.y, r.z - dist * cost);
+    wall_contact(rr, vv, ww, rrc, nvec, ri - dist, km, gm, p.mu_pw, p.fr_pw, p.dt, mass, false, ff,
+                 tq);
+  }
+}
+
+// force_dep.cu:10-178 with the deposit plane of avf_geometry.
+__device__ void deposit_floor(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph, double ri,
+                              double mass, d3 &ff, d3 &tq) {
+  const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
+  const double sind = p.dep_sin, cosd = p.dep_cos, tand = sind / cosd;
+  for (int ns = 0; ns < 4; ns++) {
+    d3 r = sph[ns];
+    double rect = tand * (r.x - p.dep_x0 - p.dep_x1) - (r.z - p.dep_z0), dist = cosd * (-rect);
+    if (!(dist <= ri)) continue;
+    d3 nvec = make_d3(sind, 0.0, -cosd);
+    d3 rrc;
+    rrc.x = r.x + dist * sind;
+    rrc.y = r.y;
+    rrc.z = tand * (rrc.x - p.dep_x0 - p.dep_x1) + p.dep_z0;
+    wall_contact(rr, vv, ww, rrc, nvec, ri - dist, km, gm, p.mu_pw, p.fr_pw, p.dt, mass, false, ff,
+                 tq);
+  }
+}
+
+// ------------------------------------------------------------------- pp contacts ----
+
+// Reference clump-cell of a clump centre by the cell_locate arithmetic (multiply by
+// cs_inv, cells.cu:69-77) as the linear index cell_locate stores it (cells.cu:91; an index
+// above ncell aliases into another cell exactly as in the reference).  Returns -1 if the
+// reference would abort (negative index) or write outside its cell table.
+__device__ __forceinline__ int64_t ref_cell_locate(const InstParams &p, d3 r) {
+  int64_t ii = (int64_t)(p.cs_inv[0] * (r.x - p.rr_min[0]));
+  if (ii == p.ncell[0]) ii--;
+  int64_t jj = (int64_t)(p.cs_inv[1] * (r.y - p.rr_min[1]));
+  if (jj == p.ncell[1]) jj--;
+  int64_t kk = (int64_t)(p.cs_inv[2] * (r.z - p.rr_min[2]));
+  if (kk == p.ncell[2]) kk--;
+  if (ii < 0 || jj < 0 || kk < 0) return -1;
+  int64_t idx = ii + p.ncell[0] * (jj + kk * p.ncell[1]);
+  if (idx >= p.ncell[0] * p.ncell[1] * p.ncell[2]) return -1;
+  return idx;
+}
+
+template <int MAXN, int BS>
+struct PPShared {
+  int32_t m[MAXN][BS];       // storage index of each touching clump, sorted by key
+  uint32_t key[MAXN][BS];    // (clump-cell offset << 27) | reference index
+  uint16_t mask[MAXN][BS];   // bit (sdiv*4 + smod): sphere pairs in contact
+  uint32_t pm[16][BS];       // per pairing: bit e set when entry e touches in that pairing
+};
+
+// Main per-clump force pass.  One lane per clump (storage order = spatial order).
+template <int MAXN, int BS>
+__global__ __launch_bounds__(BS) void k_force(
+    double *__restrict__ st, const double *__restrict__ sta, const double *__restrict__ stage,
+    const int32_t *__restrict__ inst_of, const InstParams *__restrict__ P,
+    const uint32_t *__restrict__ starts, const double4 *__restrict__ spos,
+    const uint32_t *__restrict__ sid, double *__restrict__ fout, DevDiag *__restrict__ diag,
+    int64_t n, int64_t cap, int initial) {
+  __shared__ PPShared<MAXN, BS> sh;
+  const int lane = threadIdx.x;
+  const int64_t c = (int64_t)blockIdx.x * BS + lane;
+  if (c >= n) return;
+  const InstParams &p = P[inst_of[c]];
+  const double time = p.time;
+
+  const d3 rr = make_d3(st[S_RX * cap + c], st[S_RY * cap + c], st[S_RZ * cap + c]);
+  d3 vv = make_d3(st[S_VX * cap + c], st[S_VY * cap + c], st[S_VZ * cap + c]);
+  d3 ww = make_d3(st[S_WX * cap + c], st[S_WY * cap + c], st[S_WZ * cap + c]);
+  double R[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) R[k] = st[(S_R00 + k) * cap + c];
+  const double rad = sta[C_RAD * cap + c], rad0 = sta[C_RAD0 * cap + c];
+  const double rho = sta[C_RHO * cap + c];
+  d3 sph[4];
+  sphere_centres(rr, R, rad0 / p.sphere_div, sph);
+
+  const bool frz = frozen(time, rr.x, p.side_x);
+  const int mods = p.phase ? p.modules_aval : p.modules_fill;
+  const double massw = kSphFact * rad * rad * rad * rho;  // force_hop.cu:45 order
+  d3 ff = make_d3(0.0, 0.0, 0.0), tq = make_d3(0.0, 0.0, 0.0);
+
+  if (p.phase) {  // avalanche_hst.cu:688-696
+    if (mods & AVF_MOD_HOP)
+      for (int nw = 0; nw < 3; nw++) hop_walls(p, nw, time, rr, vv, ww, sph, rad, massw, ff, tq);
+    if (mods & AVF_MOD_FUNNEL) funnel_walls(p, rr, vv, ww, sph, rad, massw, ff, tq);
+    if (mods & AVF_MOD_FCORNERS) funnel_corners(p, rr, vv, ww, sph, rad, massw, ff, tq);
+    if ((mods & AVF_MOD_WALLS) && !frz) side_walls(p, time, rr, vv, ww, sph, rad, massw, ff, tq);
+  }
+
+  if ((mods & AVF_MOD_PP) && !frz) {
+    // ---- detection: sweep the 27 sphere cells around each of the 4 spheres ----------
+    const int nx = p.sg_n[0], ny = p.sg_n[1], nz = p.sg_n[2];
+    int cnt = 0;
+    bool overflow = false;
+    // l's clump cell by DIVISION (force_part.cu:77-85)
+    int64_t li = (int64_t)((rr.x - p.rr_min[0]) / p.cell_side[0]);
+    if (li == p.ncell[0]) li--;
+    int64_t lj = (int64_t)((rr.y - p.rr_min[1]) / p.cell_side[1]);
+    if (lj == p.ncell[1]) lj--;
+    int64_t lk = (int64_t)((rr.z - p.rr_min[2]) / p.cell_side[2]);
+    if (lk == p.ncell[2]) lk--;
+    const int64_t nxy = p.ncell[0] * p.ncell[1];
+
+    for (int s = 0; s < 4; s++) {
+      const d3 q = sph[s];
+      const int ix = grid_index(q.x, p.sg_lo[0], p.sg_inv_h, nx);
+      const int iy = grid_index(q.y, p.sg_lo[1], p.sg_inv_h, ny);
+      const int iz = grid_index(q.z, p.sg_lo[2], p.sg_inv_h, nz);
+      const int x0 = ix > 0 ? ix - 1 : 0, x1 = ix + 1 < nx ? ix + 1 : nx - 1;
+      for (int dz = -1; dz <= 1; dz++) {
+        const int z = iz + dz;
+        if (z < 0 || z >= nz) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+          const int y = iy + dy;
+          if (y < 0 || y >= ny) continue;
+          const int64_t row = p.sg_base + (int64_t)nx * (y + (int64_t)ny * z);
+          const uint32_t j0 = starts[row + x0], j1 = starts[row + x1 + 1];
+          for (uint32_t j = j0; j < j1; j++) {
+            const uint32_t id = sid[j];
+            const int64_t cm = (int64_t)(id >> 2);
+            if (cm == c) continue;
+            const double4 o = spos[j];
+            const double sep = rad + o.w, sep2 = sep * sep;
+            const double dx = o.x - q.x, dyy = o.y - q.y, dzz = o.z - q.z;
+            const double d2 = dx * dx + dyy * dyy + dzz * dzz;
+            if (!(d2 < sep2)) continue;
+            // contact (l sphere s, m sphere id&3): add to m's entry or insert m by key
+            const uint32_t bit = 1u << (((id & 3u) << 2) | (uint32_t)s);
+            int e = 0;
+            for (; e < cnt; e++)
+              if (sh.m[e][lane] == (int32_t)cm) break;
+            if (e < cnt) {
+              sh.mask[e][lane] = (uint16_t)(sh.mask[e][lane] | bit);
+              continue;
+            }
+            const double *g = stage + cm * kStageDoubles;
+            const d3 rm = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
+            const uint32_t morig = (uint32_t)__double_as_longlong(g[G_ID]);
+            const int64_t L = ref_cell_locate(p, rm);
+            if (L < 0) {
+              atomicAdd(&diag->cell_below, 1ull);
+              continue;
+            }
+            const int64_t ak = L / nxy, rem = L - ak * nxy;
+            const int64_t aj = rem / p.ncell[0], ai = rem - aj * p.ncell[0];
+            const int64_t di = ai - li, dj = aj - lj, dk = ak - lk;
+            if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) {
+              // outside the 27 clump cells the reference visits (force_part.cu:88-96)
+              atomicAdd(&diag->cell_beyond, 1ull);
+              continue;
+            }
+            const uint32_t off = (uint32_t)((di + 1) * 9 + (dj + 1) * 3 + (dk + 1));
+            const uint32_t key = (off << 27) | morig;
+            if (cnt == MAXN) {
+              overflow = true;
+              continue;
+            }
+            int pos = cnt;
+            while (pos > 0 && sh.key[pos - 1][lane] > key) {
+              sh.key[pos][lane] = sh.key[pos - 1][lane];
+              sh.m[pos][lane] = sh.m[pos - 1][lane];
+              sh.mask[pos][lane] = sh.mask[pos - 1][lane];
+              pos--;
+            }
+            sh.key[pos][lane] = key;
+            sh.m[pos][lane] = (int32_t)cm;
+            sh.mask[pos][lane] = (uint16_t)bit;
+            cnt++;
+          }
+        }
+      }
+    }
+    if (overflow) atomicAdd(&diag->neigh_overflow, 1ull);
+    atomicMax(&diag->max_neighbours, (unsigned long long)cnt);
+
+    // ---- fold: pairings sph_id = 1..16 (m sphere outer, l sphere inner), then clump
+    //      cells in (idel, jdel, kdel) order, then ascending m (force_part.cu:40-41,88-108)
+    const double massl = kSphFact * rho * rad * rad * rad;
+    const double kappa = p.kappa_pp, gamma = p.gamma_pp, mu = p.mu_pp, cns_1 = p.fr_pp;
+    const double dt = p.dt;
+#pragma unroll
+    for (int pr = 0; pr < 16; pr++) {
+      uint32_t bits = 0;
+      for (int e = 0; e < cnt; e++) bits |= (uint32_t)((sh.mask[e][lane] >> pr) & 1u) << e;
+      sh.pm[pr][lane] = bits;
+    }
+    // walk the set bits lane by lane so every lane evaluates only its own contacts
+    int pr = 0;
+    uint32_t cur = sh.pm[0][lane];
+    while (true) {
+      while (cur == 0u && pr < 15) cur = sh.pm[++pr][lane];
+      if (cur == 0u) break;
+      const int e = __builtin_ctz(cur);
+      cur &= cur - 1u;
+      const int sdiv = pr >> 2, smod = pr & 3;
+      const d3 r1 = pick(sph, smod);
+      {
+        const int64_t cm = sh.m[e][lane];
+        const double *g = stage + cm * kStageDoubles;
+        const d3 rrm = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
+        const double rad_2 = g[G_RAD], rhom = g[G_RHO];
+        const d3 vvm = make_d3(g[G_VV], g[G_VV + 1], g[G_VV + 2]);
+        const d3 wwm = make_d3(g[G_WW], g[G_WW + 1], g[G_WW + 2]);
+        const d3 r2 = make_d3(g[G_SPH + 3 * sdiv], g[G_SPH + 3 * sdiv + 1], g[G_SPH + 3 * sdiv + 2]);
+        // force_part.cu:119-242
+        const double massm = kSphFact * rhom * rad_2 * rad_2 * rad_2;
+        const double meff = 0.5 * (massm + massl);
+        const double km = 0.5 * kappa * meff, gm = 0.5 * gamma * meff;
+        const double sep = rad + rad_2;
+        d3 drr = make_d3(r2.x - r1.x, r2.y - r1.y, r2.z - r1.z);
+        const double dist2 = drr.x * drr.x + drr.y * drr.y + drr.z * drr.z;
+        const double dist = sqrt(dist2), comp = sep - dist;
+        const d3 nvec = make_d3(drr.x / dist, drr.y / dist, drr.z / dist);
+        const d3 rrc = make_d3(r1.x + rad * nvec.x, r1.y + rad * nvec.y, r1.z + rad * nvec.z);
+        const d3 vl = point_vel(vv, ww, rrc, rr), vm = point_vel(vvm, wwm, rrc, rrm);
+        const d3 dvvc = make_d3(vm.x - vl.x, vm.y - vl.y, vm.z - vl.z);
+        Contact cl;
+        if (contact_normal(cl, dvvc, nvec, comp, km, gm) > 0.0) {
+          const d3 f = contact_friction(cl, mu, cns_1, dt, massl, true);
+          accumulate(ff, tq, rrc, rr, f);
+        }
+      }
+    }
+  }
+
+  if (!p.phase) {  // avalanche_hst.cu:460-461 (pp first, then hop, while filling)
+    if (mods & AVF_MOD_HOP)
+      for (int nw = 0; nw < 3; nw++) hop_walls(p, nw, time, rr, vv, ww, sph, rad, massw, ff, tq);
+  } else {  // avalanche_hst.cu:704
+    if ((mods & AVF_MOD_FLUME) && !frz) flume_floor(p, rr, vv, ww, sph, rad, massw, ff, tq);
+    if (mods & AVF_MOD_DEP) deposit_floor(p, rr, vv, ww, sph, rad, massw, ff, tq);
+  }
+
+  fout[F_FX * cap + c] = ff.x;
+  fout[F_FY * cap + c] = ff.y;
+  fout[F_FZ * cap + c] = ff.z;
+  fout[F_TX * cap + c] = tq.x;
+  fout[F_TY * cap + c] = tq.y;
+  fout[F_TZ * cap + c] = tq.z;
+
+  if (frz) {
+    if (!initial) {  // finish_Verlet.cu:28-32
+      st[S_VX * cap + c] = 0.0;
+      st[S_VY * cap + c] = 0.0;
+      st[S_VZ * cap + c] = 0.0;
+    }
+    return;
+  }
+
+  // get_accel.cu:27-71
+  const double mass = sta[C_MASS * cap + c];
+  const d3 I = make_d3(sta[C_IX * cap + c], sta[C_IY * cap + c], sta[C_IZ * cap + c]);
+  const double Cxy = (I.x - I.y) / I.z, Cyz = (I.y - I.z) / I.x, Czx = (I.z - I.x) / I.y;
+  double adrag = p.drag * rad * rad / mass;
+  const double vmag = sqrt(vv.x * vv.x + vv.y * vv.y + vv.z * vv.z);
+  adrag *= vmag;
+  d3 af;
+  af.x = ff.x / mass - adrag * vv.x;
+  af.y = ff.y / mass - adrag * vv.y;
+  af.z = ff.z / mass - adrag * vv.z - p.grav;
+  const d3 wb = make_d3(ww.x * R[0] + ww.y * R[3] + ww.z * R[6], ww.x * R[1] + ww.y * R[4] + ww.z * R[7],
+                        ww.x * R[2] + ww.y * R[5] + ww.z * R[8]);
+  const d3 tb = make_d3(tq.x * R[0] + tq.y * R[3] + tq.z * R[6], tq.x * R[1] + tq.y * R[4] + tq.z * R[7],
+                        tq.x * R[2] + tq.y * R[5] + tq.z * R[8]);
+  const d3 ab = make_d3(tb.x / I.x + Cyz * wb.y * wb.z, tb.y / I.y + Czx * wb.z * wb.x,
+                        tb.z / I.z + Cxy * wb.x * wb.y);
+  d3 gf;
+  gf.x = ab.x * R[0] + ab.y * R[1] + ab.z * R[2];
+  gf.y = ab.x * R[3] + ab.y * R[4] + ab.z * R[5];
+  gf.z = ab.x * R[6] + ab.y * R[7] + ab.z * R[8];
+
+  if (!initial) {  // finish_Verlet.cu:43-52
+    const double hdt = p.hdt;
+    const d3 ai = make_d3(st[S_AX * cap + c], st[S_AY * cap + c], st[S_AZ * cap + c]);
+    const d3 gi = make_d3(st[S_GX * cap + c], st[S_GY * cap + c], st[S_GZ * cap + c]);
+    vv.x += hdt * (af.x - ai.x);
+    vv.y += hdt * (af.y - ai.y);
+    vv.z += hdt * (af.z - ai.z);
+    ww.x += hdt * (gf.x - gi.x);
+    ww.y += hdt * (gf.y - gi.y);
+    ww.z += hdt * (gf.z - gi.z);
+    st[S_VX * cap + c] = vv.x;
+    st[S_VY * cap + c] = vv.y;
+    st[S_VZ * cap + c] = vv.z;
+    st[S_WX * cap + c] = ww.x;
+    st[S_WY * cap + c] = ww.y;
+    st[S_WZ * cap + c] = ww.z;
+  }
+  // the accelerations become a_i of the next step (avalanche_hst.cu:844-850)
+  st[S_AX * cap + c] = af.x;
+  st[S_AY * cap + c] = af.y;
+  st[S_AZ * cap + c] = af.z;
+  st[S_GX * cap + c] = gf.x;
+  st[S_GY * cap + c] = gf.y;
+  st[S_GZ * cap + c] = gf.z;
+}
+
+// -------------------------------------------------------------- clump-cell regrid ---
+
+// cell_dim.cu:27-52 over every clump (frozen included) when cc % regrid_interval == 0
+// (force = 1 runs it unconditionally).  acc per instance: min x,y,z, max x,y,z, max vx,
+// min vz as order-preserving integers.
+__global__ void k_cellreduce(const double *__restrict__ st, const int32_t *__restrict__ inst_of,
+                             const InstParams *__restrict__ P, long long *__restrict__ acc,
+                             int64_t n, int64_t cap, int force) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int inst = inst_of[c];
+  const InstParams &p = P[inst];
+  if (!force && !(p.regrid_interval > 0 && p.cc % p.regrid_interval == 0)) return;
+  long long *a = acc + 8 * inst;
+  atomicMin(&a[0], order_key(st[S_RX * cap + c]));
+  atomicMin(&a[1], order_key(st[S_RY * cap + c]));
+  atomicMin(&a[2], order_key(st[S_RZ * cap + c]));
+  atomicMax(&a[3], order_key(st[S_RX * cap + c]));
+  atomicMax(&a[4], order_key(st[S_RY * cap + c]));
+  atomicMax(&a[5], order_key(st[S_RZ * cap + c]));
+  atomicMax(&a[6], order_key(st[S_VX * cap + c]));
+  atomicMin(&a[7], order_key(st[S_VZ * cap + c]));
+}
+
+// cell_dim.cu:55-60 margins and the regrid arithmetic of avalanche_hst.cu:311-333; then
+// cc++ (avalanche_hst.cu:600, 852).  step = 0 recomputes the grid without counting a step.
+__global__ void k_finalize(InstParams *__restrict__ P, long long *__restrict__ acc, int32_t ninst,
+                           int force, int step) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ninst) return;
+  InstParams &p = P[i];
+  long long *a = acc + 8 * i;
+  if (force || (p.regrid_interval > 0 && p.cc % p.regrid_interval == 0)) {
+    const double rgyr = 0.5 * p.gyr_max, ddt = p.regrid_ddt_steps * p.dt;
+    double rmin[3], rmax[3];
+    for (int k = 0; k < 3; k++) {
+      rmin[k] = order_val(a[k]);
+      rmax[k] = order_val(a[3 + k]);
+    }
+    const double vxmax = order_val(a[6]), vzmin = order_val(a[7]);
+    rmax[0] += 1.05 * rgyr + vxmax * ddt;
+    rmin[0] -= 1.05 * rgyr;
+    rmax[1] += 1.05 * rgyr;
+    rmin[1] -= 1.05 * rgyr;
+    rmax[2] += 1.05 * rgyr;
+    rmin[2] -= 1.05 * rgyr - vzmin * ddt;
+    const double dist = p.cell_dist;
+    for (int k = 0; k < 3; k++) {
+      p.rr_min[k] = rmin[k];
+      p.rr_max[k] = rmax[k];
+      p.length[k] = rmax[k] - rmin[k];
+      p.ncell[k] = (int64_t)(p.length[k] / (dist * p.rmax));
+      p.cell_side[k] = p.length[k] / ((double)p.ncell[k]);
+      p.cs_inv[k] = 1.0 / p.cell_side[k];
+    }
+    for (int k = 0; k < 3; k++) {
+      a[k] = 0x7fffffffffffffffll;
+      a[3 + k] = (long long)0x8000000000000000ull;
+    }
+    a[6] = (long long)0x8000000000000000ull;
+    a[7] = 0x7fffffffffffffffll;
+  }
+  if (step) p.cc += 1;
+}
+
+// --------------------------------------------------------------- sphere-cell grid ---
+
+// AABB of clump centres and max |v| per instance (acc: 8 per instance as in
+// k_cellreduce; slot 6 holds max |v|^2 as an ordered double).
+__global__ void k_sgrid_reduce(const double *__restrict__ st, const int32_t *__restrict__ inst_of,
+                               long long *__restrict__ acc, int64_t n, int64_t cap) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  long long *a = acc + 8 * inst_of[c];
+  const double x = st[S_RX * cap + c], y = st[S_RY * cap + c], z = st[S_RZ * cap + c];
+  const double vx = st[S_VX * cap + c], vy = st[S_VY * cap + c], vz = st[S_VZ * cap + c];
+  atomicMin(&a[0], order_key(x));
+  atomicMin(&a[1], order_key(y));
+  atomicMin(&a[2], order_key(z));
+  atomicMax(&a[3], order_key(x));
+  atomicMax(&a[4], order_key(y));
+  atomicMax(&a[5], order_key(z));
+  atomicMax(&a[6], order_key(vx * vx + vy * vy + vz * vz));
+}
+
+// Lays out every instance's sphere grid (cell side h >= 2 * largest sphere radius, so any
+// touching pair sits in neighbouring cells; indices are clamped into the grid, which keeps
+// that property for spheres outside it) and assigns key ranges.  Coarsens h until the
+// total fits the cell capacity.  Single thread.
+__global__ void k_sgrid_finalize(InstParams *__restrict__ P, long long *__restrict__ acc,
+                                 int32_t ninst, int64_t cell_cap, double factor, double horizon,
+                                 int64_t *__restrict__ ncells_out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double scale = 1.0;
+  for (int attempt = 0; attempt < 64; attempt++) {
+    int64_t total = 0;
+    for (int i = 0; i < ninst; i++) {
+      InstParams &p = P[i];
+      long long *a = acc + 8 * i;
+      const double vmax = sqrt(order_val(a[6]));
+      const double margin = p.sg_reach + vmax * horizon * p.dt + 1e-9;
+      const double h = 2.0 * p.sg_rsph * factor * scale * (1.0 + 1e-9);
+      int64_t nd[3];
+      for (int k = 0; k < 3; k++) {
+        const double lo = order_val(a[k]) - margin, hi = order_val(a[3 + k]) + margin;
+        double len = hi - lo;
+        if (!(len > 0.0)) len = h;
+        int64_t m = (int64_t)ceil(len / h);
+        if (m < 1) m = 1;
+        if (m > (1 << 20)) m = 1 << 20;
+        nd[k] = m;
+        p.sg_lo[k] = lo;
+      }
+      p.sg_h = h;
+      p.sg_inv_h = 1.0 / h;
+      p.sg_n[0] = (int32_t)nd[0];
+      p.sg_n[1] = (int32_t)nd[1];
+      p.sg_n[2] = (int32_t)nd[2];
+      p.sg_base = total;
+      total += nd[0] * nd[1] * nd[2];
+    }
+    if (total <= cell_cap) {
+      *ncells_out = total;
+      break;
+    }
+    scale *= cbrt((double)total / (double)cell_cap) * 1.05;
+  }
+  for (int i = 0; i < ninst; i++) {
+    long long *a = acc + 8 * i;
+    for (int k = 0; k < 3; k++) {
+      a[k] = 0x7fffffffffffffffll;
+      a[3 + k] = (long long)0x8000000000000000ull;
+    }
+    a[6] = (long long)0x8000000000000000ull;
+    a[7] = 0x7fffffffffffffffll;
+  }
+}
+
+// ---------------------------------------------------------------------- reorder -----
+
+// Key of each clump for the spatial reorder: the sphere cell of its centre.
+__global__ void k_reorder_key(const double *__restrict__ st, const int32_t *__restrict__ inst_of,
+                              const InstParams *__restrict__ P, uint32_t *__restrict__ counts,
+                              uint32_t *__restrict__ ckey, uint32_t *__restrict__ crank,
+                              int64_t n, int64_t cap) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const InstParams &p = P[inst_of[c]];
+  const uint32_t key =
+      sphere_key(p, make_d3(st[S_RX * cap + c], st[S_RY * cap + c], st[S_RZ * cap + c]));
+  ckey[c] = key;
+  crank[c] = atomicAdd(&counts[key], 1u);
+}
+
+// new index = starts[key] + rank; gathers every per-clump array into the new order.
+__global__ void k_reorder_move(const uint32_t *__restrict__ ckey, const uint32_t *__restrict__ crank,
+                               const uint32_t *__restrict__ starts, const double *__restrict__ st,
+                               double *__restrict__ st2, const double *__restrict__ sta,
+                               double *__restrict__ sta2, const int32_t *__restrict__ orig,
+                               int32_t *__restrict__ orig2, const int32_t *__restrict__ inst_of,
+                               int32_t *__restrict__ inst2, const InstParams *__restrict__ P,
+                               int64_t *__restrict__ inv, int64_t n, int64_t cap) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int64_t d = (int64_t)starts[ckey[c]] + crank[c];
+  for (int f = 0; f < S_NFIELDS; f++) st2[f * cap + d] = st[f * cap + c];
+  for (int f = 0; f < C_NFIELDS; f++) sta2[f * cap + d] = sta[f * cap + c];
+  orig2[d] = orig[c];
+  inst2[d] = inst_of[c];
+  inv[P[inst_of[c]].begin + orig[c]] = d;
+}
+
+// ----------------------------------------------------------- export / debug kernels ---
+
+// Reference AoS records (cabecera.hu:77-98) from the device state, in reference order.
+// Sphere centres use the pre-freeze radius (stale centres of a frozen clump); rrot_* are
+// the rotate_vectors.cu:28-46 images of the centres.
+__global__ void k_export_aos(const double *__restrict__ st, const double *__restrict__ sta,
+                             const int32_t *__restrict__ orig, const int32_t *__restrict__ inst_of,
+                             const InstParams *__restrict__ P, int32_t inst, double *__restrict__ out,
+                             double *__restrict__ aa, double *__restrict__ fo,
+                             const double *__restrict__ fin, int64_t n, int64_t cap) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n || inst_of[c] != inst) return;
+  const InstParams &p = P[inst];
+  const int64_t o = orig[c];
+  double *r = out + o * 51;
+  const d3 rr = make_d3(st[S_RX * cap + c], st[S_RY * cap + c], st[S_RZ * cap + c]);
+  double R[9];
+  for (int k = 0; k < 9; k++) R[k] = st[(S_R00 + k) * cap + c];
+  d3 sph[4];
+  sphere_centres(rr, R, sta[C_RAD0 * cap + c] / p.sphere_div, sph);
+  r[0] = sta[C_RAD * cap + c];
+  r[1] = sta[C_MASS * cap + c];
+  r[2] = sta[C_RHO * cap + c];
+  r[3] = sta[C_IX * cap + c];
+  r[4] = sta[C_IY * cap + c];
+  r[5] = sta[C_IZ * cap + c];
+  for (int s = 0; s < 4; s++) {
+    r[6 + 3 * s] = sph[s].x;
+    r[7 + 3 * s] = sph[s].y;
+    r[8 + 3 * s] = sph[s].z;
+    const d3 q = to_slope(sph[s], p.cost, p.sint);
+    r[18 + 3 * s] = q.x;
+    r[19 + 3 * s] = q.y;
+    r[20 + 3 * s] = q.z;
+  }
+  r[30] = rr.x;
+  r[31] = rr.y;
+  r[32] = rr.z;
+  const d3 q = to_slope(rr, p.cost, p.sint);
+  r[33] = q.x;
+  r[34] = q.y;
+  r[35] = q.z;
+  for (int k = 0; k < 3; k++) r[36 + k] = st[(S_VX + k) * cap + c];
+  for (int k = 0; k < 3; k++) r[39 + k] = st[(S_WX + k) * cap + c];
+  for (int k = 0; k < 9; k++) r[42 + k] = R[k];
+  if (aa)
+    for (int k = 0; k < 6; k++) aa[o * 6 + k] = st[(S_AX + k) * cap + c];
+  if (fo)
+    for (int k = 0; k < 6; k++) fo[o * 6 + k] = fin[k * cap + c];
+}
+
+// Inverse of k_export_aos for uploads (sphere centres and rrot are derived, not stored).
+// extra holds 8 doubles per clump: a_i[3], alpha_i[3], pre-freeze radius, unused.
+__global__ void k_import_aos(double *__restrict__ st, double *__restrict__ sta,
+                             const int64_t *__restrict__ inv, int64_t begin,
+                             const double *__restrict__ in, const double *__restrict__ extra,
+                             int64_t count, int64_t cap) {
+  int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= count) return;
+  const int64_t c = inv[begin + o];
+  const double *r = in + o * 51;
+  const double *x = extra + o * 8;
+  sta[C_RAD * cap + c] = r[0];
+  sta[C_MASS * cap + c] = r[1];
+  sta[C_RHO * cap + c] = r[2];
+  sta[C_IX * cap + c] = r[3];
+  sta[C_IY * cap + c] = r[4];
+  sta[C_IZ * cap + c] = r[5];
+  sta[C_RAD0 * cap + c] = x[6];
+  for (int k = 0; k < 3; k++) st[(S_RX + k) * cap + c] = r[30 + k];
+  for (int k = 0; k < 3; k++) st[(S_VX + k) * cap + c] = r[36 + k];
+  for (int k = 0; k < 3; k++) st[(S_WX + k) * cap + c] = r[39 + k];
+  for (int k = 0; k < 9; k++) st[(S_R00 + k) * cap + c] = r[42 + k];
+  for (int k = 0; k < 6; k++) st[(S_AX + k) * cap + c] = x[k];
+}
+
+// Reference clump-cell index per clump (cells.cu:69-91), -1 for clumps cell_locate skips.
+__global__ void k_export_cells(const double *__restrict__ st, const int32_t *__restrict__ orig,
+                               const int32_t *__restrict__ inst_of, const InstParams *__restrict__ P,
+                               int32_t inst, int64_t *__restrict__ out, int64_t n, int64_t cap) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n || inst_of[c] != inst) return;
+  const InstParams &p = P[inst];
+  const d3 r = make_d3(st[S_RX * cap + c], st[S_RY * cap + c], st[S_RZ * cap + c]);
+  int64_t v = -1;
+  if (!frozen(p.time, r.x, p.side_x)) {
+    int64_t ii = (int64_t)(p.cs_inv[0] * (r.x - p.rr_min[0]));
+    if (ii == p.ncell[0]) ii--;
+    int64_t jj = (int64_t)(p.cs_inv[1] * (r.y - p.rr_min[1]));
+    if (jj == p.ncell[1]) jj--;
+    int64_t kk = (int64_t)(p.cs_inv[2] * (r.z - p.rr_min[2]));
+    if (kk == p.ncell[2]) kk--;
+    v = (ii < 0 || jj < 0 || kk < 0) ? -2 : ii + p.ncell[0] * (jj + kk * p.ncell[1]);
+  }
+  out[orig[c]] = v;
+}
+
+// Directed contacts of the current state through the sphere grid (debug export).
+__global__ void k_export_contacts(const double *__restrict__ st, const double *__restrict__ sta,
+                                  const double *__restrict__ stage, const int32_t *__restrict__ orig,
+                                  const int32_t *__restrict__ inst_of, const InstParams *__restrict__ P,
+                                  int32_t inst, const uint32_t *__restrict__ starts,
+                                  const double4 *__restrict__ spos, const uint32_t *__restrict__ sid,
+                                  int64_t *__restrict__ out, unsigned long long *__restrict__ count,
+                                  int64_t capn, int64_t n, int64_t cap) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n || inst_of[c] != inst) return;
+  const InstParams &p = P[inst];
+  const double *gl = stage + c * kStageDoubles;
+  const d3 rr = make_d3(gl[G_RR], gl[G_RR + 1], gl[G_RR + 2]);
+  if (frozen(p.time, rr.x, p.side_x)) return;
+  const double rad = gl[G_RAD], rho = gl[G_RHO];
+  const d3 vv = make_d3(gl[G_VV], gl[G_VV + 1], gl[G_VV + 2]);
+  const d3 ww = make_d3(gl[G_WW], gl[G_WW + 1], gl[G_WW + 2]);
+  const double massl = kSphFact * rho * rad * rad * rad;
+  const int nx = p.sg_n[0], ny = p.sg_n[1], nz = p.sg_n[2];
+  for (int s = 0; s < 4; s++) {
+    const d3 q = make_d3(gl[G_SPH + 3 * s], gl[G_SPH + 3 * s + 1], gl[G_SPH + 3 * s + 2]);
+    const int ix = grid_index(q.x, p.sg_lo[0], p.sg_inv_h, nx);
+    const int iy = grid_index(q.y, p.sg_lo[1], p.sg_inv_h, ny);
+    const int iz = grid_index(q.z, p.sg_lo[2], p.sg_inv_h, nz);
+    const int x0 = ix > 0 ? ix - 1 : 0, x1 = ix + 1 < nx ? ix + 1 : nx - 1;
+    for (int z = iz - 1; z <= iz + 1; z++) {
+      if (z < 0 || z >= nz) continue;
+      for (int y = iy - 1; y <= iy + 1; y++) {
+        if (y < 0 || y >= ny) continue;
+        const int64_t row = p.sg_base + (int64_t)nx * (y + (int64_t)ny * z);
+        for (uint32_t j = starts[row + x0]; j < starts[row + x1 + 1]; j++) {
+          const uint32_t id = sid[j];
+          const int64_t cm = (int64_t)(id >> 2);
+          if (cm == c) continue;
+          const double4 o = spos[j];
+          const double sep = rad + o.w, sep2 = sep * sep;
+          const double dx = o.x - q.x, dy = o.y - q.y, dz = o.z - q.z;
+          const double d2 = dx * dx + dy * dy + dz * dz;
+          if (!(d2 < sep2)) continue;
+          const double *g = stage + cm * kStageDoubles;
+          const double rad_2 = g[G_RAD], rhom = g[G_RHO];
+          const double massm = kSphFact * rhom * rad_2 * rad_2 * rad_2;
+          const double meff = 0.5 * (massm + massl);
+          const double km = 0.5 * p.kappa_pp * meff, gm = 0.5 * p.gamma_pp * meff;
+          const double dist = sqrt(d2);
+          const d3 nvec = make_d3(dx / dist, dy / dist, dz / dist);
+          const d3 rrc = make_d3(q.x + rad * nvec.x, q.y + rad * nvec.y, q.z + rad * nvec.z);
+          const d3 rrm = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
+          const d3 vl = point_vel(vv, ww, rrc, rr);
+          const d3 vm = point_vel(make_d3(g[G_VV], g[G_VV + 1], g[G_VV + 2]),
+                                  make_d3(g[G_WW], g[G_WW + 1], g[G_WW + 2]), rrc, rrm);
+          Contact cl;
+          const bool rep =
+              contact_normal(cl, make_d3(vm.x - vl.x, vm.y - vl.y, vm.z - vl.z), nvec,
+                             sep - dist, km, gm) > 0.0;
+          const unsigned long long k = atomicAdd(count, 1ull);
+          if ((int64_t)k < capn) {
+            out[4 * k + 0] = orig[c];
+            out[4 * k + 1] = (int64_t)(uint32_t)__double_as_longlong(g[G_ID]);
+            out[4 * k + 2] = ((int64_t)s) | ((int64_t)(id & 3) << 8);
+            out[4 * k + 3] = rep ? 1 : 0;
+          }
+        }
+      }
+    }
+  }
+}
+
+}  // namespace avf

@@ -1,0 +1,256 @@
+"""Python front end of the C ABI (include/avflume.h): setup helpers and the device context.
+
+The reference's operator set (cabecera.hu:115-157) is reached through one context object:
+``Context.step`` runs the step loop of avalanche_hst.cu:668-710 (or :438-467 while filling)
+on the GPU, ``upload``/``download`` move reference-layout ``particle`` arrays.  Every call
+checks the status code and raises ``AvfError`` with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+from .abi import ptr
+
+
+class AvfError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{abi.STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib() -> C.CDLL:
+    return abi.load_library()
+
+
+def _check(rc: int, ctx=None) -> None:
+    if rc != 0:
+        msg = lib().avf_last_error(ctx)
+        raise AvfError(rc, msg.decode() if msg else "")
+
+
+# --------------------------------------------------------------------------- setup ---
+
+def reference_geometry() -> abi.Geometry:
+    g = abi.Geometry()
+    lib().avf_geometry_reference(ptr(g))
+    return g
+
+
+def default_options() -> abi.Options:
+    o = abi.Options()
+    lib().avf_options_default(ptr(o))
+    return o
+
+
+class Rng:
+    """rannew64 (rannew64.cu:55-126) with explicit state."""
+
+    def __init__(self):
+        self.st = abi.Rng()
+        lib().avf_rng_init(ptr(self.st))
+
+    def __call__(self, idum: int) -> tuple[float, int]:
+        d = C.c_int64(idum)
+        v = lib().avf_rannew64(ptr(self.st), C.byref(d))
+        return v, d.value
+
+
+def parse_datainput(text: str) -> tuple[abi.System, int, int]:
+    sys = abi.System()
+    ntags, pf = C.c_int64(0), C.c_int32(0)
+    _check(lib().avf_parse_datainput(text.encode(), ptr(sys), C.byref(ntags), C.byref(pf)))
+    return sys, ntags.value, pf.value
+
+
+def derive_system(sys: abi.System, geo: abi.Geometry) -> abi.Funnel:
+    fun = abi.Funnel()
+    lib().avf_derive_system(ptr(sys), ptr(fun), ptr(geo))
+    return fun
+
+
+def error_check(sys: abi.System) -> tuple[int, str]:
+    buf = C.create_string_buffer(256)
+    code = lib().avf_error_check(ptr(sys), buf, 256)
+    return code, buf.value.decode()
+
+
+def new_particles(n: int) -> np.ndarray:
+    return np.zeros(n, dtype=abi.particle_dtype)
+
+
+def generate_particles(pp: np.ndarray, sys: abi.System, density: float = 2.6) -> None:
+    lib().avf_generate_particles(ptr(pp), ptr(sys), C.c_double(density))
+
+
+def calculate_particle_parameters(pp: np.ndarray, sys: abi.System, geo: abi.Geometry) -> None:
+    lib().avf_calculate_particle_parameters(ptr(pp), ptr(sys), ptr(geo))
+
+
+def place_particles(pp: np.ndarray, sys: abi.System, geo: abi.Geometry) -> None:
+    _check(lib().avf_place_particles(ptr(pp), ptr(sys), ptr(geo)))
+
+
+def set_initial_velocity(pp: np.ndarray, sys: abi.System, rng: Rng | None = None,
+                         idum0: int = 0) -> Rng:
+    rng = rng or Rng()
+    lib().avf_set_initial_velocity(ptr(pp), ptr(sys), ptr(rng.st), C.c_int64(idum0))
+    return rng
+
+
+def place_lattice(pp: np.ndarray, geo: abi.Geometry, lo, hi, spacing: float, jitter: float,
+                  seed: int) -> None:
+    _check(lib().avf_place_lattice(ptr(pp), C.c_int64(len(pp)), ptr(geo), abi.Double3(*lo),
+                                   abi.Double3(*hi), C.c_double(spacing), C.c_double(jitter),
+                                   C.c_int64(seed)))
+
+
+def sphere_centres(pp: np.ndarray, geo: abi.Geometry) -> None:
+    lib().avf_sphere_centres(ptr(pp), C.c_int64(len(pp)), ptr(geo))
+
+
+def cell_setup(pp: np.ndarray, sys: abi.System, geo: abi.Geometry, ntags: int) -> abi.Cell:
+    cell = abi.Cell()
+    cell.ntags = ntags
+    lib().avf_cell_setup(ptr(cell), ptr(pp), ptr(sys), ptr(geo))
+    return cell
+
+
+# ------------------------------------------------------------------------- context ---
+
+@dataclass
+class Instance:
+    """One independent avalanche: its system, geometry, funnel and clump records."""
+    sys: abi.System
+    geo: abi.Geometry
+    fun: abi.Funnel
+    particles: np.ndarray
+    ntags: int = 9
+    phase: int = abi.PHASE_FILL
+    aa_i: np.ndarray | None = None
+    anga_i: np.ndarray | None = None
+    extra: dict = field(default_factory=dict)
+
+    def desc(self) -> abi.InstanceDesc:
+        d = abi.InstanceDesc()
+        d.sys = self.sys
+        d.sys.npart = len(self.particles)
+        d.geo = self.geo
+        d.fun = self.fun
+        d.ntags = self.ntags
+        d.phase = self.phase
+        return d
+
+
+class Context:
+    """Device context stepping one or more instances in the same kernels."""
+
+    def __init__(self, instances: list[Instance], options: abi.Options | None = None,
+                 upload: bool = True):
+        self.instances = list(instances)
+        descs = (abi.InstanceDesc * len(self.instances))(*[i.desc() for i in self.instances])
+        self.options = options or default_options()
+        h = C.c_void_p()
+        _check(lib().avf_create(C.byref(h), descs, len(self.instances), ptr(self.options)))
+        self.h = h
+        if upload:
+            for k, inst in enumerate(self.instances):
+                self.upload(k, inst.particles, inst.aa_i, inst.anga_i)
+
+    def _ck(self, rc: int) -> None:
+        _check(rc, self.h)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().avf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def n(self, inst: int = 0) -> int:
+        return len(self.instances[inst].particles)
+
+    def upload(self, inst: int, pp: np.ndarray, aa=None, anga=None) -> None:
+        n = self.n(inst)
+        if aa is None:
+            aa = abi.v3(n)
+        if anga is None:
+            anga = abi.v3(n)
+        pp = np.ascontiguousarray(pp, dtype=abi.particle_dtype)
+        self._ck(lib().avf_upload(self.h, inst, ptr(pp), ptr(np.ascontiguousarray(aa, np.float64)),
+                                  ptr(np.ascontiguousarray(anga, np.float64))))
+
+    def download(self, inst: int = 0):
+        n = self.n(inst)
+        pp = np.zeros(n, dtype=abi.particle_dtype)
+        aa, anga = abi.v3(n), abi.v3(n)
+        self._ck(lib().avf_download(self.h, inst, ptr(pp), ptr(aa), ptr(anga)))
+        return pp, aa, anga
+
+    def set_phase(self, inst: int, phase: int, time: float, side_x: float, cc: int = 0) -> None:
+        self._ck(lib().avf_set_phase(self.h, inst, phase, C.c_double(time), C.c_double(side_x),
+                                     C.c_int64(cc)))
+
+    def time(self, inst: int = 0) -> tuple[float, int]:
+        t, cc = C.c_double(), C.c_int64()
+        self._ck(lib().avf_get_time(self.h, inst, C.byref(t), C.byref(cc)))
+        return t.value, cc.value
+
+    def regrid(self) -> None:
+        self._ck(lib().avf_regrid(self.h))
+
+    def cell(self, inst: int = 0) -> abi.Cell:
+        c = abi.Cell()
+        self._ck(lib().avf_get_cell(self.h, inst, ptr(c)))
+        return c
+
+    def initial_forces(self) -> None:
+        self._ck(lib().avf_initial_forces(self.h))
+
+    def step(self, n: int = 1) -> None:
+        self._ck(lib().avf_step(self.h, C.c_int64(n)))
+
+    def sync(self) -> None:
+        self._ck(lib().avf_sync(self.h))
+
+    def forces(self, inst: int = 0):
+        n = self.n(inst)
+        f, t = abi.v3(n), abi.v3(n)
+        self._ck(lib().avf_get_forces(self.h, inst, ptr(f), ptr(t)))
+        return f, t
+
+    def clump_cells(self, inst: int = 0) -> np.ndarray:
+        out = np.zeros(self.n(inst), dtype=np.int64)
+        self._ck(lib().avf_get_clump_cells(self.h, inst, ptr(out)))
+        return out
+
+    def check_cells(self, inst: int = 0) -> tuple[int, int]:
+        bad = C.c_int64(-1)
+        rc = lib().avf_check_cells(self.h, inst, C.byref(bad))
+        if rc not in (0, 4, 5):
+            self._ck(rc)
+        return rc, bad.value
+
+    def contacts(self, inst: int = 0, cap: int | None = None) -> np.ndarray:
+        cap = cap if cap is not None else 64 * self.n(inst)
+        out = np.zeros(cap, dtype=abi.contact_dtype)
+        cnt = C.c_int64()
+        self._ck(lib().avf_get_contacts(self.h, inst, ptr(out), C.c_int64(cap), C.byref(cnt)))
+        if cnt.value > cap:
+            return self.contacts(inst, cnt.value)
+        return out[:cnt.value]
+
+    def diag(self) -> dict:
+        d = abi.Diag()
+        self._ck(lib().avf_get_diag(self.h, ptr(d)))
+        return {k: getattr(d, k) for k, _ in abi.Diag._fields_}

@@ -1,0 +1,205 @@
+"""ctypes bindings of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loads oracle/liboracle.so (this repository's C restatement of the reference's host ops)
+and, where it was built, oracle/_ref/libref_hst.so (the reference's own *_hst ops compiled
+from their sources).  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+use this module; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from avalancheflume3d_amd import abi
+from avalancheflume3d_amd.abi import ptr
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libref_hst.so")
+
+_orc = None
+_ref = None
+
+
+def build(ref: bool = True) -> None:
+    """Compiles the oracle (and the reference library when its sources are present)."""
+    subprocess.run(["make", "-C", HERE, "oracle"] + (["ref"] if ref else []), check=True,
+                   capture_output=True)
+
+
+def oracle_lib() -> C.CDLL:
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_LIB):
+            raise RuntimeError(f"{ORACLE_LIB} missing: run `make -C oracle oracle`")
+        lib = C.CDLL(ORACLE_LIB)
+        vp, i64, i32 = C.c_void_p, C.c_int64, C.c_int32
+        lib.orc_sim_new.restype = vp
+        lib.orc_sim_new.argtypes = [vp, vp, vp, i64, i32, vp]
+        lib.orc_sim_delete.argtypes = [vp]
+        lib.orc_sim_steps.argtypes = [vp, i64]
+        lib.orc_sim_initial.argtypes = [vp]
+        lib.orc_sim_forces.argtypes = [vp]
+        lib.orc_sim_regrid.argtypes = [vp]
+        lib.orc_sim_open.argtypes = [vp, C.c_double]
+        lib.orc_sim_get.argtypes = [vp] * 8
+        lib.orc_sim_set.argtypes = [vp] * 4
+        lib.orc_sim_record.argtypes = [vp, vp, i64]
+        lib.orc_sim_recorded.restype = i64
+        lib.orc_sim_recorded.argtypes = [vp]
+        lib.orc_sim_error.restype = i32
+        lib.orc_sim_error.argtypes = [vp, vp]
+        lib.orc_brute_contacts.restype = i64
+        lib.orc_brute_contacts.argtypes = [vp, i64, vp, vp, i64]
+        lib.orc_cell_setup.argtypes = [vp, vp, vp, vp]
+        _orc = lib
+    return _orc
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def ref_lib() -> C.CDLL:
+    global _ref
+    if _ref is None:
+        lib = C.CDLL(REF_LIB)
+        vp = C.c_void_p
+        lib.ref_sim_create.restype = vp
+        lib.ref_sim_create.argtypes = [vp, vp, C.c_long, C.c_int, vp]
+        lib.ref_sim_destroy.argtypes = [vp]
+        lib.ref_sim_initial.argtypes = [vp]
+        lib.ref_sim_step.argtypes = [vp, C.c_long]
+        lib.ref_sim_open.argtypes = [vp, C.c_double]
+        lib.ref_sim_get.argtypes = [vp] * 8
+        lib.ref_rannew64.restype = C.c_double
+        lib.ref_rannew64.argtypes = [vp]
+        lib.ref_error_check.restype = C.c_int
+        lib.ref_error_check.argtypes = [vp, vp]
+        _ref = lib
+    return _ref
+
+
+class _Sim:
+    """Common wrapper: one instance of the host step loop."""
+
+    def __init__(self, sys: abi.System, n: int):
+        self.n = n
+        self.sys0 = sys
+
+    def get(self) -> dict:
+        n = self.n
+        out = {"pp": np.zeros(n, dtype=abi.particle_dtype), "aa_i": abi.v3(n),
+               "anga_i": abi.v3(n), "ff": abi.v3(n), "tq": abi.v3(n)}
+        sys, cell = abi.System(), abi.Cell()
+        self._get(ptr(out["pp"]), ptr(out["aa_i"]), ptr(out["anga_i"]), ptr(out["ff"]),
+                  ptr(out["tq"]), ptr(sys), ptr(cell))
+        out["sys"], out["cell"] = sys, cell
+        return out
+
+
+class OracleSim(_Sim):
+    def __init__(self, sys: abi.System, geo: abi.Geometry, fun: abi.Funnel, ntags: int,
+                 phase: int, pp: np.ndarray):
+        super().__init__(sys, len(pp))
+        self.lib = oracle_lib()
+        s = abi.System.from_buffer_copy(sys)
+        s.npart = len(pp)
+        self.h = self.lib.orc_sim_new(ptr(s), ptr(geo), ptr(fun), ntags, phase,
+                                      ptr(np.ascontiguousarray(pp)))
+        if not self.h:
+            raise RuntimeError("orc_sim_new failed")
+        self._recbuf = None
+
+    def _get(self, *a):
+        self.lib.orc_sim_get(self.h, *a)
+
+    def set(self, pp=None, aa=None, anga=None):
+        self.lib.orc_sim_set(self.h, ptr(pp), ptr(aa), ptr(anga))
+
+    def _rc(self, rc):
+        if rc:
+            bad = C.c_int64()
+            self.lib.orc_sim_error(self.h, C.byref(bad))
+            raise RuntimeError(f"oracle error {abi.STATUS.get(rc, rc)} at clump {bad.value}")
+
+    def initial(self):
+        self._rc(self.lib.orc_sim_initial(self.h))
+
+    def forces(self):
+        self._rc(self.lib.orc_sim_forces(self.h))
+
+    def regrid(self):
+        self._rc(self.lib.orc_sim_regrid(self.h))
+
+    def step(self, k: int = 1):
+        self._rc(self.lib.orc_sim_steps(self.h, k))
+
+    def open(self, xfinal: float):
+        self.lib.orc_sim_open(self.h, C.c_double(xfinal))
+
+    def record(self, cap: int):
+        self._recbuf = np.zeros(cap, dtype=abi.contact_dtype)
+        self.lib.orc_sim_record(self.h, ptr(self._recbuf), cap)
+
+    def recorded(self) -> np.ndarray:
+        k = self.lib.orc_sim_recorded(self.h)
+        assert k <= len(self._recbuf), "contact recorder overflow"
+        return self._recbuf[:k].copy()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_sim_delete(self.h)
+            self.h = None
+
+
+class RefSim(_Sim):
+    """The reference's own _hst ops composed in the avalanche_hst.cu step order."""
+
+    def __init__(self, sys: abi.System, fun: abi.Funnel, ntags: int, phase: int, pp: np.ndarray):
+        super().__init__(sys, len(pp))
+        self.lib = ref_lib()
+        s = abi.System.from_buffer_copy(sys)
+        s.npart = len(pp)
+        self.h = self.lib.ref_sim_create(ptr(s), ptr(fun), ntags, phase,
+                                         ptr(np.ascontiguousarray(pp)))
+
+    def _get(self, *a):
+        self.lib.ref_sim_get(self.h, *a)
+
+    def initial(self):
+        self.lib.ref_sim_initial(self.h)
+
+    def step(self, k: int = 1):
+        self.lib.ref_sim_step(self.h, k)
+
+    def open(self, xfinal: float):
+        self.lib.ref_sim_open(self.h, C.c_double(xfinal))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.ref_sim_destroy(self.h)
+            self.h = None
+
+
+def brute_contacts(pp: np.ndarray, sys: abi.System) -> np.ndarray:
+    """O(N^2) contact enumeration (independent of any cell grid), sorted (l, sm, sl, m)."""
+    lib = oracle_lib()
+    s = abi.System.from_buffer_copy(sys)
+    s.npart = len(pp)
+    k = lib.orc_brute_contacts(ptr(pp), len(pp), ptr(s), None, 0)
+    buf = np.zeros(max(k, 1), dtype=abi.contact_dtype)
+    lib.orc_brute_contacts(ptr(pp), len(pp), ptr(s), ptr(buf), len(buf))
+    return buf[:k]
+
+
+def oracle_cell_setup(pp: np.ndarray, sys: abi.System, geo: abi.Geometry, ntags: int) -> abi.Cell:
+    cell = abi.Cell()
+    cell.ntags = ntags
+    s = abi.System.from_buffer_copy(sys)
+    s.npart = len(pp)
+    oracle_lib().orc_cell_setup(ptr(cell), ptr(pp), ptr(s), ptr(geo))
+    return cell
