@@ -184,6 +184,7 @@ def load_library() -> C.CDLL:
             "avf_get_contacts": (C.c_int, [vp, i32, vp, i64, vp]),
             "avf_get_diag": (C.c_int, [vp, vp]),
             "avf_sync": (C.c_int, [vp]),
+            "avf_step_profiled": (C.c_int, [vp, i64, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -202,4 +203,6 @@ EXPORTED = [
     "avf_last_error", "avf_upload", "avf_download", "avf_set_phase", "avf_get_time",
     "avf_regrid", "avf_get_cell", "avf_initial_forces", "avf_step", "avf_get_forces",
     "avf_get_clump_cells", "avf_check_cells", "avf_get_contacts", "avf_get_diag", "avf_sync",
+    "avf_step_profiled",
 ]
+KERNELS = ["predict", "scan_a", "scan_b", "scan_c", "scatter", "force", "cellreduce", "finalize"]

@@ -57,6 +57,7 @@ struct avf_ctx {
   int64_t io_cap = 0;
 
   bool sgrid_valid = false;
+  bool binned = false;  // sorted sphere arrays describe the current positions
   int64_t steps_since_reorder = 0;
   int64_t steps_total = 0;
   std::map<int64_t, hipGraphExec_t> graphs;
@@ -190,12 +191,34 @@ void launch_force(avf_ctx *c, int initial) {
 }
 
 // One full step: avalanche_hst.cu:669-710 (or :439-467) plus the regrid and cc++.
-void launch_step(avf_ctx *c) {
-  launch_bin(c, 1);
+// ev (optional, AVF_NKERNELS + 1 events) brackets every kernel for avf_step_profiled.
+void launch_step(avf_ctx *c, hipEvent_t *ev = nullptr) {
+  auto mark = [&](int k) {
+    if (ev) (void)hipEventRecord(ev[k], c->stream);
+  };
+  const unsigned nb = grid_for(c->cell_cap, kScanTile);
+  mark(AVF_K_PREDICT);
+  k_predict<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(
+      c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
+      c->d_srank, c->n, c->cap, 1);
+  mark(AVF_K_SCAN_A);
+  k_scan_a<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
+  mark(AVF_K_SCAN_B);
+  k_scan_b<<<1, 1024, 0, c->stream>>>(c->d_bsum, c->d_ncells, c->d_starts);
+  mark(AVF_K_SCAN_C);
+  k_scan_c<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_starts, c->d_ncells);
+  mark(AVF_K_SCATTER);
+  k_scatter<<<std::max(1u, grid_for(4 * c->n, kBlock)), kBlock, 0, c->stream>>>(
+      c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid, c->d_P, c->ninst,
+      4 * c->n, 1);
+  mark(AVF_K_FORCE);
   launch_force(c, 0);
+  mark(AVF_K_CELLREDUCE);
   k_cellreduce<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(c->d_st, c->d_inst, c->d_P,
                                                                   c->d_acc_cell, c->n, c->cap, 0);
+  mark(AVF_K_FINALIZE);
   k_finalize<<<grid_for(c->ninst, 64), 64, 0, c->stream>>>(c->d_P, c->d_acc_cell, c->ninst, 0, 1);
+  mark(AVF_NKERNELS);
 }
 
 // Sphere-cell grid from the current positions, then (optionally) the spatial reorder of the
@@ -454,6 +477,7 @@ int avf_upload(avf_ctx *c, int32_t inst, const avf_particle *pp, const avf_doubl
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->sgrid_valid = false;
+  c->binned = false;
   return AVF_OK;
 }
 
@@ -566,6 +590,7 @@ int avf_initial_forces(avf_ctx *c) {
   if (!c->sgrid_valid && (rc = rebuild_grid(c, c->opt.reorder_interval > 0))) return rc;
   launch_bin(c, 0);
   launch_force(c, 1);
+  c->binned = true;
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return AVF_OK;
@@ -583,11 +608,64 @@ int avf_step(avf_ctx *c, int64_t nsteps) {
     }
     int64_t k = std::min(left, chunk_max - (c->steps_since_reorder % chunk_max));
     if ((rc = run_steps(c, k))) return rc;
+    c->binned = true;
     c->steps_since_reorder += k;
     c->steps_total += k;
     left -= k;
   }
   return AVF_OK;
+}
+
+int avf_step_profiled(avf_ctx *c, int64_t nsteps, double *kernel_ms, double *total_ms) {
+  if (!c || nsteps < 0) return AVF_ERR_ARG;
+  constexpr int kBatch = 64;
+  std::vector<hipEvent_t> ev((size_t)kBatch * (AVF_NKERNELS + 1));
+  for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
+  hipEvent_t first, last;
+  HIPCHK(c, hipEventCreate(&first));
+  HIPCHK(c, hipEventCreate(&last));
+  double sum[AVF_NKERNELS] = {0};
+  int rc = AVF_OK;
+  int64_t done = 0;
+  bool started = false;
+  while (done < nsteps && rc == AVF_OK) {
+    int64_t k = std::min<int64_t>(kBatch, nsteps - done);
+    for (int64_t i = 0; i < k; i++) {
+      if (!c->sgrid_valid ||
+          (c->opt.reorder_interval > 0 && c->steps_since_reorder >= c->opt.reorder_interval)) {
+        if ((rc = rebuild_grid(c, c->opt.reorder_interval > 0))) break;
+      }
+      if (!started) {
+        HIPCHK(c, hipEventRecord(first, c->stream));
+        started = true;
+      }
+      launch_step(c, &ev[(size_t)i * (AVF_NKERNELS + 1)]);
+      c->steps_since_reorder++;
+      c->steps_total++;
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < k; i++)
+      for (int j = 0; j < AVF_NKERNELS; j++) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, ev[(size_t)i * (AVF_NKERNELS + 1) + j],
+                                      ev[(size_t)i * (AVF_NKERNELS + 1) + j + 1]));
+        sum[j] += ms;
+      }
+    done += k;
+  }
+  HIPCHK(c, hipEventRecord(last, c->stream));
+  HIPCHK(c, hipEventSynchronize(last));
+  float tot = 0.f;
+  if (started) HIPCHK(c, hipEventElapsedTime(&tot, first, last));
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(first);
+  (void)hipEventDestroy(last);
+  c->binned = true;
+  for (int j = 0; j < AVF_NKERNELS; j++)
+    if (kernel_ms) kernel_ms[j] = nsteps ? sum[j] / (double)nsteps : 0.0;
+  if (total_ms) *total_ms = tot;
+  return rc;
 }
 
 int avf_sync(avf_ctx *c) {
@@ -639,6 +717,8 @@ int avf_check_cells(avf_ctx *c, int32_t inst, int64_t *bad) {
 int avf_get_contacts(avf_ctx *c, int32_t inst, avf_contact *out, int64_t capn, int64_t *count) {
   int rc = check_inst(c, inst);
   if (rc) return rc;
+  if (!c->binned)
+    return fail(c, AVF_ERR_ARG, "no force stage since the last upload (call avf_initial_forces)");
   int64_t room = std::max<int64_t>(capn, 0);
   int64_t *d_out = nullptr;
   if ((rc = dalloc(c, &d_out, 4 * std::max<int64_t>(room, 1)))) return rc;

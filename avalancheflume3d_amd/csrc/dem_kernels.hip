@@ -286,8 +286,8 @@ __global__ void k_scatter(const uint32_t *__restrict__ skey, const uint32_t *__r
                           InstParams *__restrict__ P, int32_t ninst, int64_t nsph,
                           int advance) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (advance && blockIdx.x == 0 && threadIdx.x < ninst) {
-    InstParams &p = P[threadIdx.x];
+  if (advance && i < ninst) {  // ninst <= clumps < spheres, so every instance is covered
+    InstParams &p = P[i];
     p.time = p.time + p.dt;  // avalanche_hst.cu:672 (sys_par.time += sys_par.dt)
   }
   if (i >= nsph) return;
@@ -853,7 +853,8 @@ __global__ void k_sgrid_finalize(InstParams *__restrict__ P, long long *__restri
                                  int64_t *__restrict__ ncells_out) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   double scale = 1.0;
-  for (int attempt = 0; attempt < 64; attempt++) {
+  bool fit = false;
+  for (int attempt = 0; attempt < 64 && !fit; attempt++) {
     int64_t total = 0;
     for (int i = 0; i < ninst; i++) {
       InstParams &p = P[i];
@@ -882,9 +883,17 @@ __global__ void k_sgrid_finalize(InstParams *__restrict__ P, long long *__restri
     }
     if (total <= cell_cap) {
       *ncells_out = total;
-      break;
+      fit = true;
+    } else {
+      scale *= cbrt((double)total / (double)cell_cap) * 1.05;
     }
-    scale *= cbrt((double)total / (double)cell_cap) * 1.05;
+  }
+  if (!fit) {  // one cell per instance always fits (ninst <= clumps <= cell_cap)
+    for (int i = 0; i < ninst; i++) {
+      P[i].sg_n[0] = P[i].sg_n[1] = P[i].sg_n[2] = 1;
+      P[i].sg_base = i;
+    }
+    *ncells_out = ninst;
   }
   for (int i = 0; i < ninst; i++) {
     long long *a = acc + 8 * i;

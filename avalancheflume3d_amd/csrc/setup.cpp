@@ -341,8 +341,9 @@ void avf_sphere_centres(avf_particle *pp, int64_t n, const avf_geometry *geo) {
 int avf_place_lattice(avf_particle *pp, int64_t n, const avf_geometry *geo, avf_double3 lo,
                       avf_double3 hi, double a, double jit, int64_t seed) {
   if (n < 0 || a <= 0.0) return AVF_ERR_ARG;
-  int64_t nx = (int64_t)((hi.x - lo.x) / a), ny = (int64_t)((hi.y - lo.y) / a);
-  int64_t nz = (int64_t)((hi.z - lo.z) / a);
+  // counts per axis; the 1e-9 guards box lengths that are exact multiples of a
+  int64_t nx = (int64_t)((hi.x - lo.x) / a + 1e-9), ny = (int64_t)((hi.y - lo.y) / a + 1e-9);
+  int64_t nz = (int64_t)((hi.z - lo.z) / a + 1e-9);
   if (nx < 1 || ny < 1 || nz < 1 || nx * ny * nz < n) return AVF_ERR_NOFIT;
   avf_rng st;
   avf_rng_init(&st);

@@ -220,6 +220,13 @@ class Context:
     def step(self, n: int = 1) -> None:
         self._ck(lib().avf_step(self.h, C.c_int64(n)))
 
+    def step_profiled(self, n: int) -> tuple[dict, float]:
+        """n eager steps with HIP events between kernels: ({kernel: mean ms}, total ms)."""
+        ks = np.zeros(len(abi.KERNELS), dtype=np.float64)
+        tot = C.c_double()
+        self._ck(lib().avf_step_profiled(self.h, C.c_int64(n), ptr(ks), C.byref(tot)))
+        return dict(zip(abi.KERNELS, ks.tolist())), tot.value
+
     def sync(self) -> None:
         self._ck(lib().avf_sync(self.h))
 

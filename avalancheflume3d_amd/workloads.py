@@ -1,0 +1,168 @@
+"""Synthetic workloads of the benchmark configurations, in the reference's cgs units.
+
+Clump: 4 equal spheres of radius r on a regular tetrahedron with centres d = 0.75 r from the
+centroid (sphere_div = sqrt(3) r / d); density 2.5 g/cm^3.  Contact stiffness k_n is given
+instead of the contact time: the reference's normal spring is km = 0.5 kappa m_eff
+(force_part.cu:121), so kappa = 2 k_n / m_s and t_coll = sqrt((pi^2 + ln^2 eps) / kappa);
+dt = t_coll / 50.
+
+    small (S)     2,000 clumps, 4x4x8 cm release box on a 35 deg flume 4 cm wide, 30 cm long
+    headline (H)  250,000 clumps, funnel release into a 35 deg flume 20 cm wide, 2 m long,
+                  flat deposition zone
+    large (L)     2,000,000 clumps, r ~ U[0.09, 0.11] cm per clump, hopper discharge onto an
+                  incline, mu 0.6, eps 0.4
+    batch         64 instances, N ~ log-uniform[5e3, 6e4], mu, eps ~ U[0.3, 0.7],
+                  slope ~ U[28, 40] deg
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi, dem
+
+R_SPH = 0.1        # cm
+D_OFF = 0.075      # cm, centre offset
+RHO = 2.5          # g/cm^3
+K_N = 1.0e7        # dyn/cm (1e4 N/m)
+
+
+def sphere_div(r: float = R_SPH, d: float = D_OFF) -> float:
+    return math.sqrt(3.0) * r / d
+
+
+def t_coll_for(k_n: float, eps: float, m_s: float) -> float:
+    kappa = 2.0 * k_n / m_s
+    return math.sqrt((math.pi ** 2 + math.log(eps) ** 2) / kappa)
+
+
+def sphere_mass(r: float, rho: float = RHO) -> float:
+    return abi.SPH_FACT * rho * r * r * r  # force_part.cu:58 form
+
+
+@dataclass
+class FlumeSpec:
+    n: int
+    tilt_deg: float = 35.0
+    width: float = 4.0            # flume width side.y
+    length: float = 30.0          # Lflume
+    mu: float = 0.5
+    eps: float = 0.5
+    k_n: float = K_N
+    rmin: float = R_SPH
+    rmax: float = R_SPH
+    box: tuple = (4.0, 4.0, 8.0)  # release box (x, y, z) lengths
+    funnel_width: float | None = None  # YFUNNEL; None: no funnel
+    funnel_length: float = 0.0         # LFUNNEL
+    deposit_length: float = 0.0        # flat zone past the flume end (0: none)
+    spacing: float = 0.4
+    jitter: float = 0.005
+    seed: int = 1
+    dt_frac: float = 50.0
+
+
+def build(spec: FlumeSpec) -> dem.Instance:
+    """Config-mode instance: avalanche phase from t = 0 (gate open), zero velocity."""
+    geo = dem.reference_geometry()
+    geo.sphere_div = sphere_div(spec.rmax, 0.75 * spec.rmax)
+    geo.clump_mode = 1
+    bx, by, bz = spec.box
+    fw = spec.funnel_width if spec.funnel_width is not None else spec.width
+    geo.yfunnel = fw
+    geo.lfunnel = spec.funnel_length
+    geo.hop_width = bx
+    geo.yhop_l = 0.5 * (fw - by)
+    geo.yhop_r = geo.yhop_l + by
+    geo.hop_height = 0.0
+    mods = abi.MOD_HOP | abi.MOD_WALLS | abi.MOD_PP | abi.MOD_FLUME
+    if spec.funnel_width is not None and spec.funnel_length > 0:
+        mods |= abi.MOD_FUNNEL | abi.MOD_FCORNERS
+    tilt = math.radians(spec.tilt_deg)
+    x_end = (spec.funnel_length + spec.length) * math.cos(tilt)
+    if spec.deposit_length > 0:
+        mods |= abi.MOD_DEP
+        geo.dep_sin, geo.dep_cos = 0.0, 1.0
+        geo.dep_x0, geo.dep_x1 = 0.0, 0.0
+        geo.dep_z0 = -(spec.funnel_length + spec.length) * math.sin(tilt)
+    geo.modules_fill = mods
+    geo.modules_avalanche = mods
+
+    ms = sphere_mass(spec.rmin)
+    t_c = t_coll_for(spec.k_n, spec.eps, ms)
+    sys = abi.System()
+    sys.npart = spec.n
+    sys.side.y = spec.width
+    sys.side.z = bz
+    sys.rmin, sys.rmax = spec.rmin, spec.rmax
+    sys.v0 = 0.0
+    sys.dt = t_c / spec.dt_frac
+    sys.eps_pw = sys.eps_pp = spec.eps
+    sys.mu_pw = sys.mu_pp = spec.mu
+    sys.t_coll = t_c
+    sys.Lflume = spec.length
+    sys.tilt = spec.tilt_deg
+    fun = dem.derive_system(sys, geo)
+    # gate open: avalanche phase from t = 0, clumps past the deposition zone removed
+    sys.time = 0.0
+    sys.side.x = x_end + spec.deposit_length * (1.0 if spec.deposit_length > 0 else 0.0)
+
+    pp = dem.new_particles(spec.n)
+    pp["density"] = RHO
+    if spec.rmax > spec.rmin:
+        rng = np.random.default_rng(spec.seed)
+        pp["radius"] = rng.uniform(spec.rmin, spec.rmax, spec.n)
+    else:
+        pp["radius"] = spec.rmin
+    # per-clump offset 0.75 r: sphere_div is the same for every radius
+    dem.calculate_particle_parameters(pp, sys, geo)
+    lo = (0.0, geo.yhop_l, geo.hop_height)
+    hi = (bx, geo.yhop_r, geo.hop_height + bz)
+    dem.place_lattice(pp, geo, lo, hi, spec.spacing, spec.jitter, spec.seed)
+    sys.tot_mass = float(pp["mass"].sum())
+    return dem.Instance(sys=sys, geo=geo, fun=fun, particles=pp, ntags=9,
+                        phase=abi.PHASE_AVALANCHE)
+
+
+def small(seed: int = 1) -> dem.Instance:
+    """S: 2,000 clumps, 10x10x20 lattice at 0.4 cm in the 4x4x8 cm box."""
+    return build(FlumeSpec(n=2000, seed=seed))
+
+
+def headline(seed: int = 1, n: int = 250_000) -> dem.Instance:
+    """H: 250,000 clumps in a 20 x 20 cm funnel-fed release box (100 layers at 0.4 cm,
+    jitter +-5 % of the spacing), 35 deg flume 20 cm wide and 2 m long after a 40 cm
+    funnel narrowing 40 -> 20 cm, flat 50 cm deposition zone."""
+    layers = math.ceil(n / 2500)
+    return build(FlumeSpec(n=n, width=20.0, length=200.0, box=(20.0, 20.0, 0.4 * layers + 0.01),
+                           funnel_width=40.0, funnel_length=40.0, deposit_length=50.0,
+                           spacing=0.4, jitter=0.02, seed=seed))
+
+
+def large(seed: int = 1, n: int = 2_000_000) -> dem.Instance:
+    """L: 2,000,000 polydisperse clumps (r ~ U[0.09, 0.11]) in a 40 x 40 cm hopper above a
+    wide incline; mu 0.6, eps 0.4; dt from the smallest pair."""
+    a = 0.44
+    per_layer = int(40.0 / a) ** 2
+    layers = math.ceil(n / per_layer)
+    return build(FlumeSpec(n=n, width=40.0, length=300.0, box=(40.0, 40.0, a * layers + 0.01),
+                           mu=0.6, eps=0.4, rmin=0.09, rmax=0.11, spacing=a, jitter=0.02,
+                           seed=seed))
+
+
+def batch(count: int = 64, seed: int = 7, scale: float = 1.0) -> list[dem.Instance]:
+    """Batched ensemble: per-instance sizes, seeds, friction, restitution and slope."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(5e3), math.log(6e4))) * scale))
+        n = max(n, 8)
+        mu, eps = rng.uniform(0.3, 0.7), rng.uniform(0.3, 0.7)
+        tilt = rng.uniform(28.0, 40.0)
+        side = 4.0 * max(1.0, math.sqrt(n / 2000.0))  # keep ~20 layers in a square box
+        per = int(side / 0.4) ** 2
+        layers = math.ceil(n / per)
+        out.append(build(FlumeSpec(n=n, tilt_deg=tilt, width=side, length=30.0, mu=mu, eps=eps,
+                                   box=(side, side, 0.4 * layers + 0.01), seed=seed * 1000 + i)))
+    return out

@@ -261,6 +261,16 @@ int avf_check_cells(avf_ctx *ctx, int32_t inst, int64_t *bad);
 int avf_get_contacts(avf_ctx *ctx, int32_t inst, avf_contact *out, int64_t cap,
                      int64_t *count);
 int avf_get_diag(avf_ctx *ctx, avf_diag *diag);
+
+/* Kernels of one step, in launch order (indices of avf_step_profiled's kernel_ms). */
+enum {
+  AVF_K_PREDICT, AVF_K_SCAN_A, AVF_K_SCAN_B, AVF_K_SCAN_C, AVF_K_SCATTER, AVF_K_FORCE,
+  AVF_K_CELLREDUCE, AVF_K_FINALIZE, AVF_NKERNELS
+};
+/* Like avf_step, but launches eagerly with HIP events between consecutive kernels on the
+ * context's stream: kernel_ms[k] receives the mean duration of kernel k per step and
+ * *total_ms the elapsed time from the first to the last kernel of the run. */
+int avf_step_profiled(avf_ctx *ctx, int64_t nsteps, double *kernel_ms, double *total_ms);
 /* Waits for all work queued on the context's stream. */
 int avf_sync(avf_ctx *ctx);
 

@@ -1,0 +1,172 @@
+"""GPU parity: the gfx950 path (through the C ABI) against the CPU oracle on seeded states.
+
+Bars: contact sets and clump-cell assignment bit-exact; per-step F/T
+within 1e-12 relative; 100-step trajectories within 1e-9 relative.  The kernels keep the
+reference's operation order, so the tests also report whether results are bit-identical.
+"""
+import numpy as np
+import pytest
+
+from avalancheflume3d_amd import abi, dem
+from oracle import oracle
+from tests import scenarios
+
+pytestmark = pytest.mark.gpu
+
+F_RTOL = 1e-12   # per-step force / torque
+T_RTOL = 1e-9    # trajectories over 100 steps
+STATE = ["rr", "vv", "ww", "Rmatx", "Rmaty", "Rmatz", "rr_a", "rr_b", "rr_c", "rr_d", "radius"]
+
+
+def opts(graph=1, reorder=50, factor=1.0):
+    o = dem.default_options()
+    o.use_graph = graph
+    o.reorder_interval = reorder
+    o.sphere_cell_factor = factor
+    return o
+
+
+def gpu_ctx(insts, **kw):
+    ctx = dem.Context(insts if isinstance(insts, list) else [insts], opts(**kw))
+    ctx.regrid()  # the initial clump-cell grid, avalanche_hst.cu:311-333
+    return ctx
+
+
+def oracle_sim(inst):
+    return oracle.OracleSim(inst.sys, inst.geo, inst.fun, inst.ntags, inst.phase, inst.particles)
+
+
+def vec_close(a, b, rtol, what):
+    """|a - b| <= rtol * |b| row-wise (vector norms), NaN-aware."""
+    a = np.asarray(a, np.float64).reshape(len(a), -1)
+    b = np.asarray(b, np.float64).reshape(len(b), -1)
+    nan_ok = np.isnan(a).all(1) & np.isnan(b).all(1)
+    err = np.linalg.norm(a - b, axis=1)
+    scale = np.maximum(np.linalg.norm(b, axis=1), 1e-300)
+    bad = ~nan_ok & ~(err <= rtol * scale)
+    assert not bad.any(), f"{what}: {bad.sum()} rows off, worst rel {np.nanmax(err / scale):.3e}"
+    return bool((a.view(np.uint64) == b.view(np.uint64)).all())
+
+
+def contact_keys(c):
+    return set(zip(c["l"].tolist(), c["m"].tolist(), c["sl"].tolist(), c["sm"].tolist(),
+                   c["repulsive"].tolist()))
+
+
+@pytest.mark.parametrize("kind", ["hopper", "avalanche"])
+def test_contact_set_and_cells_bit_exact(gpu, kind):
+    inst = scenarios.hopper_state(400, 11) if kind == "hopper" else scenarios.avalanche_state(16, 7)
+    ctx = gpu_ctx(inst)
+    ctx.initial_forces()
+    got = ctx.contacts()
+    o = oracle_sim(inst)
+    o.record(200000)
+    o.forces()
+    ref = o.recorded()
+    assert len(ref) > 20
+    assert contact_keys(got) == contact_keys(ref)
+    brute = oracle.brute_contacts(inst.particles, inst.sys)
+    assert set(zip(brute["l"], brute["m"], brute["sl"], brute["sm"])) == \
+        set(zip(ref["l"], ref["m"], ref["sl"], ref["sm"]))
+    # clump-cell assignment: the cell_locate index (cells.cu:69-91) of every clump
+    cell = o.get()["cell"]
+    pp = inst.particles
+    ii = np.trunc(cell.cs_inv.x * (pp["rr"][:, 0] - cell.rr_min.x)).astype(np.int64)
+    jj = np.trunc(cell.cs_inv.y * (pp["rr"][:, 1] - cell.rr_min.y)).astype(np.int64)
+    kk = np.trunc(cell.cs_inv.z * (pp["rr"][:, 2] - cell.rr_min.z)).astype(np.int64)
+    ii[ii == cell.ncell.x] -= 1
+    jj[jj == cell.ncell.y] -= 1
+    kk[kk == cell.ncell.z] -= 1
+    want = ii + cell.ncell.x * (jj + kk * cell.ncell.y)
+    frozen = (inst.sys.time > 0) & (pp["rr"][:, 0] > inst.sys.side.x)
+    want[frozen] = -1
+    assert np.array_equal(ctx.clump_cells(), want)
+    ctx.close()
+
+
+@pytest.mark.parametrize("kind", ["hopper", "avalanche"])
+def test_initial_forces(gpu, kind):
+    inst = scenarios.hopper_state(400, 12) if kind == "hopper" else scenarios.avalanche_state(16, 8)
+    ctx = gpu_ctx(inst)
+    ctx.initial_forces()
+    f, t = ctx.forces()
+    _, aa, anga = ctx.download()
+    o = oracle_sim(inst)
+    o.initial()
+    r = o.get()
+    assert np.abs(r["ff"]).max() > 0
+    exact = [vec_close(f, r["ff"], F_RTOL, "force"), vec_close(t, r["tq"], F_RTOL, "torque")]
+    live = ~((inst.sys.time > 0) & (inst.particles["rr"][:, 0] > inst.sys.side.x))
+    exact.append(vec_close(aa[live], r["aa_i"][live], F_RTOL, "accel"))
+    exact.append(vec_close(anga[live], r["anga_i"][live], F_RTOL, "ang accel"))
+    print(f"bit-exact initial forces ({kind}):", all(exact))
+    ctx.close()
+
+
+def run_pair(inst, steps, chunk, **kw):
+    ctx = gpu_ctx(inst, **kw)
+    ctx.initial_forces()
+    o = oracle_sim(inst)
+    o.initial()
+    done, exact = 0, True
+    while done < steps:
+        k = min(chunk, steps - done)
+        ctx.step(k)
+        o.step(k)
+        done += k
+        pp, aa, anga = ctx.download()
+        f, t = ctx.forces()
+        r = o.get()
+        for fld in STATE:
+            exact &= vec_close(pp[fld], r["pp"][fld], T_RTOL, f"{fld} after {done} steps")
+        exact &= vec_close(f, r["ff"], T_RTOL, f"force after {done} steps")
+        tg, cc = ctx.time()
+        assert tg == r["sys"].time
+    ctx.close()
+    return exact
+
+
+def test_trajectory_fill_100_steps(gpu):
+    exact = run_pair(scenarios.hopper_state(500, 21), 100, 25)
+    print("bit-exact 100-step fill trajectory:", exact)
+
+
+def test_trajectory_avalanche_100_steps(gpu):
+    exact = run_pair(scenarios.avalanche_state(16, 22), 100, 50)
+    print("bit-exact 100-step avalanche trajectory:", exact)
+
+
+def test_graph_eager_and_reorder_invariance(gpu):
+    inst = scenarios.hopper_state(300, 31)
+    outs = []
+    for graph, reorder, factor in ((1, 50, 1.0), (0, 0, 1.0), (1, 1, 1.0), (1, 7, 2.5)):
+        ctx = gpu_ctx(inst, graph=graph, reorder=reorder, factor=factor)
+        ctx.initial_forces()
+        ctx.step(20)
+        pp, aa, anga = ctx.download()
+        outs.append((pp, aa, ctx.forces()[0]))
+        ctx.close()
+    for pp, aa, f in outs[1:]:
+        for fld in STATE:
+            assert np.array_equal(pp[fld].view(np.uint64), outs[0][0][fld].view(np.uint64)), fld
+        assert np.array_equal(aa.view(np.uint64), outs[0][1].view(np.uint64))
+        assert np.array_equal(f.view(np.uint64), outs[0][2].view(np.uint64))
+
+
+def test_batch_equals_separate_runs(gpu):
+    insts = [scenarios.hopper_state(120, 41), scenarios.avalanche_state(10, 42),
+             scenarios.hopper_state(333, 43)]
+    insts[2].sys.mu_pp = 0.3
+    insts[2].sys.fr_fact_pp = 0.3 / 1.0e12
+    ctx = gpu_ctx(insts)
+    ctx.initial_forces()
+    ctx.step(30)
+    for k, inst in enumerate(insts):
+        one = gpu_ctx(inst)
+        one.initial_forces()
+        one.step(30)
+        a, b = ctx.download(k), one.download(0)
+        for fld in STATE:
+            assert np.array_equal(a[0][fld].view(np.uint64), b[0][fld].view(np.uint64)), (k, fld)
+        one.close()
+    ctx.close()
