@@ -100,6 +100,17 @@ class Diag(C.Structure):
         "steps")]
 
 
+class Observables(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "vmean", "wmean", "xcm", "zcm", "runout_mean", "runout_front", "mass_live")] + \
+        [("live", C.c_int64)]
+
+
+class ProfileSpec(C.Structure):
+    _fields_ = [("x0", C.c_double), ("x1", C.c_double), ("width", C.c_double),
+                ("sigma", C.c_double), ("nbins", C.c_int32), ("reserved", C.c_int32)]
+
+
 class Rng(C.Structure):
     _fields_ = [("useed", C.c_uint64 * 250), ("uuy", C.c_uint64), ("np", C.c_int32),
                 ("nq", C.c_int32), ("npp", C.c_int32), ("not_init", C.c_int32)]
@@ -185,6 +196,7 @@ def load_library() -> C.CDLL:
             "avf_get_diag": (C.c_int, [vp, vp]),
             "avf_sync": (C.c_int, [vp]),
             "avf_step_profiled": (C.c_int, [vp, i64, vp, vp]),
+            "avf_observe": (C.c_int, [vp, i32, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -203,6 +215,6 @@ EXPORTED = [
     "avf_last_error", "avf_upload", "avf_download", "avf_set_phase", "avf_get_time",
     "avf_regrid", "avf_get_cell", "avf_initial_forces", "avf_step", "avf_get_forces",
     "avf_get_clump_cells", "avf_check_cells", "avf_get_contacts", "avf_get_diag", "avf_sync",
-    "avf_step_profiled",
+    "avf_step_profiled", "avf_observe",
 ]
 KERNELS = ["predict", "scan_a", "scan_b", "scan_c", "scatter", "force", "cellreduce", "finalize"]

@@ -758,6 +758,63 @@ int avf_get_contacts(avf_ctx *c, int32_t inst, avf_contact *out, int64_t capn, i
   return AVF_OK;
 }
 
+int avf_observe(avf_ctx *c, int32_t inst, avf_observables *obs, const avf_profile_spec *spec,
+                double *profile) {
+  int rc = check_inst(c, inst);
+  if (rc) return rc;
+  const int64_t cnt = c->desc[inst].sys.npart;
+  int nbins = 0;
+  double x0 = 0.0, bw = 1.0, sigma = 1.0, width = 1.0;
+  if (spec && profile) {
+    if (spec->nbins <= 0 || spec->nbins > kMaxBins || !(spec->x1 > spec->x0) ||
+        !(spec->width > 0.0))
+      return fail(c, AVF_ERR_ARG, "bad deposit profile spec (nbins 1..%d, x1 > x0, width > 0)",
+                  kMaxBins);
+    nbins = spec->nbins;
+    x0 = spec->x0;
+    bw = (spec->x1 - spec->x0) / nbins;
+    sigma = spec->sigma > 0.0 ? spec->sigma : bw;
+    width = spec->width;
+  }
+  const int64_t nb = (cnt + kObsBlock - 1) / kObsBlock;
+  const int64_t w = kObsScalars + nbins;
+  double *d_part = nullptr, *d_out = nullptr;
+  if ((rc = dalloc(c, &d_part, nb * w)) || (rc = dalloc(c, &d_out, w))) {
+    if (d_part) (void)hipFree(d_part);
+    return rc;
+  }
+  k_observe<<<(unsigned)nb, kObsBlock, 0, c->stream>>>(c->d_st, c->d_sta, c->d_inv, c->d_P, inst,
+                                                      cnt, c->cap, nbins, x0, bw, sigma, width,
+                                                      d_part);
+  k_observe_final<<<1, 1, 0, c->stream>>>(d_part, nb, nbins, d_out);
+  std::vector<double> h((size_t)w);
+  hipError_t he = hipGetLastError();
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(h.data(), d_out, w * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+  (void)hipFree(d_part);
+  (void)hipFree(d_out);
+  if (he != hipSuccess) return fail(c, AVF_ERR_HIP, "observe: %s", hipGetErrorString(he));
+  if (obs) {
+    obs->vmean = h[0] / (double)cnt;  // avalanche_hst.cu:573-574 (vmean /= nn)
+    obs->wmean = h[1] / (double)cnt;
+    obs->xcm = h[2];
+    obs->zcm = h[3];
+    // xcm /= tot_mass (avalanche_hst.cu:827-828); tot_mass from the instance description or,
+    // when it was not given, the sum of the clump masses
+    double tot = c->desc[inst].sys.tot_mass;
+    if (!(tot > 0.0)) tot = h[8];
+    obs->xcm /= tot;
+    obs->zcm /= tot;
+    obs->runout_mean = h[6] > 0.0 ? h[4] / h[6] : 0.0;
+    obs->runout_front = h[5];
+    obs->mass_live = h[6];
+    obs->live = (int64_t)h[7];
+  }
+  for (int b = 0; b < nbins; b++) profile[b] = h[kObsScalars + b];
+  return AVF_OK;
+}
+
 int avf_get_diag(avf_ctx *c, avf_diag *d) {
   if (!c || !d) return AVF_ERR_ARG;
   DevDiag h;

@@ -41,6 +41,17 @@ __device__ __forceinline__ void sphere_centres(d3 rr, const double *R, double po
                  rr.z + pos * (-R[6] + R[7] - R[8]));
 }
 
+// Centre of sphere i (0..3 = a..d) alone.  The sign factors are exact, and x + (-y) rounds
+// as x - y, so this is bit-identical to the matching row of sphere_centres.
+__device__ __forceinline__ d3 sphere_of(d3 rr, const double *R, double pos, int i) {
+  const double sx = i < 2 ? 1.0 : -1.0;
+  const double sy = (i == 0 || i == 3) ? 1.0 : -1.0;
+  const double sz = (i == 0 || i == 2) ? 1.0 : -1.0;
+  return make_d3(rr.x + pos * (sx * R[0] + sy * R[1] + sz * R[2]),
+                 rr.y + pos * (sx * R[3] + sy * R[4] + sz * R[5]),
+                 rr.z + pos * (sx * R[6] + sy * R[7] + sz * R[8]));
+}
+
 __device__ __forceinline__ d3 pick(const d3 *s, int i) {
   d3 r = s[0];
   if (i == 1) r = s[1];
@@ -304,13 +315,13 @@ __global__ void k_scatter(const uint32_t *__restrict__ skey, const uint32_t *__r
 // ------------------------------------------------------------------ boundaries -----
 
 // force_hop.cu:9-244 for one nw (0: x walls, 1: y walls, 2: bottom gate while time < 0).
-__device__ void hop_walls(const InstParams &p, int nw, double time, d3 rr, d3 vv, d3 ww,
+__device__ __forceinline__ void hop_walls(const InstParams &p, int nw, double time, d3 rr, d3 vv, d3 ww,
                           const d3 *sph, double ri, double mass, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
   const double xfront = p.hop_width, xrear = 0.0, yleft = p.yhop_l, yright = p.yhop_r;
   const double zbottom = p.hop_height;
   for (int ns = 0; ns < 4; ns++) {
-    d3 r = sph[ns], rrc = make_d3(0, 0, 0), nvec = make_d3(0, 0, 0);
+    d3 r = pick(sph, ns), rrc = make_d3(0, 0, 0), nvec = make_d3(0, 0, 0);
     double dist, comp = 0.0;
     int touch = 0;
     if (r.z < zbottom || r.x > p.hop_width) continue;
@@ -360,12 +371,12 @@ __device__ void hop_walls(const InstParams &p, int nw, double time, d3 rr, d3 vv
 }
 
 // force_funnel.cu:9-213: converging side planes of the funnel (slope frame).
-__device__ void funnel_walls(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph, double ri,
+__device__ __forceinline__ void funnel_walls(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph, double ri,
                              double mass, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
   const double sinfi = p.sinfi, cosfi = p.cosfi, tanfi = p.tanfi, cost = p.cost, sint = p.sint;
   for (int ns = 0; ns < 4; ns++) {
-    d3 q = to_slope(sph[ns], cost, sint), rrc = make_d3(0, 0, 0), nvec = make_d3(0, 0, 0);
+    d3 q = to_slope(pick(sph, ns), cost, sint), rrc = make_d3(0, 0, 0), nvec = make_d3(0, 0, 0);
     double dist, comp = 0.0;
     int touch = 0;
     if (q.x > p.lfunnel - ri * sinfi) continue;
@@ -392,13 +403,13 @@ __device__ void funnel_walls(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 
 }
 
 // force_funnel.cu:216-404: vertical corner edges at the funnel exit.
-__device__ void funnel_corners(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph,
+__device__ __forceinline__ void funnel_corners(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph,
                                double ri, double mass, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
   const double yfun_l = p.dy2, yfun_r = p.side_y + p.dy2, hsy = p.dy2 + p.side_y / 2.0;
   const double lf = p.lfunnel, cost = p.cost, sint = p.sint;
   for (int ns = 0; ns < 4; ns++) {
-    d3 q = to_slope(sph[ns], cost, sint);
+    d3 q = to_slope(pick(sph, ns), cost, sint);
     if (q.x < lf - ri * p.sinfi || q.x > lf + ri) continue;
     if (q.y > yfun_r || q.y < yfun_l) continue;
     double yc = (q.y <= hsy) ? yfun_l : yfun_r;
@@ -419,13 +430,13 @@ __device__ void funnel_corners(const InstParams &p, d3 rr, d3 vv, d3 ww, const d
 }
 
 // force_walls.cu:9-199: flume side walls beyond the funnel (caller skips frozen clumps).
-__device__ void side_walls(const InstParams &p, double time, d3 rr, d3 vv, d3 ww, const d3 *sph,
+__device__ __forceinline__ void side_walls(const InstParams &p, double time, d3 rr, d3 vv, d3 ww, const d3 *sph,
                            double ri, double mass, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass;
   double gm = 0.5 * p.gamma_pw * mass, mu = p.mu_pw, cns_1 = p.fr_pw;
   const double yleft = p.dy2, yright = p.side_y + p.dy2;
   for (int ns = 0; ns < 4; ns++) {
-    d3 r = sph[ns], q = to_slope(r, p.cost, p.sint);
+    d3 r = pick(sph, ns), q = to_slope(r, p.cost, p.sint);
     d3 rrc = make_d3(0, 0, 0), nvec = make_d3(0, 0, 0);
     double dist, comp = 0.0;
     int touch = 0;
@@ -451,12 +462,12 @@ __device__ void side_walls(const InstParams &p, double time, d3 rr, d3 vv, d3 ww
 }
 
 // force_flume.cu:10-182: inclined floor z = -x tan(theta).
-__device__ void flume_floor(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph, double ri,
+__device__ __forceinline__ void flume_floor(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph, double ri,
                             double mass, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
   const double sint = p.sint, cost = p.cost, tant = sint / cost;
   for (int ns = 0; ns < 4; ns++) {
-    d3 r = sph[ns];
+    d3 r = pick(sph, ns);
     double rect = -tant * r.x - r.z, dist = fabs(cost * rect);
     if (!(dist <= ri)) continue;
     d3 nvec = make_d3(-sint, 0.0, -cost);
@@ -467,12 +478,12 @@ __device__ void flume_floor(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *
 }
 
 // force_dep.cu:10-178 with the deposit plane of avf_geometry.
-__device__ void deposit_floor(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph, double ri,
+__device__ __forceinline__ void deposit_floor(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph, double ri,
                               double mass, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
   const double sind = p.dep_sin, cosd = p.dep_cos, tand = sind / cosd;
   for (int ns = 0; ns < 4; ns++) {
-    d3 r = sph[ns];
+    d3 r = pick(sph, ns);
     double rect = tand * (r.x - p.dep_x0 - p.dep_x1) - (r.z - p.dep_z0), dist = cosd * (-rect);
     if (!(dist <= ri)) continue;
     d3 nvec = make_d3(sind, 0.0, -cosd);
@@ -535,8 +546,9 @@ __global__ __launch_bounds__(BS) void k_force(
   for (int k = 0; k < 9; k++) R[k] = st[(S_R00 + k) * cap + c];
   const double rad = sta[C_RAD * cap + c], rad0 = sta[C_RAD0 * cap + c];
   const double rho = sta[C_RHO * cap + c];
+  const double pos0 = rad0 / p.sphere_div;
   d3 sph[4];
-  sphere_centres(rr, R, rad0 / p.sphere_div, sph);
+  sphere_centres(rr, R, pos0, sph);
 
   const bool frz = frozen(time, rr.x, p.side_x);
   const int mods = p.phase ? p.modules_aval : p.modules_fill;
@@ -566,7 +578,7 @@ __global__ __launch_bounds__(BS) void k_force(
     const int64_t nxy = p.ncell[0] * p.ncell[1];
 
     for (int s = 0; s < 4; s++) {
-      const d3 q = sph[s];
+      const d3 q = sphere_of(rr, R, pos0, s);
       const int ix = grid_index(q.x, p.sg_lo[0], p.sg_inv_h, nx);
       const int iy = grid_index(q.y, p.sg_lo[1], p.sg_inv_h, ny);
       const int iz = grid_index(q.z, p.sg_lo[2], p.sg_inv_h, nz);
@@ -657,7 +669,7 @@ __global__ __launch_bounds__(BS) void k_force(
       const int e = __builtin_ctz(cur);
       cur &= cur - 1u;
       const int sdiv = pr >> 2, smod = pr & 3;
-      const d3 r1 = pick(sph, smod);
+      const d3 r1 = sphere_of(rr, R, pos0, smod);
       {
         const int64_t cm = sh.m[e][lane];
         const double *g = stage + cm * kStageDoubles;
@@ -1102,6 +1114,115 @@ __global__ void k_export_contacts(const double *__restrict__ st, const double *_
         }
       }
     }
+  }
+}
+
+// ------------------------------------------------------------------- observables ----
+
+constexpr int kObsBlock = 256;
+constexpr int kObsScalars = 9;  // |v|, |w|, m x, m z, m x', max x', m_live, live, m
+constexpr int kMaxBins = 1024;
+
+__device__ __forceinline__ double tree_sum(double *red, double v) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kObsBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double tree_max(double *red, double v) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kObsBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double gauss(double u, double sigma) {
+  const double k = 0.3989422804014327;  // 1/sqrt(2 pi)
+  return k / sigma * exp(-0.5 * (u / sigma) * (u / sigma));
+}
+
+// Per-block partial sums over reference indices [b*256, (b+1)*256) of one instance in a
+// fixed tree order, so the result does not depend on the storage order.  partial holds
+// kObsScalars + nbins doubles per block.
+__global__ __launch_bounds__(kObsBlock) void k_observe(
+    const double *__restrict__ st, const double *__restrict__ sta, const int64_t *__restrict__ inv,
+    const InstParams *__restrict__ P, int32_t inst, int64_t count, int64_t cap, int nbins,
+    double x0, double bw, double sigma, double width, double *__restrict__ partial) {
+  __shared__ double red[kObsBlock];
+  const InstParams &p = P[inst];
+  const int64_t o = (int64_t)blockIdx.x * kObsBlock + threadIdx.x;
+  double sv = 0.0, sw = 0.0, mx = 0.0, mz = 0.0, mxs = 0.0, xmax = -1.0e300, ml = 0.0, nl = 0.0;
+  double mall = 0.0;
+  double xs = 0.0, vol = 0.0;
+  bool live = false;
+  if (o < count) {
+    const int64_t c = inv[p.begin + o];
+    const double x = st[S_RX * cap + c], z = st[S_RZ * cap + c];
+    const double vx = st[S_VX * cap + c], vy = st[S_VY * cap + c], vz = st[S_VZ * cap + c];
+    const double wx = st[S_WX * cap + c], wy = st[S_WY * cap + c], wz = st[S_WZ * cap + c];
+    const double m = sta[C_MASS * cap + c];
+    double vm2 = vx * vx;  // avalanche_hst.cu:551-558
+    vm2 += vy * vy;
+    vm2 += vz * vz;
+    sv = sqrt(vm2);
+    double wm2 = wx * wx;
+    wm2 += wy * wy;
+    wm2 += wz * wz;
+    sw = sqrt(wm2);
+    mx = m * x;  // avalanche_hst.cu:822-823
+    mz = m * z;
+    mall = m;
+    live = !frozen(p.time, x, p.side_x);
+    if (live) {
+      xs = x * p.cost - z * p.sint;
+      mxs = m * xs;
+      xmax = xs;
+      ml = m;
+      nl = 1.0;
+      vol = m / sta[C_RHO * cap + c];
+    }
+  }
+  double *out = partial + (size_t)blockIdx.x * (kObsScalars + nbins);
+  const double r0 = tree_sum(red, sv), r1 = tree_sum(red, sw), r2 = tree_sum(red, mx);
+  const double r3 = tree_sum(red, mz), r4 = tree_sum(red, mxs), r5 = tree_max(red, xmax);
+  const double r6 = tree_sum(red, ml), r7 = tree_sum(red, nl), r8 = tree_sum(red, mall);
+  if (threadIdx.x == 0) {
+    out[0] = r0; out[1] = r1; out[2] = r2; out[3] = r3;
+    out[4] = r4; out[5] = r5; out[6] = r6; out[7] = r7;
+    out[8] = r8;
+  }
+  for (int b = 0; b < nbins; b++) {
+    const double cb = x0 + (b + 0.5) * bw;
+    const double u = xs - cb;
+    const double contrib = (live && fabs(u) < 8.0 * sigma) ? vol * gauss(u, sigma) / width : 0.0;
+    const double rb = tree_sum(red, contrib);
+    if (threadIdx.x == 0) out[kObsScalars + b] = rb;
+  }
+}
+
+// Sums the block partials in block order (single thread).
+__global__ void k_observe_final(const double *__restrict__ partial, int64_t nblocks, int nbins,
+                                double *__restrict__ out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int w = kObsScalars + nbins;
+  for (int k = 0; k < w; k++) {
+    double acc = (k == 5) ? -1.0e300 : 0.0;
+    for (int64_t b = 0; b < nblocks; b++) {
+      const double v = partial[b * w + k];
+      acc = (k == 5) ? fmax(acc, v) : acc + v;
+    }
+    out[k] = acc;
   }
 }
 

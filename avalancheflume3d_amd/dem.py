@@ -257,6 +257,15 @@ class Context:
             return self.contacts(inst, cnt.value)
         return out[:cnt.value]
 
+    def observe(self, inst: int = 0, profile: abi.ProfileSpec | None = None):
+        """Reference stats (vmean, wmean, xcm, zcm), runout and the deposit profile."""
+        o = abi.Observables()
+        prof = np.zeros(profile.nbins, np.float64) if profile is not None else None
+        self._ck(lib().avf_observe(self.h, inst, ptr(o), ptr(profile) if profile else None,
+                                   ptr(prof)))
+        d = {k: getattr(o, k) for k, _ in abi.Observables._fields_}
+        return (d, prof) if profile is not None else d
+
     def diag(self) -> dict:
         d = abi.Diag()
         self._ck(lib().avf_get_diag(self.h, ptr(d)))

@@ -262,6 +262,33 @@ int avf_get_contacts(avf_ctx *ctx, int32_t inst, avf_contact *out, int64_t cap,
                      int64_t *count);
 int avf_get_diag(avf_ctx *ctx, avf_diag *diag);
 
+/* Observables of one instance, reduced on the device in reference clump order (so they do
+ * not depend on the storage order).  vmean / wmean: mean |v| and |w| over all clumps
+ * (avalanche_hst.cu:551-558, 573-574); xcm / zcm: centre of mass
+ * (avalanche_hst.cu:819-829).  Runout and the deposit profile are defined in the
+ * slope frame (x' along the slope, rotate_vectors.cu:28-30) over live clumps. */
+typedef struct avf_observables {
+  double vmean, wmean;
+  double xcm, zcm;
+  double runout_mean;   /* mass-weighted mean x' of the live clumps                        */
+  double runout_front;  /* largest x' of a live clump                                      */
+  double mass_live;     /* mass of the live clumps                                         */
+  int64_t live;         /* clumps not past side.x                                          */
+} avf_observables;
+
+/* Deposit thickness profile along the slope: bin b (centre c_b = x0 + (b + 0.5)(x1-x0)/nbins)
+ * holds sum_i V_i G(x'_i - c_b) / width with V_i the clump's solid volume (mass / density)
+ * and G a normalised Gaussian of standard deviation sigma (0: the bin width), i.e. the
+ * height of the deposited material smoothed over sigma. */
+typedef struct avf_profile_spec {
+  double x0, x1, width, sigma;
+  int32_t nbins, reserved;
+} avf_profile_spec;
+
+/* Fills *obs and, if spec and profile are given, profile[0..nbins). */
+int avf_observe(avf_ctx *ctx, int32_t inst, avf_observables *obs, const avf_profile_spec *spec,
+                double *profile);
+
 /* Kernels of one step, in launch order (indices of avf_step_profiled's kernel_ms). */
 enum {
   AVF_K_PREDICT, AVF_K_SCAN_A, AVF_K_SCAN_B, AVF_K_SCAN_C, AVF_K_SCATTER, AVF_K_FORCE,
