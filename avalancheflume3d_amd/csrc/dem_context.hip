@@ -184,10 +184,19 @@ void launch_bin(avf_ctx *c, int mode) {
       4 * c->n, mode);
 }
 
-void launch_force(avf_ctx *c, int initial) {
-  k_force<kMaxNeigh, kForceBlock><<<grid_for(c->n, kForceBlock), kForceBlock, 0, c->stream>>>(
+template <int BS>
+void launch_force_bs(avf_ctx *c, int initial) {
+  k_force<kMaxNeigh, BS><<<grid_for(c->n, BS), BS, 0, c->stream>>>(
       c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_P, c->d_starts, c->d_spos, c->d_sid, c->d_f,
       c->d_diag, c->n, c->cap, initial);
+}
+
+// k_force block shape: 64 (one wavefront) or 128 lanes (avf_options.block_clumps).
+void launch_force(avf_ctx *c, int initial) {
+  if (c->opt.block_clumps >= 128)
+    launch_force_bs<128>(c, initial);
+  else
+    launch_force_bs<kForceBlock>(c, initial);
 }
 
 // One full step: avalanche_hst.cu:669-710 (or :439-467) plus the regrid and cc++.
@@ -310,6 +319,7 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
   else
     avf_options_default(&c->opt);
   if (c->opt.sphere_cell_factor < 1.0) c->opt.sphere_cell_factor = 1.0;
+  c->opt.block_clumps = c->opt.block_clumps >= 128 ? 128 : 64;
   c->device = c->opt.device;
   c->ninst = ninst;
   c->desc.assign(inst, inst + ninst);

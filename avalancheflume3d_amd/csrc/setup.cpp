@@ -58,7 +58,7 @@ void avf_options_default(avf_options *o) {
   o->reorder_interval = 50;
   o->use_graph = 1;
   o->sphere_cell_factor = 1.0;
-  o->block_clumps = 128;
+  o->block_clumps = 64;
   o->max_neighbours = 32;
   o->device = 0;
   o->stream = nullptr;

@@ -48,6 +48,41 @@ def cpu_baseline(inst, budget_s: float) -> dict:
                       f"({dt:.1f} s on one host core)"}
 
 
+def batch_compare(steps: int, warmup: int = 20) -> dict:
+    """64 independent flumes (N ~ log-uniform[5e3, 6e4], per-instance seeds, mu, eps and
+    slope) stepped together in one context vs the same instances as sequential plain runs
+    (one context each, synchronised between instances)."""
+    from avalancheflume3d_amd import dem, workloads
+    insts = workloads.batch(64)
+    total = sum(len(i.particles) for i in insts)
+    ctx = dem.Context(insts)
+    ctx.regrid()
+    ctx.initial_forces()
+    ctx.step(warmup)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.sync()
+    t_batch = time.perf_counter() - t0
+    ctx.close()
+    t_seq = 0.0
+    for inst in insts:
+        c = dem.Context([inst])
+        c.regrid()
+        c.initial_forces()
+        c.step(warmup)
+        c.sync()
+        t0 = time.perf_counter()
+        c.step(steps)
+        c.sync()
+        t_seq += time.perf_counter() - t0
+        c.close()
+    return {"instances": len(insts), "clumps": total, "steps": steps,
+            "batched_clump_steps_per_s": total * steps / t_batch,
+            "sequential_clump_steps_per_s": total * steps / t_seq,
+            "speedup": t_seq / t_batch}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,6 +91,7 @@ def main() -> int:
     ap.add_argument("--workload", default="headline", choices=["headline", "small", "large"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding of "
@@ -107,6 +143,8 @@ def main() -> int:
         "events_total_ms": total_ms,
         "diag": diag,
     }
+    if args.batch_steps > 0:
+        out["batched"] = batch_compare(args.batch_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(inst, args.cpu_seconds)
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]

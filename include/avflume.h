@@ -136,8 +136,8 @@ typedef struct avf_options {
   int32_t reorder_interval; /* spatial reorder of clump storage every N steps (0: never) */
   int32_t use_graph;        /* capture each reorder chunk as a hipGraph and replay it    */
   double sphere_cell_factor;/* sphere-cell side = factor * 2 * max sphere radius (>= 1)  */
-  int32_t block_clumps;     /* threads per block of the clump kernels (64..1024)       */
-  int32_t max_neighbours;   /* capacity of the per-clump list of touching clumps       */
+  int32_t block_clumps;     /* lanes per block of the force kernel: 64 or 128          */
+  int32_t max_neighbours;   /* informational: touching-clump list capacity (32)        */
   int32_t device;           /* HIP device ordinal                                      */
   int32_t reserved;
   void *stream;             /* hipStream_t to run on; NULL = library-owned stream      */
