@@ -628,18 +628,36 @@ int avf_step(avf_ctx *c, int64_t nsteps) {
 
 int avf_step_profiled(avf_ctx *c, int64_t nsteps, double *kernel_ms, double *total_ms) {
   if (!c || nsteps < 0) return AVF_ERR_ARG;
-  constexpr int kBatch = 64;
-  std::vector<hipEvent_t> ev((size_t)kBatch * (AVF_NKERNELS + 1));
+  // Two batches of events: batch b+1 is queued before batch b is read, so reading the
+  // timings never leaves the GPU idle.
+  constexpr int kBatch = 128, kEv = AVF_NKERNELS + 1;
+  std::vector<hipEvent_t> ev((size_t)2 * kBatch * kEv);
   for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
   hipEvent_t first, last;
   HIPCHK(c, hipEventCreate(&first));
   HIPCHK(c, hipEventCreate(&last));
   double sum[AVF_NKERNELS] = {0};
   int rc = AVF_OK;
-  int64_t done = 0;
+  int64_t queued = 0;
+  int64_t batch_len[2] = {0, 0};
+  int cur = 0;
   bool started = false;
-  while (done < nsteps && rc == AVF_OK) {
-    int64_t k = std::min<int64_t>(kBatch, nsteps - done);
+  auto read_batch = [&](int b) -> int {
+    if (batch_len[b] == 0) return AVF_OK;
+    hipEvent_t *base = &ev[(size_t)b * kBatch * kEv];
+    HIPCHK(c, hipEventSynchronize(base[(batch_len[b] - 1) * kEv + AVF_NKERNELS]));
+    for (int64_t i = 0; i < batch_len[b]; i++)
+      for (int j = 0; j < AVF_NKERNELS; j++) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, base[i * kEv + j], base[i * kEv + j + 1]));
+        sum[j] += ms;
+      }
+    batch_len[b] = 0;
+    return AVF_OK;
+  };
+  while (queued < nsteps && rc == AVF_OK) {
+    const int64_t k = std::min<int64_t>(kBatch, nsteps - queued);
+    hipEvent_t *base = &ev[(size_t)cur * kBatch * kEv];
     for (int64_t i = 0; i < k; i++) {
       if (!c->sgrid_valid ||
           (c->opt.reorder_interval > 0 && c->steps_since_reorder >= c->opt.reorder_interval)) {
@@ -649,21 +667,17 @@ int avf_step_profiled(avf_ctx *c, int64_t nsteps, double *kernel_ms, double *tot
         HIPCHK(c, hipEventRecord(first, c->stream));
         started = true;
       }
-      launch_step(c, &ev[(size_t)i * (AVF_NKERNELS + 1)]);
+      launch_step(c, &base[i * kEv]);
       c->steps_since_reorder++;
       c->steps_total++;
+      batch_len[cur]++;
     }
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int64_t i = 0; i < k; i++)
-      for (int j = 0; j < AVF_NKERNELS; j++) {
-        float ms = 0.f;
-        HIPCHK(c, hipEventElapsedTime(&ms, ev[(size_t)i * (AVF_NKERNELS + 1) + j],
-                                      ev[(size_t)i * (AVF_NKERNELS + 1) + j + 1]));
-        sum[j] += ms;
-      }
-    done += k;
+    queued += k;
+    cur ^= 1;
+    if ((rc = read_batch(cur))) break;  // the previous batch, while this one runs
   }
+  if (rc == AVF_OK) rc = read_batch(cur ^ 1);
   HIPCHK(c, hipEventRecord(last, c->stream));
   HIPCHK(c, hipEventSynchronize(last));
   float tot = 0.f;

@@ -773,26 +773,82 @@ __global__ __launch_bounds__(BS) void k_force(
 
 // -------------------------------------------------------------- clump-cell regrid ---
 
+// Block-level min / max of order-preserving images, then one atomic per block and quantity
+// (lanes of other instances than the block's first contribute through their own atomics).
+// vals: 8 images per lane; mode bit k: 1 = max, 0 = min.
+__device__ __forceinline__ void block_minmax8(long long *acc_inst0, int inst0, int inst,
+                                             long long *acc_lane, const long long *vals,
+                                             bool active, unsigned mode) {
+  __shared__ long long red[8][256];
+  __shared__ int mixed;
+  if (threadIdx.x == 0) mixed = 0;
+  __syncthreads();
+  if (active && inst != inst0) mixed = 1;
+  for (int k = 0; k < 8; k++) {
+    const bool mx = (mode >> k) & 1u;
+    const long long idle = mx ? (long long)0x8000000000000000ull : 0x7fffffffffffffffll;
+    red[k][threadIdx.x] = (active && inst == inst0) ? vals[k] : idle;
+  }
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int k = 0; k < 8; k++) {
+        const bool mx = (mode >> k) & 1u;
+        const long long a = red[k][threadIdx.x], b = red[k][threadIdx.x + s];
+        red[k][threadIdx.x] = mx ? (a > b ? a : b) : (a < b ? a : b);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 8) {
+    const bool mx = (mode >> threadIdx.x) & 1u;
+    if (mx)
+      atomicMax(&acc_inst0[threadIdx.x], red[threadIdx.x][0]);
+    else
+      atomicMin(&acc_inst0[threadIdx.x], red[threadIdx.x][0]);
+  }
+  if (mixed && active && inst != inst0)
+    for (int k = 0; k < 8; k++) {
+      const bool mx = (mode >> k) & 1u;
+      if (mx)
+        atomicMax(&acc_lane[k], vals[k]);
+      else
+        atomicMin(&acc_lane[k], vals[k]);
+    }
+}
+
 // cell_dim.cu:27-52 over every clump (frozen included) when cc % regrid_interval == 0
 // (force = 1 runs it unconditionally).  acc per instance: min x,y,z, max x,y,z, max vx,
-// min vz as order-preserving integers.
-__global__ void k_cellreduce(const double *__restrict__ st, const int32_t *__restrict__ inst_of,
-                             const InstParams *__restrict__ P, long long *__restrict__ acc,
-                             int64_t n, int64_t cap, int force) {
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  const int inst = inst_of[c];
+// min vz as order-preserving integers.  256 threads per block.
+__global__ __launch_bounds__(256) void k_cellreduce(const double *__restrict__ st,
+                                                    const int32_t *__restrict__ inst_of,
+                                                    const InstParams *__restrict__ P,
+                                                    long long *__restrict__ acc, int64_t n,
+                                                    int64_t cap, int force) {
+  const int64_t c0 = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t c = c0 + threadIdx.x;
+  const int inst0 = inst_of[c0];
+  const int inst = c < n ? inst_of[c] : inst0;
   const InstParams &p = P[inst];
-  if (!force && !(p.regrid_interval > 0 && p.cc % p.regrid_interval == 0)) return;
-  long long *a = acc + 8 * inst;
-  atomicMin(&a[0], order_key(st[S_RX * cap + c]));
-  atomicMin(&a[1], order_key(st[S_RY * cap + c]));
-  atomicMin(&a[2], order_key(st[S_RZ * cap + c]));
-  atomicMax(&a[3], order_key(st[S_RX * cap + c]));
-  atomicMax(&a[4], order_key(st[S_RY * cap + c]));
-  atomicMax(&a[5], order_key(st[S_RZ * cap + c]));
-  atomicMax(&a[6], order_key(st[S_VX * cap + c]));
-  atomicMin(&a[7], order_key(st[S_VZ * cap + c]));
+  const bool due = force || (p.regrid_interval > 0 && p.cc % p.regrid_interval == 0);
+  // whole-block early exit when the first instance is not due and the block is uniform
+  const InstParams &p0 = P[inst0];
+  const bool due0 = force || (p0.regrid_interval > 0 && p0.cc % p0.regrid_interval == 0);
+  const int64_t clast = (c0 + blockDim.x - 1 < n) ? c0 + blockDim.x - 1 : n - 1;
+  if (!due0 && inst_of[clast] == inst0) return;
+  long long v[8];
+  const bool active = c < n && due;
+  if (active) {
+    v[0] = order_key(st[S_RX * cap + c]);
+    v[1] = order_key(st[S_RY * cap + c]);
+    v[2] = order_key(st[S_RZ * cap + c]);
+    v[3] = v[0];
+    v[4] = v[1];
+    v[5] = v[2];
+    v[6] = order_key(st[S_VX * cap + c]);
+    v[7] = order_key(st[S_VZ * cap + c]);
+  }
+  block_minmax8(acc + 8 * inst0, inst0, inst, acc + 8 * inst, v, active,
+                0x78u /* max: 3,4,5,6 */);
 }
 
 // cell_dim.cu:55-60 margins and the regrid arithmetic of avalanche_hst.cu:311-333; then
@@ -839,21 +895,30 @@ __global__ void k_finalize(InstParams *__restrict__ P, long long *__restrict__ a
 // --------------------------------------------------------------- sphere-cell grid ---
 
 // AABB of clump centres and max |v| per instance (acc: 8 per instance as in
-// k_cellreduce; slot 6 holds max |v|^2 as an ordered double).
-__global__ void k_sgrid_reduce(const double *__restrict__ st, const int32_t *__restrict__ inst_of,
-                               long long *__restrict__ acc, int64_t n, int64_t cap) {
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  long long *a = acc + 8 * inst_of[c];
-  const double x = st[S_RX * cap + c], y = st[S_RY * cap + c], z = st[S_RZ * cap + c];
-  const double vx = st[S_VX * cap + c], vy = st[S_VY * cap + c], vz = st[S_VZ * cap + c];
-  atomicMin(&a[0], order_key(x));
-  atomicMin(&a[1], order_key(y));
-  atomicMin(&a[2], order_key(z));
-  atomicMax(&a[3], order_key(x));
-  atomicMax(&a[4], order_key(y));
-  atomicMax(&a[5], order_key(z));
-  atomicMax(&a[6], order_key(vx * vx + vy * vy + vz * vz));
+// k_cellreduce; slot 6 holds max |v|^2 as an ordered double).  256 threads per block.
+__global__ __launch_bounds__(256) void k_sgrid_reduce(const double *__restrict__ st,
+                                                      const int32_t *__restrict__ inst_of,
+                                                      long long *__restrict__ acc, int64_t n,
+                                                      int64_t cap) {
+  const int64_t c0 = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t c = c0 + threadIdx.x;
+  const int inst0 = inst_of[c0];
+  const int inst = c < n ? inst_of[c] : inst0;
+  long long v[8];
+  const bool active = c < n;
+  if (active) {
+    const double x = st[S_RX * cap + c], y = st[S_RY * cap + c], z = st[S_RZ * cap + c];
+    const double vx = st[S_VX * cap + c], vy = st[S_VY * cap + c], vz = st[S_VZ * cap + c];
+    v[0] = order_key(x);
+    v[1] = order_key(y);
+    v[2] = order_key(z);
+    v[3] = v[0];
+    v[4] = v[1];
+    v[5] = v[2];
+    v[6] = order_key(vx * vx + vy * vy + vz * vz);
+    v[7] = 0x7fffffffffffffffll;
+  }
+  block_minmax8(acc + 8 * inst0, inst0, inst, acc + 8 * inst, v, active, 0x78u);
 }
 
 // Lays out every instance's sphere grid (cell side h >= 2 * largest sphere radius, so any

@@ -112,12 +112,20 @@ def main() -> int:
     ctx.step(args.warmup)
     ctx.sync()
     torch.cuda.synchronize()
+    # timed region: K steps with HIP events between consecutive kernels on the library's
+    # stream (per-kernel durations for the roofline)
     t0 = time.perf_counter()
     kern, total_ms = ctx.step_profiled(args.steps)
     ctx.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     diag = ctx.diag()
+    # the same number of steps through the captured-graph path, for comparison
+    t0 = time.perf_counter()
+    ctx.step(args.steps)
+    ctx.sync()
+    torch.cuda.synchronize()
+    wall_graph = time.perf_counter() - t0
     ctx.close()
 
     value = n * args.steps / wall
@@ -141,6 +149,7 @@ def main() -> int:
         "kernel_ms": kern,
         "step_hbm_gbs_algorithmic": STEP_BYTES_PER_CLUMP * n / (wall / args.steps) / 1e9,
         "events_total_ms": total_ms,
+        "graph_path_clump_steps_per_s": n * args.steps / wall_graph,
         "diag": diag,
     }
     if args.batch_steps > 0:
