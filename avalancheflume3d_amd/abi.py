@@ -217,4 +217,5 @@ EXPORTED = [
     "avf_get_clump_cells", "avf_check_cells", "avf_get_contacts", "avf_get_diag", "avf_sync",
     "avf_step_profiled", "avf_observe",
 ]
-KERNELS = ["predict", "scan_a", "scan_b", "scan_c", "scatter", "force", "cellreduce", "finalize"]
+KERNELS = ["predict", "scan_a", "scan_b", "scan_c", "scatter", "detect", "force", "cellreduce",
+           "finalize"]

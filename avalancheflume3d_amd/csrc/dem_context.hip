@@ -49,7 +49,8 @@ struct avf_ctx {
   int32_t *d_orig = nullptr, *d_orig2 = nullptr, *d_inst = nullptr, *d_inst2 = nullptr;
   int64_t *d_inv = nullptr, *d_ncells = nullptr, *d_tmp64 = nullptr;
   uint32_t *d_counts = nullptr, *d_starts = nullptr, *d_bsum = nullptr;
-  uint32_t *d_skey = nullptr, *d_srank = nullptr, *d_sid = nullptr;
+  uint32_t *d_skey = nullptr, *d_srank = nullptr, *d_sid = nullptr, *d_clist = nullptr;
+  int32_t *d_ccnt = nullptr;
   double4 *d_spos = nullptr;
   long long *d_acc_cell = nullptr, *d_acc_sg = nullptr;
   DevDiag *d_diag = nullptr;
@@ -187,8 +188,14 @@ void launch_bin(avf_ctx *c, int mode) {
 template <int BS>
 void launch_force_bs(avf_ctx *c, int initial) {
   k_force<kMaxNeigh, BS><<<grid_for(c->n, BS), BS, 0, c->stream>>>(
-      c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_P, c->d_starts, c->d_spos, c->d_sid, c->d_f,
-      c->d_diag, c->n, c->cap, initial);
+      c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_P, c->d_clist, c->d_ccnt, c->d_f, c->d_diag,
+      c->n, c->cap, initial);
+}
+
+void launch_detect(avf_ctx *c) {
+  k_detect<<<grid_for(4 * c->n, kBlock), kBlock, 0, c->stream>>>(
+      c->d_stage, c->d_P, c->d_inst, c->d_starts, c->d_spos, c->d_sid, c->d_clist, c->d_ccnt,
+      c->d_diag, 4 * c->n);
 }
 
 // k_force block shape: 64 (one wavefront) or 128 lanes (avf_options.block_clumps).
@@ -220,6 +227,8 @@ void launch_step(avf_ctx *c, hipEvent_t *ev = nullptr) {
   k_scatter<<<std::max(1u, grid_for(4 * c->n, kBlock)), kBlock, 0, c->stream>>>(
       c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid, c->d_P, c->ninst,
       4 * c->n, 1);
+  mark(AVF_K_DETECT);
+  launch_detect(c);
   mark(AVF_K_FORCE);
   launch_force(c, 0);
   mark(AVF_K_CELLREDUCE);
@@ -377,6 +386,7 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
       (rc = dalloc(c, &c->d_bsum, grid_for(c->cell_cap, kScanTile) + 1)) ||
       (rc = dalloc(c, &c->d_skey, 4 * N)) || (rc = dalloc(c, &c->d_srank, 4 * N)) ||
       (rc = dalloc(c, &c->d_sid, 4 * N)) || (rc = dalloc(c, &c->d_spos, 4 * N)) ||
+      (rc = dalloc(c, &c->d_clist, 4 * N * kSphList)) || (rc = dalloc(c, &c->d_ccnt, 4 * N)) ||
       (rc = dalloc(c, &c->d_acc_cell, 8 * ninst)) || (rc = dalloc(c, &c->d_acc_sg, 8 * ninst)) ||
       (rc = dalloc(c, &c->d_diag, 1)) || (rc = dalloc(c, &c->d_count, 1)))
     return bail(rc);
@@ -432,7 +442,7 @@ int avf_destroy(avf_ctx *c) {
                   c->d_f,      c->d_io,    c->d_io2,   c->d_orig,  c->d_orig2, c->d_inst,
                   c->d_inst2,  c->d_inv,   c->d_ncells, c->d_tmp64, c->d_counts, c->d_starts,
                   c->d_bsum,   c->d_skey,  c->d_srank, c->d_sid,   c->d_spos,  c->d_acc_cell,
-                  c->d_acc_sg, c->d_diag,  c->d_count};
+                  c->d_acc_sg, c->d_diag,  c->d_count, c->d_clist, c->d_ccnt};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -599,6 +609,7 @@ int avf_initial_forces(avf_ctx *c) {
   int rc;
   if (!c->sgrid_valid && (rc = rebuild_grid(c, c->opt.reorder_interval > 0))) return rc;
   launch_bin(c, 0);
+  launch_detect(c);
   launch_force(c, 1);
   c->binned = true;
   HIPCHK(c, hipGetLastError());

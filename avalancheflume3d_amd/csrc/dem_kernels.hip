@@ -515,6 +515,71 @@ __device__ __forceinline__ int64_t ref_cell_locate(const InstParams &p, d3 r) {
   return idx;
 }
 
+constexpr int kSphList = 16;  // touching spheres of other clumps per sphere
+
+// Contact detection, one lane per sphere (lane 4c + s is sphere s of clump c): the 27
+// sphere cells around the sphere, 9 contiguous rows of 3 cells whose 18 bounds are loaded
+// up front; every touching sphere of another clump is listed as (m << 2 | sm).  The test
+// is force_part.cu:140-157's ((dx^2 + dy^2) + dz^2) < sep^2 on the same operands, so the
+// listed set is exactly the set force_part evaluates.
+__global__ __launch_bounds__(256) void k_detect(
+    const double *__restrict__ stage, const InstParams *__restrict__ P,
+    const int32_t *__restrict__ inst_of, const uint32_t *__restrict__ starts,
+    const double4 *__restrict__ spos, const uint32_t *__restrict__ sid,
+    uint32_t *__restrict__ clist, int32_t *__restrict__ ccnt, DevDiag *__restrict__ diag,
+    int64_t nsph) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nsph) return;
+  const int64_t c = i >> 2;
+  const int s = (int)(i & 3);
+  const InstParams &p = P[inst_of[c]];
+  const double *g = stage + c * kStageDoubles;
+  if (frozen(p.time, g[G_RR], p.side_x)) {
+    ccnt[i] = 0;
+    return;
+  }
+  const double rad = g[G_RAD];
+  const d3 q = make_d3(g[G_SPH + 3 * s], g[G_SPH + 3 * s + 1], g[G_SPH + 3 * s + 2]);
+  const int nx = p.sg_n[0], ny = p.sg_n[1], nz = p.sg_n[2];
+  const int ix = grid_index(q.x, p.sg_lo[0], p.sg_inv_h, nx);
+  const int iy = grid_index(q.y, p.sg_lo[1], p.sg_inv_h, ny);
+  const int iz = grid_index(q.z, p.sg_lo[2], p.sg_inv_h, nz);
+  const int x0 = ix > 0 ? ix - 1 : 0, x1 = ix + 1 < nx ? ix + 1 : nx - 1;
+  uint32_t lo[9], hi[9];
+#pragma unroll
+  for (int r = 0; r < 9; r++) {
+    const int y = iy + (r % 3) - 1, z = iz + (r / 3) - 1;
+    lo[r] = hi[r] = 0;
+    if (y >= 0 && y < ny && z >= 0 && z < nz) {
+      const int64_t row = p.sg_base + (int64_t)nx * (y + (int64_t)ny * z);
+      lo[r] = starts[row + x0];
+      hi[r] = starts[row + x1 + 1];
+    }
+  }
+  int cnt = 0;
+  uint32_t *out = clist + i * kSphList;
+#pragma unroll
+  for (int r = 0; r < 9; r++) {
+    for (uint32_t j = lo[r]; j < hi[r]; j++) {
+      const uint32_t id = sid[j];
+      const double4 o = spos[j];
+      if ((int64_t)(id >> 2) == c) continue;
+      const double sep = rad + o.w, sep2 = sep * sep;
+      const double dx = o.x - q.x, dy = o.y - q.y, dz = o.z - q.z;
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 < sep2) {
+        if (cnt < kSphList) out[cnt] = id;
+        cnt++;
+      }
+    }
+  }
+  if (cnt > kSphList) {
+    atomicAdd(&diag->neigh_overflow, 1ull);
+    cnt = kSphList;
+  }
+  ccnt[i] = cnt;
+}
+
 template <int MAXN, int BS>
 struct PPShared {
   int32_t m[MAXN][BS];       // storage index of each touching clump, sorted by key
@@ -528,9 +593,8 @@ template <int MAXN, int BS>
 __global__ __launch_bounds__(BS) void k_force(
     double *__restrict__ st, const double *__restrict__ sta, const double *__restrict__ stage,
     const int32_t *__restrict__ inst_of, const InstParams *__restrict__ P,
-    const uint32_t *__restrict__ starts, const double4 *__restrict__ spos,
-    const uint32_t *__restrict__ sid, double *__restrict__ fout, DevDiag *__restrict__ diag,
-    int64_t n, int64_t cap, int initial) {
+    const uint32_t *__restrict__ clist, const int32_t *__restrict__ ccnt,
+    double *__restrict__ fout, DevDiag *__restrict__ diag, int64_t n, int64_t cap, int initial) {
   __shared__ PPShared<MAXN, BS> sh;
   const int lane = threadIdx.x;
   const int64_t c = (int64_t)blockIdx.x * BS + lane;
@@ -564,8 +628,7 @@ __global__ __launch_bounds__(BS) void k_force(
   }
 
   if ((mods & AVF_MOD_PP) && !frz) {
-    // ---- detection: sweep the 27 sphere cells around each of the 4 spheres ----------
-    const int nx = p.sg_n[0], ny = p.sg_n[1], nz = p.sg_n[2];
+    // ---- the touching clumps of the 4 sphere lists, sorted by the reference key -------
     int cnt = 0;
     bool overflow = false;
     // l's clump cell by DIVISION (force_part.cu:77-85)
@@ -576,37 +639,19 @@ __global__ __launch_bounds__(BS) void k_force(
     int64_t lk = (int64_t)((rr.z - p.rr_min[2]) / p.cell_side[2]);
     if (lk == p.ncell[2]) lk--;
     const int64_t nxy = p.ncell[0] * p.ncell[1];
-
     for (int s = 0; s < 4; s++) {
-      const d3 q = sphere_of(rr, R, pos0, s);
-      const int ix = grid_index(q.x, p.sg_lo[0], p.sg_inv_h, nx);
-      const int iy = grid_index(q.y, p.sg_lo[1], p.sg_inv_h, ny);
-      const int iz = grid_index(q.z, p.sg_lo[2], p.sg_inv_h, nz);
-      const int x0 = ix > 0 ? ix - 1 : 0, x1 = ix + 1 < nx ? ix + 1 : nx - 1;
-      for (int dz = -1; dz <= 1; dz++) {
-        const int z = iz + dz;
-        if (z < 0 || z >= nz) continue;
-        for (int dy = -1; dy <= 1; dy++) {
-          const int y = iy + dy;
-          if (y < 0 || y >= ny) continue;
-          const int64_t row = p.sg_base + (int64_t)nx * (y + (int64_t)ny * z);
-          const uint32_t j0 = starts[row + x0], j1 = starts[row + x1 + 1];
-          for (uint32_t j = j0; j < j1; j++) {
-            const uint32_t id = sid[j];
-            const int64_t cm = (int64_t)(id >> 2);
-            if (cm == c) continue;
-            const double4 o = spos[j];
-            const double sep = rad + o.w, sep2 = sep * sep;
-            const double dx = o.x - q.x, dyy = o.y - q.y, dzz = o.z - q.z;
-            const double d2 = dx * dx + dyy * dyy + dzz * dzz;
-            if (!(d2 < sep2)) continue;
+      const int k = ccnt[4 * c + s];
+      for (int e = 0; e < k; e++) {
+        const uint32_t id = clist[(4 * c + s) * kSphList + e];
+        const int64_t cm = (int64_t)(id >> 2);
+        {
             // contact (l sphere s, m sphere id&3): add to m's entry or insert m by key
             const uint32_t bit = 1u << (((id & 3u) << 2) | (uint32_t)s);
-            int e = 0;
-            for (; e < cnt; e++)
-              if (sh.m[e][lane] == (int32_t)cm) break;
-            if (e < cnt) {
-              sh.mask[e][lane] = (uint16_t)(sh.mask[e][lane] | bit);
+            int e2 = 0;
+            for (; e2 < cnt; e2++)
+              if (sh.m[e2][lane] == (int32_t)cm) break;
+            if (e2 < cnt) {
+              sh.mask[e2][lane] = (uint16_t)(sh.mask[e2][lane] | bit);
               continue;
             }
             const double *g = stage + cm * kStageDoubles;
@@ -642,7 +687,6 @@ __global__ __launch_bounds__(BS) void k_force(
             sh.m[pos][lane] = (int32_t)cm;
             sh.mask[pos][lane] = (uint16_t)bit;
             cnt++;
-          }
         }
       }
     }

@@ -291,8 +291,8 @@ int avf_observe(avf_ctx *ctx, int32_t inst, avf_observables *obs, const avf_prof
 
 /* Kernels of one step, in launch order (indices of avf_step_profiled's kernel_ms). */
 enum {
-  AVF_K_PREDICT, AVF_K_SCAN_A, AVF_K_SCAN_B, AVF_K_SCAN_C, AVF_K_SCATTER, AVF_K_FORCE,
-  AVF_K_CELLREDUCE, AVF_K_FINALIZE, AVF_NKERNELS
+  AVF_K_PREDICT, AVF_K_SCAN_A, AVF_K_SCAN_B, AVF_K_SCAN_C, AVF_K_SCATTER, AVF_K_DETECT,
+  AVF_K_FORCE, AVF_K_CELLREDUCE, AVF_K_FINALIZE, AVF_NKERNELS
 };
 /* Like avf_step, but launches eagerly with HIP events between consecutive kernels on the
  * context's stream: kernel_ms[k] receives the mean duration of kernel k per step and
