@@ -111,6 +111,22 @@ class ProfileSpec(C.Structure):
                 ("sigma", C.c_double), ("nbins", C.c_int32), ("reserved", C.c_int32)]
 
 
+class WindowSpec(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("checkpoint", C.c_int64), ("skin", C.c_double),
+                ("w_runout", C.c_double), ("w_profile", C.c_double), ("profile", ProfileSpec)]
+
+
+class WindowDiag(C.Structure):
+    _fields_ = [("max_disp", C.c_double), ("max_list", C.c_int64), ("max_in", C.c_int64),
+                ("missed", C.c_int64), ("list_valid", C.c_int32), ("reserved", C.c_int32), ("forward_ms", C.c_double),
+                ("tangent_ms", C.c_double), ("backward_ms", C.c_double)]
+
+
+# theta layout per instance (avf_window_run)
+THETA = ["mu_pp", "eps_pp", "kappa_pp", "mu_pw", "eps_pw", "kappa_pw"]
+NTHETA = len(THETA)
+
+
 class Rng(C.Structure):
     _fields_ = [("useed", C.c_uint64 * 250), ("uuy", C.c_uint64), ("np", C.c_int32),
                 ("nq", C.c_int32), ("npp", C.c_int32), ("not_init", C.c_int32)]
@@ -197,6 +213,7 @@ def load_library() -> C.CDLL:
             "avf_sync": (C.c_int, [vp]),
             "avf_step_profiled": (C.c_int, [vp, i64, vp, vp]),
             "avf_observe": (C.c_int, [vp, i32, vp, vp, vp]),
+            "avf_window_run": (C.c_int, [vp] * 15),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -215,7 +232,7 @@ EXPORTED = [
     "avf_last_error", "avf_upload", "avf_download", "avf_set_phase", "avf_get_time",
     "avf_regrid", "avf_get_cell", "avf_initial_forces", "avf_step", "avf_get_forces",
     "avf_get_clump_cells", "avf_check_cells", "avf_get_contacts", "avf_get_diag", "avf_sync",
-    "avf_step_profiled", "avf_observe",
+    "avf_step_profiled", "avf_observe", "avf_window_run",
 ]
 KERNELS = ["predict", "scan_a", "scan_b", "scan_c", "scatter", "detect", "force", "cellreduce",
            "finalize"]

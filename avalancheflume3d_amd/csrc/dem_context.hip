@@ -17,6 +17,7 @@
 
 #include "avflume.h"
 #include "dem_kernels.hip"
+#include "dem_window.hip"
 
 using namespace avf;
 
@@ -63,6 +64,7 @@ struct avf_ctx {
   int64_t steps_total = 0;
   std::map<int64_t, hipGraphExec_t> graphs;
   std::vector<hipGraph_t> graph_defs;
+  struct WinBuf *win = nullptr;  // buffers of the differentiable window (on first use)
 };
 
 namespace {
@@ -434,6 +436,53 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
   return AVF_OK;
 }
 
+}  // extern "C"
+
+// Buffers of the differentiable window, grown on demand and kept with the context.
+struct WinBuf {
+  int64_t cap = 0, nseg = 0, K = 0;
+  int32_t ninst = 0, nbins = -1;
+  bool tangent = false;
+  int32_t *list = nullptr, *cnt = nullptr, *in = nullptr, *icnt = nullptr;
+  uint8_t *eflag = nullptr;
+  double *sph0 = nullptr, *dmax2 = nullptr, *wa = nullptr, *wb = nullptr, *ck = nullptr;
+  double *segst = nullptr, *segstage = nullptr, *segtq = nullptr;
+  double *Sb = nullptr, *Sb2 = nullptr, *sbar = nullptr, *ebuf = nullptr, *tbar = nullptr;
+  double *part = nullptr, *out = nullptr, *instw = nullptr, *wbins = nullptr, *vec = nullptr;
+  Dual *stD = nullptr, *stD2 = nullptr, *stageD = nullptr;
+  WinParams *WP = nullptr;
+  InstParams *allP = nullptr;  // parameters of every window step (clump-cell grid, cc)
+  int64_t allP_steps = 0;
+  long long *acc = nullptr;    // k_cellreduce accumulators of the window
+  WinDiag *wd = nullptr;
+  unsigned long long *umax = nullptr, *missed = nullptr;
+  // the window's own binning of a state (list check)
+  double *bstage = nullptr;
+  double4 *bspos = nullptr;
+  uint32_t *bkey = nullptr, *brank = nullptr, *bsid = nullptr, *bstarts = nullptr;
+};
+
+namespace {
+
+void free_window(avf_ctx *c) {
+  WinBuf *w = c->win;
+  if (!w) return;
+  void *bufs[] = {w->list,  w->cnt,   w->in,    w->icnt,     w->eflag, w->sph0,  w->dmax2,
+                  w->wa,    w->wb,    w->ck,    w->segst,    w->segstage, w->segtq, w->Sb,
+                  w->Sb2,   w->sbar,  w->ebuf,  w->tbar,     w->part,  w->out,   w->instw,
+                  w->wbins, w->vec,   w->stD,   w->stD2,     w->stageD, w->WP,   w->wd,
+                  w->umax,  w->missed, w->bstage, w->bspos, w->bkey, w->brank, w->bsid,
+                  w->bstarts, w->allP, w->acc};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  delete w;
+  c->win = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
 int avf_destroy(avf_ctx *c) {
   if (!c) return AVF_OK;
   for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
@@ -445,6 +494,7 @@ int avf_destroy(avf_ctx *c) {
                   c->d_acc_sg, c->d_diag,  c->d_count, c->d_clist, c->d_ccnt};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
+  free_window(c);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return AVF_OK;
@@ -793,36 +843,27 @@ int avf_get_contacts(avf_ctx *c, int32_t inst, avf_contact *out, int64_t capn, i
   return AVF_OK;
 }
 
-int avf_observe(avf_ctx *c, int32_t inst, avf_observables *obs, const avf_profile_spec *spec,
-                double *profile) {
-  int rc = check_inst(c, inst);
-  if (rc) return rc;
+}  // extern "C"
+
+namespace {
+
+// Observable sums of one instance for the state st (kObsScalars + nbins doubles; see
+// k_observe) into h.
+int observe_state(avf_ctx *c, const double *st, int32_t inst, int nbins, double x0, double bw,
+                  double sigma, double width, std::vector<double> &h) {
   const int64_t cnt = c->desc[inst].sys.npart;
-  int nbins = 0;
-  double x0 = 0.0, bw = 1.0, sigma = 1.0, width = 1.0;
-  if (spec && profile) {
-    if (spec->nbins <= 0 || spec->nbins > kMaxBins || !(spec->x1 > spec->x0) ||
-        !(spec->width > 0.0))
-      return fail(c, AVF_ERR_ARG, "bad deposit profile spec (nbins 1..%d, x1 > x0, width > 0)",
-                  kMaxBins);
-    nbins = spec->nbins;
-    x0 = spec->x0;
-    bw = (spec->x1 - spec->x0) / nbins;
-    sigma = spec->sigma > 0.0 ? spec->sigma : bw;
-    width = spec->width;
-  }
   const int64_t nb = (cnt + kObsBlock - 1) / kObsBlock;
   const int64_t w = kObsScalars + nbins;
   double *d_part = nullptr, *d_out = nullptr;
+  int rc;
   if ((rc = dalloc(c, &d_part, nb * w)) || (rc = dalloc(c, &d_out, w))) {
     if (d_part) (void)hipFree(d_part);
     return rc;
   }
-  k_observe<<<(unsigned)nb, kObsBlock, 0, c->stream>>>(c->d_st, c->d_sta, c->d_inv, c->d_P, inst,
-                                                      cnt, c->cap, nbins, x0, bw, sigma, width,
-                                                      d_part);
+  k_observe<<<(unsigned)nb, kObsBlock, 0, c->stream>>>(st, c->d_sta, c->d_inv, c->d_P, inst, cnt,
+                                                      c->cap, nbins, x0, bw, sigma, width, d_part);
   k_observe_final<<<1, 1, 0, c->stream>>>(d_part, nb, nbins, d_out);
-  std::vector<double> h((size_t)w);
+  h.assign((size_t)w, 0.0);
   hipError_t he = hipGetLastError();
   if (he == hipSuccess)
     he = hipMemcpyAsync(h.data(), d_out, w * sizeof(double), hipMemcpyDeviceToHost, c->stream);
@@ -830,22 +871,59 @@ int avf_observe(avf_ctx *c, int32_t inst, avf_observables *obs, const avf_profil
   (void)hipFree(d_part);
   (void)hipFree(d_out);
   if (he != hipSuccess) return fail(c, AVF_ERR_HIP, "observe: %s", hipGetErrorString(he));
-  if (obs) {
-    obs->vmean = h[0] / (double)cnt;  // avalanche_hst.cu:573-574 (vmean /= nn)
-    obs->wmean = h[1] / (double)cnt;
-    obs->xcm = h[2];
-    obs->zcm = h[3];
-    // xcm /= tot_mass (avalanche_hst.cu:827-828); tot_mass from the instance description or,
-    // when it was not given, the sum of the clump masses
-    double tot = c->desc[inst].sys.tot_mass;
-    if (!(tot > 0.0)) tot = h[8];
-    obs->xcm /= tot;
-    obs->zcm /= tot;
-    obs->runout_mean = h[6] > 0.0 ? h[4] / h[6] : 0.0;
-    obs->runout_front = h[5];
-    obs->mass_live = h[6];
-    obs->live = (int64_t)h[7];
-  }
+  return AVF_OK;
+}
+
+int profile_params(avf_ctx *c, const avf_profile_spec *spec, int *nbins, double *x0, double *bw,
+                   double *sigma, double *width) {
+  *nbins = 0;
+  *x0 = 0.0;
+  *bw = *sigma = *width = 1.0;
+  if (!spec || spec->nbins == 0) return AVF_OK;
+  if (spec->nbins < 0 || spec->nbins > kMaxBins || !(spec->x1 > spec->x0) || !(spec->width > 0.0))
+    return fail(c, AVF_ERR_ARG, "bad deposit profile spec (nbins 1..%d, x1 > x0, width > 0)",
+                kMaxBins);
+  *nbins = spec->nbins;
+  *x0 = spec->x0;
+  *bw = (spec->x1 - spec->x0) / spec->nbins;
+  *sigma = spec->sigma > 0.0 ? spec->sigma : *bw;
+  *width = spec->width;
+  return AVF_OK;
+}
+
+void fill_observables(avf_ctx *c, int32_t inst, const std::vector<double> &h, avf_observables *obs) {
+  const int64_t cnt = c->desc[inst].sys.npart;
+  obs->vmean = h[0] / (double)cnt;  // avalanche_hst.cu:573-574 (vmean /= nn)
+  obs->wmean = h[1] / (double)cnt;
+  obs->xcm = h[2];
+  obs->zcm = h[3];
+  // xcm /= tot_mass (avalanche_hst.cu:827-828); tot_mass from the instance description or,
+  // when it was not given, the sum of the clump masses
+  double tot = c->desc[inst].sys.tot_mass;
+  if (!(tot > 0.0)) tot = h[8];
+  obs->xcm /= tot;
+  obs->zcm /= tot;
+  obs->runout_mean = h[6] > 0.0 ? h[4] / h[6] : 0.0;
+  obs->runout_front = h[5];
+  obs->mass_live = h[6];
+  obs->live = (int64_t)h[7];
+}
+
+}  // namespace
+
+extern "C" {
+
+int avf_observe(avf_ctx *c, int32_t inst, avf_observables *obs, const avf_profile_spec *spec,
+                double *profile) {
+  int rc = check_inst(c, inst);
+  if (rc) return rc;
+  int nbins;
+  double x0, bw, sigma, width;
+  if ((rc = profile_params(c, profile ? spec : nullptr, &nbins, &x0, &bw, &sigma, &width)))
+    return rc;
+  std::vector<double> h;
+  if ((rc = observe_state(c, c->d_st, inst, nbins, x0, bw, sigma, width, h))) return rc;
+  if (obs) fill_observables(c, inst, h, obs);
   for (int b = 0; b < nbins; b++) profile[b] = h[kObsScalars + b];
   return AVF_OK;
 }
@@ -866,3 +944,475 @@ int avf_get_diag(avf_ctx *c, avf_diag *d) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ differentiable window ----
+
+namespace {
+
+constexpr double kPiW = 3.14159265358979323846;
+
+// gamma(eps, kappa) = -2 ln(eps) sqrt(kappa / (pi^2 + ln^2 eps)) and its partials; with
+// t_c = sqrt((pi^2 + ln^2 eps) / kappa) this is the reference's gamma = -2 ln(eps) / t_c.
+struct LawMap {
+  double kappa, gamma, mu, fr, dg_deps, dg_dkappa;
+};
+
+LawMap law_from_theta(double mu, double eps, double kappa) {
+  LawMap m;
+  const double L = std::log(eps), S2 = kPiW * kPiW + L * L, S = std::sqrt(S2);
+  m.kappa = kappa;
+  m.gamma = -2.0 * L * std::sqrt(kappa / S2);
+  m.mu = mu;
+  m.fr = mu / 1.0e12;
+  m.dg_deps = -2.0 * std::sqrt(kappa) * kPiW * kPiW / (eps * S2 * S);
+  m.dg_dkappa = m.gamma / (2.0 * kappa);
+  return m;
+}
+
+// The instance's own law (values exactly as derived by the setup), with the partials of
+// the same relation.
+LawMap law_from_system(double mu, double eps, double kappa, double gamma, double fr) {
+  LawMap m = law_from_theta(mu, eps, kappa);
+  m.gamma = gamma;
+  m.fr = fr;
+  m.dg_dkappa = gamma / (2.0 * kappa);
+  return m;
+}
+
+template <typename T>
+int walloc(avf_ctx *c, T **p, int64_t count) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  return dalloc(c, p, count);
+}
+
+int ensure_window(avf_ctx *c, int64_t W, int64_t nseg, int64_t K, bool tangent) {
+  if (!c->win) c->win = new WinBuf();
+  WinBuf *w = c->win;
+  const int64_t cap = c->cap;
+  int rc;
+  if (w->cap != cap) {
+    const int64_t nb = (cap + kObsBlock - 1) / kObsBlock;
+    if ((rc = walloc(c, &w->list, cap * kWinMaxList)) || (rc = walloc(c, &w->cnt, cap)) ||
+        (rc = walloc(c, &w->in, cap * kWinMaxIn)) || (rc = walloc(c, &w->icnt, cap)) ||
+        (rc = walloc(c, &w->eflag, cap * kWinMaxList)) || (rc = walloc(c, &w->sph0, cap * 12)) ||
+        (rc = walloc(c, &w->dmax2, cap)) || (rc = walloc(c, &w->wa, cap * S_NFIELDS)) ||
+        (rc = walloc(c, &w->wb, cap * S_NFIELDS)) || (rc = walloc(c, &w->Sb, cap * S_NFIELDS)) ||
+        (rc = walloc(c, &w->Sb2, cap * S_NFIELDS)) || (rc = walloc(c, &w->sbar, cap * kStageW)) ||
+        (rc = walloc(c, &w->ebuf, cap * kWinMaxList * kEbuf)) ||
+        (rc = walloc(c, &w->tbar, cap * TB_N)) || (rc = walloc(c, &w->part, nb * TB_N)) ||
+        (rc = walloc(c, &w->out, (int64_t)c->ninst * TB_N)) ||
+        (rc = walloc(c, &w->vec, cap * 6)) || (rc = walloc(c, &w->WP, c->ninst)) ||
+        (rc = walloc(c, &w->wd, 1)) || (rc = walloc(c, &w->umax, 1)) ||
+        (rc = walloc(c, &w->missed, 1)) || (rc = walloc(c, &w->bstage, cap * kStageDoubles)) ||
+        (rc = walloc(c, &w->bspos, cap * 4)) || (rc = walloc(c, &w->bkey, cap * 4)) ||
+        (rc = walloc(c, &w->brank, cap * 4)) || (rc = walloc(c, &w->bsid, cap * 4)) ||
+        (rc = walloc(c, &w->bstarts, c->cell_cap + 1)) ||
+        (rc = walloc(c, &w->acc, (int64_t)8 * c->ninst)) ||
+        (rc = walloc(c, &w->instw, c->ninst)) ||
+        (rc = walloc(c, &w->wbins, (int64_t)c->ninst * kMaxBins)))
+      return rc;
+    w->cap = cap;
+    w->nseg = w->K = w->allP_steps = 0;
+    w->tangent = false;
+  }
+  if (W + 1 > w->allP_steps) {
+    if ((rc = walloc(c, &w->allP, (W + 1) * c->ninst))) return rc;
+    w->allP_steps = W + 1;
+  }
+  if (nseg > w->nseg) {
+    if ((rc = walloc(c, &w->ck, nseg * cap * S_NFIELDS))) return rc;
+    w->nseg = nseg;
+  }
+  if (K > w->K) {
+    if ((rc = walloc(c, &w->segst, K * cap * S_NFIELDS)) ||
+        (rc = walloc(c, &w->segstage, K * cap * kStageW)) ||
+        (rc = walloc(c, &w->segtq, K * cap * 3)))
+      return rc;
+    w->K = K;
+  }
+  if (tangent && !w->tangent) {
+    if ((rc = walloc(c, &w->stD, cap * S_NFIELDS)) || (rc = walloc(c, &w->stD2, cap * S_NFIELDS)) ||
+        (rc = walloc(c, &w->stageD, cap * kStageW)))
+      return rc;
+    w->tangent = true;
+  }
+  return AVF_OK;
+}
+
+// One window step (double) with the step's parameters P: stage into stg, new state into
+// out.
+void win_step(avf_ctx *c, const InstParams *P, const double *cur, double *out, double *stg,
+              double *tq, const double *sph0, double *dmax2, WinDiag *wd) {
+  WinBuf *w = c->win;
+  k_win_predict<double><<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(cur, c->d_sta, P,
+                                                                         c->d_inst, stg, c->n,
+                                                                         c->cap);
+  k_win_force<double><<<grid_for(c->n, 64), 64, 0, c->stream>>>(
+      cur, out, c->d_sta, stg, P, w->WP, c->d_inst, w->list, w->cnt, tq, sph0, dmax2, wd, c->n,
+      c->cap);
+}
+
+// Bins the state st into the window's own sphere arrays (stage, sorted spheres); the
+// context's own binning stays as it is.
+void win_bin(avf_ctx *c, const double *st) {
+  WinBuf *w = c->win;
+  const unsigned nb = grid_for(c->cell_cap, kScanTile);
+  k_predict<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(
+      const_cast<double *>(st), c->d_sta, w->bstage, c->d_inst, c->d_orig, c->d_P, c->d_counts,
+      w->bkey, w->brank, c->n, c->cap, 0);
+  k_scan_a<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
+  k_scan_b<<<1, 1024, 0, c->stream>>>(c->d_bsum, c->d_ncells, w->bstarts);
+  k_scan_c<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, w->bstarts, c->d_ncells);
+  k_scatter<<<std::max(1u, grid_for(4 * c->n, kBlock)), kBlock, 0, c->stream>>>(
+      w->bkey, w->brank, w->bstarts, w->bstage, w->bspos, w->bsid, c->d_P, c->ninst, 4 * c->n, 0);
+}
+
+// Adds the touching sphere pairs of state st that the frozen list lacks to w->missed.
+void win_check_list(avf_ctx *c, const double *st) {
+  WinBuf *w = c->win;
+  win_bin(c, st);
+  k_win_missed<<<grid_for(4 * c->n, kBlock), kBlock, 0, c->stream>>>(
+      w->bstage, c->d_P, c->d_inst, w->bstarts, w->bspos, w->bsid, w->list, w->cnt, w->missed,
+      4 * c->n);
+}
+
+// Per-instance sums, in reference order, of a per-clump AoS array of width wdt (vals) or of
+// the tangent dot product (vals NULL) -> host out[inst * wdt + k].
+int win_reduce(avf_ctx *c, const double *vals, int wdt, const Dual *stD, std::vector<double> &out) {
+  WinBuf *w = c->win;
+  out.assign((size_t)c->ninst * wdt, 0.0);
+  for (int32_t i = 0; i < c->ninst; i++) {
+    const int64_t cnt = c->desc[i].sys.npart;
+    const int64_t nb = (cnt + kObsBlock - 1) / kObsBlock;
+    k_win_partial<<<(unsigned)nb, kObsBlock, 0, c->stream>>>(vals, wdt, w->Sb, stD, c->d_inv,
+                                                            c->begin[i], cnt, c->cap, w->part);
+    k_win_partial_final<<<1, 64, 0, c->stream>>>(w->part, nb, wdt, w->out + (int64_t)i * wdt);
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(out.data(), w->out, out.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return AVF_OK;
+}
+
+int read_wdiag(avf_ctx *c, WinDiag *h) {
+  HIPCHK(c, hipMemcpyAsync(h, c->win->wd, sizeof *h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return AVF_OK;
+}
+
+}  // namespace
+
+extern "C" int avf_window_run(avf_ctx *c, const avf_window_spec *spec, const double *theta,
+                              const double *dv, const double *target, double *loss,
+                              avf_observables *obs, double *profile, double *grad_theta,
+                              double *grad_dv, const double *tan_theta, const double *tan_dv,
+                              double *dloss, double *final_state, avf_window_diag *diag) {
+  if (!c || !spec || !loss) return AVF_ERR_ARG;
+  const int64_t W = spec->steps;
+  const int64_t K = spec->checkpoint > 0 ? std::min(spec->checkpoint, W) : W;
+  if (W <= 0) return fail(c, AVF_ERR_ARG, "window length must be positive");
+  if (!(spec->skin >= 0.0)) return fail(c, AVF_ERR_ARG, "skin must be >= 0");
+  if (c->cap * kWinMaxList > (int64_t)INT32_MAX)
+    return fail(c, AVF_ERR_ARG, "window supports up to %lld clumps per context",
+                (long long)(INT32_MAX / kWinMaxList));
+  for (int32_t i = 0; i < c->ninst; i++) {
+    if (c->desc[i].phase != AVF_PHASE_AVALANCHE)
+      return fail(c, AVF_ERR_INPUT, "instance %d: the window needs the avalanche phase", i);
+    if (c->desc[i].geo.modules_avalanche & (AVF_MOD_FUNNEL | AVF_MOD_FCORNERS))
+      return fail(c, AVF_ERR_INPUT,
+                  "instance %d: the window does not support the funnel / funnel-corner modules", i);
+  }
+  int rc, nbins;
+  double x0, bw, sigma, width;
+  if ((rc = profile_params(c, &spec->profile, &nbins, &x0, &bw, &sigma, &width))) return rc;
+  const int64_t nseg = (W + K - 1) / K;
+  const bool want_tan = dloss != nullptr;
+  const bool want_grad = grad_theta != nullptr || grad_dv != nullptr;
+  if ((rc = ensure_window(c, W, nseg, K, want_tan))) return rc;
+  WinBuf *w = c->win;
+  const int64_t n = c->n, cap = c->cap;
+
+  // contact-law parameters and tangent directions
+  std::vector<WinParams> hp((size_t)c->ninst);
+  std::vector<LawMap> maps((size_t)c->ninst * 2);
+  for (int32_t i = 0; i < c->ninst; i++) {
+    const avf_system &s = c->desc[i].sys;
+    for (int set = 0; set < 2; set++) {
+      LawMap m;
+      if (theta) {
+        const double *t = theta + (size_t)i * AVF_NTHETA + 3 * set;
+        if (!(t[1] > 0.0 && t[1] < 1.0) || !(t[2] > 0.0) || !(t[0] >= 0.0))
+          return fail(c, AVF_ERR_ARG, "instance %d: theta needs mu >= 0, 0 < eps < 1, kappa > 0",
+                      i);
+        m = law_from_theta(t[0], t[1], t[2]);
+      } else if (set == 0) {
+        m = law_from_system(s.mu_pp, s.eps_pp, s.kappa_pp, s.gamma_pp, s.fr_fact_pp);
+      } else {
+        m = law_from_system(s.mu_pw, s.eps_pw, s.kappa_pw, s.gamma_pw, s.fr_fact_pw);
+      }
+      maps[(size_t)i * 2 + set] = m;
+      WinParams &wp = hp[(size_t)i];
+      wp.v[set][WP_KAPPA] = m.kappa;
+      wp.v[set][WP_GAMMA] = m.gamma;
+      wp.v[set][WP_MU] = m.mu;
+      wp.v[set][WP_FR] = m.fr;
+      double tm = 0.0, te = 0.0, tk = 0.0;
+      if (tan_theta) {
+        const double *t = tan_theta + (size_t)i * AVF_NTHETA + 3 * set;
+        tm = t[0];
+        te = t[1];
+        tk = t[2];
+      }
+      wp.d[set][WP_KAPPA] = tk;
+      wp.d[set][WP_GAMMA] = m.dg_deps * te + m.dg_dkappa * tk;
+      wp.d[set][WP_MU] = tm;
+      wp.d[set][WP_FR] = tm / 1.0e12;
+    }
+  }
+  hipEvent_t ev[4];
+  for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
+  struct EvGuard {
+    hipEvent_t *e;
+    ~EvGuard() {
+      for (int k = 0; k < 4; k++) (void)hipEventDestroy(e[k]);
+    }
+  } guard{ev};
+  HIPCHK(c, hipEventRecord(ev[0], c->stream));
+  HIPCHK(c, hipMemcpyAsync(w->WP, hp.data(), hp.size() * sizeof(WinParams), hipMemcpyHostToDevice,
+                           c->stream));
+  // sphere grid of the current positions (kept as it is when valid; the storage order is not
+  // changed), the window's own binning of the start state, the frozen list and its transpose
+  if (!c->sgrid_valid) {
+    if ((rc = rebuild_grid(c, false))) return rc;
+    c->binned = false;
+  }
+  win_bin(c, c->d_st);
+  HIPCHK(c, hipMemsetAsync(w->wd, 0, sizeof(WinDiag), c->stream));
+  HIPCHK(c, hipMemsetAsync(w->icnt, 0, cap * sizeof(int32_t), c->stream));
+  HIPCHK(c, hipMemsetAsync(w->umax, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(c, hipMemsetAsync(w->missed, 0, sizeof(unsigned long long), c->stream));
+  k_win_list<<<grid_for(n, 64), 64, 0, c->stream>>>(
+      c->d_st, c->d_sta, w->bstage, c->d_P, c->d_inst, w->bstarts, w->bspos, w->bsid,
+      spec->skin, w->list, w->cnt, w->sph0, w->dmax2, w->wd, n, cap);
+  k_win_in_fill<<<grid_for(n, kBlock), kBlock, 0, c->stream>>>(w->list, w->cnt, w->in, w->icnt,
+                                                               w->wd, n);
+  k_win_in_sort<<<grid_for(n, kBlock), kBlock, 0, c->stream>>>(w->in, w->icnt, c->d_orig,
+                                                               w->wd, n);
+  // start state (checkpoint 0) with the velocity perturbation, and its dual lift
+  if (dv)
+    HIPCHK(c, hipMemcpyAsync(w->vec, dv, n * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (want_tan && tan_dv)
+    HIPCHK(c, hipMemcpyAsync(w->vec + 3 * cap, tan_dv, n * 3 * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+  k_win_start<<<grid_for(n, kBlock), kBlock, 0, c->stream>>>(
+      c->d_st, w->ck, c->d_inv, dv ? w->vec : nullptr, (want_tan && tan_dv) ? w->vec + 3 * cap : nullptr,
+      want_tan ? w->stD : nullptr, n, cap);
+  HIPCHK(c, hipGetLastError());
+  WinDiag hd;
+  if ((rc = read_wdiag(c, &hd))) return rc;
+  if (hd.frozen)
+    return fail(c, AVF_ERR_INPUT,
+                "%llu clumps are past side.x or before the gate opening (time <= 0); the window "
+                "needs live clumps in the avalanche phase", hd.frozen);
+  if (hd.aniso)
+    return fail(c, AVF_ERR_INPUT, "%llu clumps have anisotropic inertia; the window needs "
+                "isotropic clumps (clump_mode 1)", hd.aniso);
+  if (hd.list_overflow || hd.in_overflow)
+    return fail(c, AVF_ERR_NEIGH_OVERFLOW,
+                "window list overflow (%llu clumps beyond %d pairs / %d neighbours; %llu beyond "
+                "%d incoming entries): reduce the skin", hd.list_overflow, kWinMaxList,
+                kWinMaxNeigh, hd.in_overflow, kWinMaxIn);
+
+  // forward window with checkpoints at the segment starts; the parameters of every step
+  // (the clump-cell grid is regridded on the plain step's schedule) are kept
+  const size_t SF = (size_t)cap * S_NFIELDS;
+  const int32_t ni = c->ninst;
+  {
+    std::vector<long long> acc0;
+    reset_acc(acc0, ni);
+    HIPCHK(c, hipMemcpyAsync(w->acc, acc0.data(), acc0.size() * sizeof(long long),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(w->allP, c->d_P, ni * sizeof(InstParams), hipMemcpyDeviceToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  const double *cur = w->ck;
+  for (int64_t t = 0; t < W; t++) {
+    double *out;
+    if ((t + 1) % K == 0 && t + 1 < W)
+      out = w->ck + (size_t)((t + 1) / K) * SF;
+    else
+      out = (cur == w->wa) ? w->wb : w->wa;
+    InstParams *Pt = w->allP + (size_t)t * ni;
+    win_step(c, Pt, cur, out, w->segstage, nullptr, w->sph0, w->dmax2, w->wd);
+    k_cellreduce<<<grid_for(n, kBlock), kBlock, 0, c->stream>>>(out, c->d_inst, Pt, w->acc, n, cap,
+                                                                0);
+    k_win_finalize<<<grid_for(ni, 64), 64, 0, c->stream>>>(Pt, Pt + ni, w->acc, ni);
+    cur = out;
+    if ((t + 1) % K == 0 || t + 1 == W) win_check_list(c, cur);
+  }
+  HIPCHK(c, hipGetLastError());
+  const double *fin = cur;
+  if ((rc = read_wdiag(c, &hd))) return rc;
+  if (hd.left)
+    return fail(c, AVF_ERR_INPUT, "a clump reached x > side.x within the window (%llu clump-steps)",
+                hd.left);
+  // loss of the final state, and the adjoint seed
+  std::vector<double> instw((size_t)c->ninst), wbins((size_t)c->ninst * std::max(nbins, 1), 0.0);
+  for (int32_t i = 0; i < c->ninst; i++) {
+    std::vector<double> h;
+    if ((rc = observe_state(c, fin, i, nbins, x0, bw, sigma, width, h))) return rc;
+    avf_observables o;
+    fill_observables(c, i, h, &o);
+    double L = spec->w_runout * o.runout_mean;
+    for (int b = 0; b < nbins; b++) {
+      const double tb = target ? target[(size_t)i * nbins + b] : 0.0;
+      const double r = h[kObsScalars + b] - tb;
+      L += spec->w_profile * r * r;
+      wbins[(size_t)i * nbins + b] = 2.0 * spec->w_profile * r;
+    }
+    loss[i] = L;
+    instw[i] = o.mass_live > 0.0 ? spec->w_runout / o.mass_live : 0.0;
+    if (obs) obs[i] = o;
+    if (profile)
+      for (int b = 0; b < nbins; b++) profile[(size_t)i * nbins + b] = h[kObsScalars + b];
+  }
+  if (final_state) {
+    double *d_fs = nullptr;
+    if ((rc = dalloc(c, &d_fs, n * S_NFIELDS))) return rc;
+    k_win_export<<<grid_for(n, kBlock), kBlock, 0, c->stream>>>(fin, nullptr, c->d_inv, d_fs,
+                                                                nullptr, n, cap);
+    hipError_t he = hipGetLastError();
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(final_state, d_fs, n * S_NFIELDS * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_fs);
+    if (he != hipSuccess) return fail(c, AVF_ERR_HIP, "window export: %s", hipGetErrorString(he));
+  }
+  HIPCHK(c, hipEventRecord(ev[1], c->stream));
+  HIPCHK(c, hipMemcpyAsync(w->instw, instw.data(), instw.size() * sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(w->wbins, wbins.data(), wbins.size() * sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
+  k_win_loss_grad<<<grid_for(n, kBlock), kBlock, 0, c->stream>>>(
+      fin, c->d_sta, c->d_P, c->d_inst, w->instw, w->wbins, nbins,
+      x0, bw, sigma, width, w->Sb, n, cap);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // host vectors above stay alive until here
+
+  // dual-number forward along (tan_theta, tan_dv): dloss = seed . d(final state)
+  float ms_tan = 0.f;
+  if (want_tan) {
+    hipEvent_t t0 = ev[2];
+    HIPCHK(c, hipEventRecord(t0, c->stream));
+    const Dual *dcur = w->stD;
+    for (int64_t t = 0; t < W; t++) {
+      Dual *dout = (dcur == w->stD) ? w->stD2 : w->stD;
+      const InstParams *Pt = w->allP + (size_t)t * ni;
+      k_win_predict<Dual><<<grid_for(n, kBlock), kBlock, 0, c->stream>>>(
+          dcur, c->d_sta, Pt, c->d_inst, w->stageD, n, cap);
+      k_win_force<Dual><<<grid_for(n, 64), 64, 0, c->stream>>>(
+          dcur, dout, c->d_sta, w->stageD, Pt, w->WP, c->d_inst, w->list, w->cnt, nullptr,
+          nullptr, nullptr, nullptr, n, cap);
+      dcur = dout;
+    }
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> d;
+    if ((rc = win_reduce(c, nullptr, 1, dcur, d))) return rc;
+    for (int32_t i = 0; i < c->ninst; i++) dloss[i] = d[(size_t)i];
+    HIPCHK(c, hipEventRecord(ev[3], c->stream));
+    HIPCHK(c, hipEventSynchronize(ev[3]));
+    HIPCHK(c, hipEventElapsedTime(&ms_tan, t0, ev[3]));
+  }
+
+  // reverse pass: per segment, recompute the steps from the checkpoint (keeping each step's
+  // stage and torque), then reverse them
+  float ms_bwd = 0.f;
+  if (want_grad) {
+    hipEvent_t b0 = ev[2], b1 = ev[3];
+    HIPCHK(c, hipEventRecord(b0, c->stream));
+    HIPCHK(c, hipMemsetAsync(w->tbar, 0, (size_t)cap * TB_N * sizeof(double), c->stream));
+    const size_t SG = (size_t)cap * kStageW, TQ = (size_t)cap * 3;
+    double *Sb = w->Sb, *Sb2 = w->Sb2;
+    for (int64_t s = nseg - 1; s >= 0; s--) {
+      const int64_t L = std::min(K, W - s * K);
+      const double *src = w->ck + (size_t)s * SF;
+      for (int64_t k = 0; k < L; k++) {
+        double *dst = w->segst + (size_t)k * SF;
+        win_step(c, w->allP + (size_t)(s * K + k) * ni, src, dst, w->segstage + (size_t)k * SG,
+                 w->segtq + (size_t)k * TQ, nullptr, nullptr, nullptr);
+        src = dst;
+      }
+      for (int64_t k = L - 1; k >= 0; k--) {
+        const double *pre = k == 0 ? w->ck + (size_t)s * SF : w->segst + (size_t)(k - 1) * SF;
+        k_win_adj_force<<<grid_for(n, 64), 64, 0, c->stream>>>(
+            Sb, Sb2, c->d_sta, w->segstage + (size_t)k * SG, w->segtq + (size_t)k * TQ,
+            w->allP + (size_t)(s * K + k) * ni, w->WP, c->d_inst, w->list, w->cnt, w->sbar,
+            w->ebuf, w->eflag, w->tbar, n, cap);
+        k_win_adj_gather<<<grid_for(n, 64), 64, 0, c->stream>>>(
+            pre, Sb2, c->d_sta, w->sbar, w->ebuf, w->eflag, w->list, w->in, w->icnt, c->d_P,
+            c->d_inst, n, cap);
+        std::swap(Sb, Sb2);
+      }
+    }
+    HIPCHK(c, hipGetLastError());
+    if (Sb != w->Sb) {
+      // keep the start-state adjoint in w->Sb (win_reduce's tangent path reads w->Sb)
+      HIPCHK(c, hipMemcpyAsync(w->Sb, Sb, SF * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (grad_dv) {
+      k_win_export<<<grid_for(n, kBlock), kBlock, 0, c->stream>>>(nullptr, w->Sb, c->d_inv,
+                                                                  nullptr, w->vec, n, cap);
+      HIPCHK(c, hipGetLastError());
+      HIPCHK(c, hipMemcpyAsync(grad_dv, w->vec, n * 3 * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+    }
+    std::vector<double> tb;
+    if ((rc = win_reduce(c, w->tbar, TB_N, nullptr, tb))) return rc;
+    if (grad_theta) {
+      for (int32_t i = 0; i < c->ninst; i++) {
+        const double *t = &tb[(size_t)i * TB_N];
+        for (int set = 0; set < 2; set++) {
+          const LawMap &m = maps[(size_t)i * 2 + set];
+          const double Kb = t[set ? TB_KPW : TB_KPP], Gb = t[set ? TB_GPW : TB_GPP];
+          const double Mb = t[set ? TB_MPW : TB_MPP];
+          double *g = grad_theta + (size_t)i * AVF_NTHETA + 3 * set;
+          g[0] = Mb;
+          g[1] = Gb * m.dg_deps;
+          g[2] = Kb + Gb * m.dg_dkappa;
+        }
+      }
+    }
+    HIPCHK(c, hipEventRecord(b1, c->stream));
+    HIPCHK(c, hipEventSynchronize(b1));
+    HIPCHK(c, hipEventElapsedTime(&ms_bwd, b0, b1));
+  }
+  if (diag) {
+    k_win_max<<<grid_for(n, kBlock), kBlock, 0, c->stream>>>(w->dmax2, n, w->umax);
+    unsigned long long um = 0, miss = 0;
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(&um, w->umax, sizeof um, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&miss, w->missed, sizeof miss, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double d2;
+    std::memcpy(&d2, &um, sizeof d2);
+    float ms_fwd = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms_fwd, ev[0], ev[1]));
+    diag->max_disp = std::sqrt(d2);
+    diag->max_list = (int64_t)hd.max_list;
+    diag->max_in = (int64_t)hd.max_in;
+    diag->missed = (int64_t)miss;
+    diag->list_valid = miss == 0 ? 1 : 0;
+    diag->reserved = 0;
+    diag->forward_ms = ms_fwd;
+    diag->tangent_ms = ms_tan;
+    diag->backward_ms = ms_bwd;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return AVF_OK;
+}

@@ -895,14 +895,9 @@ __global__ __launch_bounds__(256) void k_cellreduce(const double *__restrict__ s
                 0x78u /* max: 3,4,5,6 */);
 }
 
-// cell_dim.cu:55-60 margins and the regrid arithmetic of avalanche_hst.cu:311-333; then
-// cc++ (avalanche_hst.cu:600, 852).  step = 0 recomputes the grid without counting a step.
-__global__ void k_finalize(InstParams *__restrict__ P, long long *__restrict__ acc, int32_t ninst,
-                           int force, int step) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ninst) return;
-  InstParams &p = P[i];
-  long long *a = acc + 8 * i;
+// cell_dim.cu:55-60 margins and the regrid arithmetic of avalanche_hst.cu:311-333 when the
+// regrid is due (or forced); then cc++ (avalanche_hst.cu:600, 852) when step is set.
+__device__ __forceinline__ void finalize_params(InstParams &p, long long *a, int force, int step) {
   if (force || (p.regrid_interval > 0 && p.cc % p.regrid_interval == 0)) {
     const double rgyr = 0.5 * p.gyr_max, ddt = p.regrid_ddt_steps * p.dt;
     double rmin[3], rmax[3];
@@ -934,6 +929,14 @@ __global__ void k_finalize(InstParams *__restrict__ P, long long *__restrict__ a
     a[7] = 0x7fffffffffffffffll;
   }
   if (step) p.cc += 1;
+}
+
+// Per instance: finalize_params (step = 0 recomputes the grid without counting a step).
+__global__ void k_finalize(InstParams *__restrict__ P, long long *__restrict__ acc, int32_t ninst,
+                           int force, int step) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ninst) return;
+  finalize_params(P[i], acc + 8 * i, force, step);
 }
 
 // --------------------------------------------------------------- sphere-cell grid ---

@@ -16,6 +16,12 @@ namespace avf {
 
 // ------------------------------------------------------------------------ dual numbers
 
+// The double versions stay visible next to the Dual overloads declared below.
+using ::cos;
+using ::fabs;
+using ::sin;
+using ::sqrt;
+
 struct Dual {
   double v, d;
 };
@@ -84,10 +90,15 @@ __host__ __device__ __forceinline__ V<T> mv(T x, T y, T z) {
 
 // ------------------------------------------------------------------ per-instance data
 
-// Differentiated parameters per instance and their tangents (the Dual forward reads both).
+// Differentiated contact-law parameters per instance, for the particle-particle (set 0)
+// and particle-wall (set 1) laws: kappa (= 2 k_n / sphere mass), gamma (damping rate from
+// the restitution coefficient), mu, and the tangential reset factor mu / 1e12; plus the
+// tangent direction read by the dual-number forward.
+enum WinSet { WS_PP = 0, WS_PW = 1 };
+enum WinPar { WP_KAPPA = 0, WP_GAMMA = 1, WP_MU = 2, WP_FR = 3 };
 struct WinParams {
-  double kappa, gamma, mu, fr;       // kappa = 2 k_n / m_s, gamma(eps, kappa), mu, mu / 1e12
-  double dkappa, dgamma, dmu, dfr;   // tangent direction (Dual forward only)
+  double v[2][4];
+  double d[2][4];
 };
 
 // Adjoint accumulators of the contact-law parameters: pp (kappa, gamma, mu) and walls.
@@ -149,14 +160,6 @@ __host__ __device__ __forceinline__ void add_ft(V<T> &ff, V<T> &tq, V<T> rrc, V<
   ff.y += f.y;
   ff.z += f.z;
 }
-
-// pp contact l <- m (force_part.cu:140-251): r1 sphere of l, r2 sphere of m.
-template <typename T>
-__host__ __device__ __forceinline__ bool pp_contact(V<T> r1, V<T> r2, V<T> xl, V<T> xm, V<T> vl,
-                                                    V<T> vm, V<T> wl, V<T> wm, double rad1,
-                                                    double rad2, double massl, double massm,
-                                                    const WinParams &wp, const Dual *dp,
-                                                    double dt, V<T> &ff, V<T> &tq);
 
 // ------------------------------------------------------------- reverse: contact law ----
 

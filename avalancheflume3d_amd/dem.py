@@ -266,6 +266,57 @@ class Context:
         d = {k: getattr(o, k) for k, _ in abi.Observables._fields_}
         return (d, prof) if profile is not None else d
 
+    def window_run(self, steps: int, checkpoint: int = 100, skin: float = 0.0,
+                   w_runout: float = 1.0, w_profile: float = 0.0,
+                   profile: abi.ProfileSpec | None = None, theta=None, dv=None, target=None,
+                   grad: bool = True, tan_theta=None, tan_dv=None, tangent: bool = False,
+                   final_state: bool = False) -> dict:
+        """Differentiable window from the current state (left unchanged): the loss per
+        instance, optional gradients wrt theta (ninst x 6: mu, eps, kappa for pp and pw)
+        and the start-velocity perturbation dv (clumps x 3, instance-major reference order),
+        and optionally the dual-number directional derivative along (tan_theta, tan_dv)."""
+        ninst = len(self.instances)
+        ntot = sum(self.n(i) for i in range(ninst))
+        spec = abi.WindowSpec()
+        spec.steps, spec.checkpoint, spec.skin = steps, checkpoint, skin
+        spec.w_runout, spec.w_profile = w_runout, w_profile
+        nbins = 0
+        if profile is not None:
+            spec.profile = profile
+            nbins = profile.nbins
+
+        def arr(a, shape):
+            if a is None:
+                return None
+            return np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(shape))
+
+        theta_a = arr(theta, (ninst, abi.NTHETA))
+        dv_a = arr(dv, (ntot, 3))
+        tgt_a = arr(target, (ninst, max(nbins, 1))) if target is not None else None
+        tt_a = arr(tan_theta, (ninst, abi.NTHETA))
+        tdv_a = arr(tan_dv, (ntot, 3))
+        loss = np.zeros(ninst)
+        obs = (abi.Observables * ninst)()
+        prof = np.zeros((ninst, max(nbins, 1)))
+        gth = np.zeros((ninst, abi.NTHETA)) if grad else None
+        gdv = np.zeros((ntot, 3)) if grad else None
+        dl = np.zeros(ninst) if tangent else None
+        fs = np.zeros((ntot, 24)) if final_state else None
+        dg = abi.WindowDiag()
+        self._ck(lib().avf_window_run(self.h, ptr(spec), ptr(theta_a), ptr(dv_a), ptr(tgt_a),
+                                      ptr(loss), ptr(obs), ptr(prof), ptr(gth), ptr(gdv), ptr(tt_a),
+                                      ptr(tdv_a), ptr(dl), ptr(fs), ptr(dg)))
+        out = {"loss": loss, "profile": prof[:, :nbins],
+               "obs": [{k: getattr(o, k) for k, _ in abi.Observables._fields_} for o in obs],
+               "diag": {k: getattr(dg, k) for k, _ in abi.WindowDiag._fields_}}
+        if grad:
+            out["grad_theta"], out["grad_dv"] = gth, gdv
+        if tangent:
+            out["dloss"] = dl
+        if final_state:
+            out["final_state"] = fs
+        return out
+
     def diag(self) -> dict:
         d = abi.Diag()
         self._ck(lib().avf_get_diag(self.h, ptr(d)))

@@ -301,6 +301,69 @@ int avf_step_profiled(avf_ctx *ctx, int64_t nsteps, double *kernel_ms, double *t
 /* Waits for all work queued on the context's stream. */
 int avf_sync(avf_ctx *ctx);
 
+/* ---------------------------------------------------------------- differentiable window
+ * No counterpart in the reference: the adjoint of its step (contact law of
+ * force_part.cu:119-242 and the boundary modules, get_accel.cu, init_/finish_Verlet.cu)
+ * over a window of steps with the contact list frozen at the window start.
+ *
+ * Contact-law parameters per instance, particle-particle (pp) and particle-wall (pw):
+ * friction mu, restitution eps and normal stiffness kappa = 2 k_n / m (the reference's
+ * kappa_pp / kappa_pw).  The damping rate follows the reference's contact-time relations
+ * (avalanche_hst.cu:187-195) with kappa held fixed: t_c = sqrt((pi^2 + ln^2 eps) / kappa),
+ * gamma = -2 ln(eps) / t_c.  theta[inst * AVF_NTHETA + k]. */
+enum {
+  AVF_THETA_MU_PP, AVF_THETA_EPS_PP, AVF_THETA_KAPPA_PP,
+  AVF_THETA_MU_PW, AVF_THETA_EPS_PW, AVF_THETA_KAPPA_PW, AVF_NTHETA
+};
+
+typedef struct avf_window_spec {
+  int64_t steps;             /* window length W (steps)                                    */
+  int64_t checkpoint;        /* K: state checkpoint every K steps; the backward pass       */
+                             /* recomputes each K-step segment from its checkpoint         */
+  double skin;               /* sphere pairs closer than r_l + r_m + skin at the window    */
+                             /* start are listed; no other pair interacts in the window   */
+  double w_runout;           /* loss = w_runout * runout_mean                              */
+  double w_profile;          /*      + w_profile * sum_b (h_b - target_b)^2, per instance */
+  avf_profile_spec profile;  /* bins of h (nbins 0: no profile term)                      */
+} avf_window_spec;
+
+typedef struct avf_window_diag {
+  double max_disp;      /* largest sphere displacement from the window start                */
+  int64_t max_list;     /* largest listed pair count of a clump                             */
+  int64_t max_in;       /* largest count of entries naming one clump                        */
+  int64_t missed;       /* touching sphere pairs absent from the list, summed over the      */
+                        /* checkpoint states and the final state                            */
+  int32_t list_valid;   /* missed == 0                                                      */
+  int32_t reserved;
+  double forward_ms;    /* list build + forward window + loss (HIP events)                  */
+  double tangent_ms;    /* dual-number forward window (0 when not requested)                */
+  double backward_ms;   /* segment recompute + reverse steps + gradient reductions          */
+} avf_window_diag;
+
+/* Runs the window from the context's current state (rr, R, v, w and the stored
+ * accelerations a_i, alpha_i of avf_upload / the last step), which it leaves unchanged.
+ * The instances must be in the avalanche phase at time > 0, with isotropic clump inertia,
+ * and without the funnel / funnel-corner modules.
+ *   theta      [ninst * AVF_NTHETA] or NULL (the instances' own parameters);
+ *   dv         [nclumps * 3] start-velocity perturbation (m/s) or NULL; clump arrays are
+ *              indexed (first clump of the instance + reference index) * 3 + axis;
+ *   target     [ninst * nbins] target profile or NULL (zero);
+ *   loss       [ninst] loss per instance; obs [ninst] / profile [ninst * nbins] optional
+ *              observables of the final state;
+ *   grad_theta [ninst * AVF_NTHETA], grad_dv [nclumps * 3]: dloss/dtheta, dloss/ddv
+ *              (the reverse pass runs if either is non-NULL);
+ *   tan_theta, tan_dv, dloss: if dloss is non-NULL, a dual-number forward along the
+ *              direction (tan_theta, tan_dv) (NULL = 0) gives dloss[inst] = directional
+ *              derivative of loss[inst];
+ *   final_state [nclumps * 24] optional: final rr, vv, ww, R (row-major), a, alpha.
+ * Returns AVF_ERR_INPUT for an unsupported state (message in avf_last_error) and
+ * AVF_ERR_NEIGH_OVERFLOW when a clump has more listed pairs than the list holds. */
+int avf_window_run(avf_ctx *ctx, const avf_window_spec *spec, const double *theta,
+                   const double *dv, const double *target, double *loss, avf_observables *obs,
+                   double *profile, double *grad_theta, double *grad_dv, const double *tan_theta,
+                   const double *tan_dv, double *dloss, double *final_state,
+                   avf_window_diag *diag);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,255 @@
+"""GPU tests of the differentiable window (avf_window_run).
+
+* The window forward (frozen contact list) reproduces the plain step (avf_step) over the
+  window when the list misses no contact.
+* Dual-number tangent of the GPU forward vs the hand-written adjoint (dot-product test):
+  dloss along a random direction equals grad . direction.
+* Gradients vs central finite differences of the host path (the oracle) with h = 1e-6
+  relative, on the same start state.
+* A batch of instances gives the same losses and gradients as separate runs.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from avalancheflume3d_amd import abi, dem, workloads
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+PRE_STEPS = 40000   # release -> flowing layer with contacts on the floor, walls and deposit
+FD_RTOL = 1e-5
+
+
+def flowing_state(seed: int, n: int = 240, length: float = 1.6, deposit: float = 6.0):
+    """A short 35 deg flume ending on a flat deposit, released from a 2 x 2 cm box and run
+    PRE_STEPS steps on the GPU: returns (instance, pp, aa_i, anga_i, time)."""
+    spec = workloads.FlumeSpec(n=n, width=2.0, length=length, box=(2.0, 2.0, 0.4 * math.ceil(n / 25) + 0.01),
+                               deposit_length=deposit, spacing=0.4, jitter=0.02, seed=seed)
+    inst = workloads.build(spec)
+    ctx = dem.Context([inst])
+    ctx.regrid()
+    ctx.initial_forces()
+    ctx.step(PRE_STEPS)
+    pp, aa, anga = ctx.download()
+    t, _ = ctx.time()
+    ctx.close()
+    return inst, pp, aa, anga, t
+
+
+def context_at(inst, pp, aa, anga, t, extra=()):
+    """Context holding the given state (and further (inst, pp, aa, anga, t) tuples)."""
+    items = [(inst, pp, aa, anga, t)] + list(extra)
+    insts = []
+    for it, p, a, g, tt in items:
+        i2 = dem.Instance(sys=abi.System.from_buffer_copy(it.sys), geo=it.geo, fun=it.fun,
+                          particles=p, ntags=it.ntags, phase=abi.PHASE_AVALANCHE, aa_i=a,
+                          anga_i=g)
+        i2.sys.time = tt
+        insts.append(i2)
+    ctx = dem.Context(insts)
+    for k, (it, p, a, g, tt) in enumerate(items):
+        ctx.set_phase(k, abi.PHASE_AVALANCHE, tt, it.sys.side.x)
+    ctx.regrid()  # the clump-cell grid of the start state, as the oracle builds it
+    return ctx
+
+
+@pytest.fixture(scope="module")
+def state():
+    return flowing_state(5)
+
+
+def profile_spec(nbins=16):
+    ps = abi.ProfileSpec()
+    ps.x0, ps.x1, ps.width, ps.sigma, ps.nbins = -0.5, 5.0, 2.0, 0.0, nbins
+    return ps
+
+
+def host_loss(pp, sys, ps, target, w_r=1.0, w_p=10.0):
+    """runout_mean + profile misfit of a final state (the device definitions)."""
+    x, z = pp["rr"][:, 0], pp["rr"][:, 2]
+    live = ~((sys.time > 0) & (x > sys.side.x))
+    m = pp["mass"][live]
+    xs = (x * sys.cost - z * sys.sint)[live]
+    vol = m / pp["density"][live]
+    runout = (m * xs).sum() / m.sum()
+    bw = (ps.x1 - ps.x0) / ps.nbins
+    sig = ps.sigma if ps.sigma > 0 else bw
+    cb = ps.x0 + (np.arange(ps.nbins) + 0.5) * bw
+    u = xs[None, :] - cb[:, None]
+    g = np.where(np.abs(u) < 8 * sig, np.exp(-0.5 * (u / sig) ** 2) / (sig * math.sqrt(2 * math.pi)), 0.0)
+    h = (vol[None, :] * g).sum(1) / ps.width
+    return w_r * runout + w_p * ((h - target) ** 2).sum(), h
+
+
+def gamma_of(eps, kappa):
+    L = math.log(eps)
+    return -2.0 * L * math.sqrt(kappa / (math.pi ** 2 + L * L))
+
+
+def oracle_loss(inst, pp, aa, anga, t, steps, theta, dv, ps, target):
+    sys = abi.System.from_buffer_copy(inst.sys)
+    sys.time = t
+    mu, eps, kap = theta[0], theta[1], theta[2]
+    sys.mu_pp, sys.kappa_pp, sys.gamma_pp, sys.fr_fact_pp = mu, kap, gamma_of(eps, kap), mu / 1e12
+    mu, eps, kap = theta[3], theta[4], theta[5]
+    sys.mu_pw, sys.kappa_pw, sys.gamma_pw, sys.fr_fact_pw = mu, kap, gamma_of(eps, kap), mu / 1e12
+    p2 = pp.copy()
+    p2["vv"] += dv
+    o = oracle.OracleSim(sys, inst.geo, inst.fun, inst.ntags, abi.PHASE_AVALANCHE, p2)
+    o.set(p2, aa, anga)
+    o.step(steps)
+    r = o.get()
+    return host_loss(r["pp"], r["sys"], ps, target)[0]
+
+
+def base_theta(sys):
+    return np.array([sys.mu_pp, sys.eps_pp, sys.kappa_pp, sys.mu_pw, sys.eps_pw, sys.kappa_pw])
+
+
+def test_window_forward_matches_plain_step(gpu, state):
+    inst, pp, aa, anga, t = state
+    steps = 230
+    ctx = context_at(inst, pp, aa, anga, t)
+    out = ctx.window_run(steps, checkpoint=50, skin=0.05, grad=False, final_state=True,
+                         profile=profile_spec())
+    d = out["diag"]
+    print("window diag", d)
+    assert d["max_list"] > 0 and d["list_valid"] == 1
+    ctx.step(steps)
+    p2, a2, g2 = ctx.download()
+    fs = out["final_state"]
+    order = {"rr": slice(0, 3), "vv": slice(3, 6), "ww": slice(6, 9)}
+    exact = True
+    for fld, sl in order.items():
+        a, b = fs[:, sl], p2[fld]
+        err = np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
+        assert err.max() <= 1e-9, (fld, err.max())
+        exact &= bool(np.array_equal(a, b))
+    R = np.stack([p2["Rmatx"], p2["Rmaty"], p2["Rmatz"]], 1).reshape(-1, 9)
+    assert np.abs(fs[:, 9:18] - R).max() <= 1e-9
+    assert np.abs(fs[:, 18:21] - a2).max() <= 1e-9 * max(1.0, np.abs(a2).max())
+    # the window's observables equal avf_observe of the plain run
+    o2, prof2 = ctx.observe(0, profile_spec())
+    assert abs(out["obs"][0]["runout_mean"] - o2["runout_mean"]) <= 1e-9 * abs(o2["runout_mean"])
+    assert np.allclose(out["profile"][0], prof2, rtol=1e-9, atol=1e-12)
+    print("window forward bit-identical to the plain step:", exact)
+    ctx.close()
+
+
+def test_tangent_matches_adjoint(gpu, state):
+    inst, pp, aa, anga, t = state
+    ps = profile_spec()
+    target = np.linspace(0.05, 0.01, ps.nbins)
+    ctx = context_at(inst, pp, aa, anga, t)
+    rng = np.random.default_rng(3)
+    th = base_theta(inst.sys)
+    tan_theta = rng.normal(size=6) * th * 0.1
+    tan_dv = rng.normal(size=(len(pp), 3))
+    out = ctx.window_run(300, checkpoint=64, skin=0.05, w_profile=10.0, profile=ps,
+                         target=target, tan_theta=tan_theta, tan_dv=tan_dv, tangent=True)
+    lhs = out["dloss"][0]
+    rhs = float((out["grad_theta"][0] * tan_theta).sum() + (out["grad_dv"] * tan_dv).sum())
+    print("dual tangent", lhs, "adjoint", rhs, "diag", out["diag"])
+    assert abs(lhs - rhs) <= 1e-9 * max(abs(lhs), abs(rhs))
+    # per-parameter: tangent along each theta alone
+    for k in range(6):
+        e = np.zeros(6)
+        e[k] = th[k] * 1e-2
+        o2 = ctx.window_run(300, checkpoint=64, skin=0.05, w_profile=10.0, profile=ps,
+                            target=target, tan_theta=e, tangent=True, grad=False)
+        want = out["grad_theta"][0][k] * e[k]
+        got = o2["dloss"][0]
+        print(abi.THETA[k], got, want)
+        assert abs(got - want) <= 1e-9 * max(abs(got), abs(want), 1e-30), abi.THETA[k]
+    ctx.close()
+
+
+def fd_agrees(label, g, fds, scale):
+    """Central differences at several step sizes: the best one must agree with the
+    adjoint to FD_RTOL (relative to |g|, or to the largest scaled gradient `scale` when the
+    component is much smaller).  Too small a step hits the double-precision noise floor
+    of the loss, too large a step the curvature."""
+    errs = [abs(fd - g) for _, fd in fds]
+    best = int(np.argmin(errs))
+    rel = errs[best] / max(abs(g), 1e-300)
+    print(f"{label:10s} adjoint {g:+.12e} " +
+          " ".join(f"h={h:.0e}: {fd:+.12e}" for h, fd in fds) + f"  best rel {rel:.2e}")
+    return rel <= FD_RTOL or errs[best] <= FD_RTOL * scale
+
+
+def test_gradients_vs_central_fd_of_host_path(gpu, state):
+    inst, pp, aa, anga, t = state
+    steps = 200
+    ps = profile_spec()
+    target = np.linspace(0.05, 0.01, ps.nbins)
+    ctx = context_at(inst, pp, aa, anga, t)
+    th = base_theta(inst.sys)
+    rng = np.random.default_rng(11)
+    dv = rng.normal(scale=1.0, size=(len(pp), 3))  # N(0, 1 cm/s)
+    out = ctx.window_run(steps, checkpoint=50, skin=0.05, w_profile=10.0, profile=ps,
+                         target=target, theta=th, dv=dv)
+    ctx.close()
+    assert out["diag"]["list_valid"] == 1
+    L0 = oracle_loss(inst, pp, aa, anga, t, steps, th, dv, ps, target)
+    print("loss gpu", out["loss"][0], "host", L0)
+    assert abs(L0 - out["loss"][0]) <= 1e-9 * abs(L0)
+    ok = True
+    gth = out["grad_theta"][0]
+    scale = np.abs(gth * th).max()
+    for k in range(6):
+        fds = []
+        for hr in (1e-6, 1e-5, 1e-4):
+            h = hr * th[k]
+            tp, tm = th.copy(), th.copy()
+            tp[k] += h
+            tm[k] -= h
+            lp = oracle_loss(inst, pp, aa, anga, t, steps, tp, dv, ps, target)
+            lm = oracle_loss(inst, pp, aa, anga, t, steps, tm, dv, ps, target)
+            fds.append((hr, (lp - lm) / (2 * h)))
+        ok &= fd_agrees(abi.THETA[k], gth[k], fds, scale / th[k])
+    gd = out["grad_dv"]
+    for flat in np.argsort(-np.abs(gd).ravel())[:4]:
+        i, a = divmod(int(flat), 3)
+        fds = []
+        for h in (1e-6, 1e-5, 1e-4):  # cm/s
+            dp, dm = dv.copy(), dv.copy()
+            dp[i, a] += h
+            dm[i, a] -= h
+            fds.append((h, (oracle_loss(inst, pp, aa, anga, t, steps, th, dp, ps, target) -
+                            oracle_loss(inst, pp, aa, anga, t, steps, th, dm, ps, target)) / (2 * h)))
+        ok &= fd_agrees(f"dv[{i},{a}]", gd[i, a], fds, np.abs(gd).max())
+    assert ok
+
+
+def test_batch_equals_separate(gpu, state):
+    inst, pp, aa, anga, t = state
+    other = flowing_state(6, n=180)
+    ps = profile_spec()
+    ctx = context_at(inst, pp, aa, anga, t, extra=[other])
+    both = ctx.window_run(120, checkpoint=40, skin=0.05, w_profile=5.0, profile=ps)
+    ctx.close()
+    for k, st in enumerate([(inst, pp, aa, anga, t), other]):
+        c1 = context_at(*st)
+        one = c1.window_run(120, checkpoint=40, skin=0.05, w_profile=5.0, profile=ps)
+        c1.close()
+        assert both["loss"][k] == one["loss"][0]
+        assert np.array_equal(both["grad_theta"][k], one["grad_theta"][0])
+        lo = 0 if k == 0 else len(pp)
+        assert np.array_equal(both["grad_dv"][lo:lo + len(st[1])], one["grad_dv"])
+
+
+def test_window_rejects_unsupported_states(gpu, state):
+    inst, pp, aa, anga, t = state
+    ctx = context_at(inst, pp, aa, anga, t)
+    ctx.set_phase(0, abi.PHASE_FILL, t, inst.sys.side.x)
+    with pytest.raises(dem.AvfError):
+        ctx.window_run(10)
+    ctx.close()
+    funnel = workloads.build(workloads.FlumeSpec(n=100, funnel_width=6.0, funnel_length=4.0))
+    c2 = dem.Context([funnel])
+    c2.step(2)
+    with pytest.raises(dem.AvfError):
+        c2.window_run(10)
+    c2.close()
