@@ -61,6 +61,10 @@ class FlumeSpec:
     jitter: float = 0.005
     seed: int = 1
     dt_frac: float = 50.0
+    # release layer resting on the slope (x' >= 0.2 cm, full width, layer_depth deep)
+    # instead of the lattice in the release box
+    slope_layer: bool = False
+    layer_depth: float = 4.0
 
 
 def build(spec: FlumeSpec) -> dem.Instance:
@@ -117,12 +121,48 @@ def build(spec: FlumeSpec) -> dem.Instance:
         pp["radius"] = spec.rmin
     # per-clump offset 0.75 r: sphere_div is the same for every radius
     dem.calculate_particle_parameters(pp, sys, geo)
-    lo = (0.0, geo.yhop_l, geo.hop_height)
-    hi = (bx, geo.yhop_r, geo.hop_height + bz)
-    dem.place_lattice(pp, geo, lo, hi, spec.spacing, spec.jitter, spec.seed)
+    if spec.slope_layer:
+        place_slope_layer(pp, geo, spec, tilt)
+    else:
+        lo = (0.0, geo.yhop_l, geo.hop_height)
+        hi = (bx, geo.yhop_r, geo.hop_height + bz)
+        dem.place_lattice(pp, geo, lo, hi, spec.spacing, spec.jitter, spec.seed)
     sys.tot_mass = float(pp["mass"].sum())
     return dem.Instance(sys=sys, geo=geo, fun=fun, particles=pp, ntags=9,
                         phase=abi.PHASE_AVALANCHE)
+
+
+def place_slope_layer(pp, geo, spec: FlumeSpec, tilt: float) -> None:
+    """Jittered lattice of randomly oriented clumps in the slope frame (x' down the slope,
+    z' normal to it), resting just above the flume floor, mapped to the lab frame."""
+    a, margin = spec.spacing, spec.jitter + 0.01
+    ext = spec.rmax * (1.0 + math.sqrt(3.0) / geo.sphere_div)  # clump radius
+    ny = int((spec.width - 2 * margin) / a + 1e-9)
+    nz = int(spec.layer_depth / a + 1e-9)
+    rows = math.ceil(spec.n / (ny * nz))
+    z0 = ext + margin - 0.5 * a
+    lo = (0.2, margin, z0)
+    hi = (0.2 + rows * a + 1e-6, margin + ny * a + 1e-6, z0 + nz * a + 1e-6)
+    dem.place_lattice(pp, geo, lo, hi, a, spec.jitter, spec.seed)
+    c, s = math.cos(tilt), math.sin(tilt)
+    q = pp["rr"].copy()
+    pp["rr"][:, 0] = q[:, 0] * c + q[:, 2] * s
+    pp["rr"][:, 2] = q[:, 2] * c - q[:, 0] * s
+    dem.sphere_centres(pp, geo)
+
+
+def slope_release(n: int = 20_000, seed: int = 1, width: float = 10.0, depth: float = 4.0,
+                  mu: float = 0.5, eps: float = 0.5) -> dem.Instance:
+    """A layer of n clumps released from rest on a 35 deg flume 10 cm wide and 1 m long
+    with a flat 50 cm deposit (the differentiable-window workload)."""
+    return build(FlumeSpec(n=n, width=width, length=100.0, box=(0.2, width, 1.0),
+                           deposit_length=50.0, spacing=0.37, jitter=0.005, seed=seed,
+                           slope_layer=True, layer_depth=depth, mu=mu, eps=eps))
+
+
+def adjoint_batch(count: int = 16, n: int = 20_000, seed: int = 3) -> list[dem.Instance]:
+    """The window benchmark: count slope releases of n clumps with different seeds."""
+    return [slope_release(n=n, seed=seed * 100 + i) for i in range(count)]
 
 
 def small(seed: int = 1) -> dem.Instance:

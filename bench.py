@@ -83,6 +83,77 @@ def batch_compare(steps: int, warmup: int = 20) -> dict:
             "speedup": t_seq / t_batch}
 
 
+def adjoint_bench(steps: int, checkpoint: int = 100, count: int = 16, n: int = 20_000,
+                  pre: int = 8000, skin: float = 0.1) -> dict:
+    """Differentiable window: count slope releases of n clumps (35 deg, different seeds)
+    after `pre` plain steps (the layer settles onto the floor and starts to slide), then
+    a steps-long window with checkpoints every `checkpoint` steps.  Loss = mean runout +
+    L2 misfit of a 64-bin deposit profile against the profile the forward window gives at
+    mu = 0.45, eps = 0.55; gradients wrt (mu, eps, kappa) for both contact laws and the
+    per-clump start velocities (dv ~ N(0, 1 cm/s)).  Cost = reverse pass (segment
+    recompute + adjoint steps) / forward window, both from HIP events inside the library;
+    plus central differences of the forward window for three parameters."""
+    import numpy as np
+
+    from avalancheflume3d_amd import abi, dem, workloads
+    insts = workloads.adjoint_batch(count, n)
+    ctx = dem.Context(insts)
+    ctx.regrid()
+    ctx.initial_forces()
+    ctx.step(pre)
+    ctx.sync()
+    ps = abi.ProfileSpec()
+    ps.x0, ps.x1, ps.width, ps.sigma, ps.nbins = 0.0, 40.0, 10.0, 0.0, 64
+    s0 = insts[0].sys
+    base = np.array([[s0.mu_pp, s0.eps_pp, s0.kappa_pp, s0.mu_pw, s0.eps_pw, s0.kappa_pw]] * count)
+    tgt_theta = base.copy()
+    tgt_theta[:, [0, 3]] = 0.45
+    tgt_theta[:, [1, 4]] = 0.55
+    kw = dict(checkpoint=checkpoint, skin=skin, w_runout=1.0, w_profile=1.0, profile=ps)
+    target = ctx.window_run(steps, theta=tgt_theta, grad=False, **kw)["profile"]
+    dv = np.random.default_rng(5).normal(scale=1.0, size=(count * n, 3))
+    out = ctx.window_run(steps, theta=base, dv=dv, target=target, **kw)
+    d = out["diag"]
+    # central differences of the forward window (same frozen list): step sizes from the
+    # 1e-6 relative of the specification up to 1e-3 (small gradients relative to the loss
+    # hit the double-precision noise floor at the smallest steps)
+    fd = {}
+    for k in (1, 3, 4, 5):
+        g = float(out["grad_theta"][0, k])
+        row = {"adjoint": g}
+        best = None
+        for hr in (1e-6, 1e-5, 1e-4, 1e-3):
+            tp, tm = base.copy(), base.copy()
+            h = hr * base[0, k]
+            tp[:, k] += h
+            tm[:, k] -= h
+            lp = ctx.window_run(steps, theta=tp, dv=dv, target=target, grad=False, **kw)["loss"][0]
+            lm = ctx.window_run(steps, theta=tm, dv=dv, target=target, grad=False, **kw)["loss"][0]
+            f = (lp - lm) / (2 * h)
+            rel = abs(f - g) / max(abs(g), 1e-300)
+            row[f"fd_h{hr:.0e}"] = f
+            row[f"rel_h{hr:.0e}"] = rel
+            best = rel if best is None else min(best, rel)
+        row["best_rel"] = best
+        fd[abi.THETA[k]] = row
+    # the plain forward over the same number of steps
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.sync()
+    plain_ms = 1e3 * (time.perf_counter() - t0)
+    ctx.close()
+    return {"instances": count, "clumps": count * n, "window": steps,
+            "checkpoint": checkpoint, "skin_cm": skin, "forward_ms": d["forward_ms"],
+            "backward_ms": d["backward_ms"], "cost_ratio": d["backward_ms"] / d["forward_ms"],
+            "plain_forward_ms": plain_ms,
+            "cost_ratio_vs_plain": d["backward_ms"] / plain_ms,
+            "list_valid": d["list_valid"], "missed_pairs": d["missed"],
+            "max_disp_cm": d["max_disp"], "max_list": d["max_list"],
+            "loss_mean": float(np.mean(out["loss"])),
+            "grad_theta_inst0": out["grad_theta"][0].tolist(),
+            "grad_dv_rms": float(np.sqrt(np.mean(out["grad_dv"] ** 2))), "fd": fd}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +163,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--adjoint-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding of "
@@ -101,6 +174,10 @@ def main() -> int:
     import torch
 
     from avalancheflume3d_amd import dem, workloads
+
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_bench(args.adjoint_steps)}), flush=True)
+        return 0
 
     make = {"headline": workloads.headline, "small": workloads.small, "large": workloads.large}
     inst = make[args.workload]()
@@ -154,6 +231,8 @@ def main() -> int:
     }
     if args.batch_steps > 0:
         out["batched"] = batch_compare(args.batch_steps)
+    if args.adjoint_steps > 0:
+        out["adjoint"] = adjoint_bench(args.adjoint_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(inst, args.cpu_seconds)
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]

@@ -13,7 +13,9 @@ import math
 import numpy as np
 import pytest
 
-from avalancheflume3d_amd import abi, dem, workloads
+import torch
+
+from avalancheflume3d_amd import abi, calibrate, dem, workloads
 from oracle import oracle
 
 pytestmark = pytest.mark.gpu
@@ -253,3 +255,29 @@ def test_window_rejects_unsupported_states(gpu, state):
     with pytest.raises(dem.AvfError):
         c2.window_run(10)
     c2.close()
+
+
+def test_autograd_harness_matches_dual_tangent(gpu, state):
+    inst, pp, aa, anga, t = state
+    ps = profile_spec()
+    target = np.linspace(0.05, 0.01, ps.nbins)
+    ctx = context_at(inst, pp, aa, anga, t)
+    prob = calibrate.WindowProblem(ctx, 150, checkpoint=50, skin=0.05, w_profile=10.0,
+                                   profile=ps, target=target[None, :])
+    ms = prob.sphere_mass
+    kn = inst.sys.kappa_pp * ms / 2.0
+    phys = torch.tensor([0.5, 0.5, kn], dtype=torch.float64, requires_grad=True)
+    dv = torch.zeros(len(pp), 3, dtype=torch.float64)
+    loss = calibrate.window_loss(calibrate.tied_theta(phys, ms), dv, prob).sum()
+    loss.backward()
+    d = np.array([0.01, 0.02, 1e-3 * kn])
+    tan = np.array([d[0], d[1], 2 * d[2] / ms] * 2)
+    out = ctx.window_run(150, checkpoint=50, skin=0.05, w_profile=10.0, profile=ps,
+                         target=target, theta=calibrate.tied_theta(phys.detach(), ms).numpy(),
+                         tan_theta=tan, tangent=True, grad=False)
+    got, want = float(phys.grad.numpy() @ d), out["dloss"][0]
+    print("autograd", got, "dual", want)
+    assert abs(got - want) <= 1e-9 * abs(want)
+    hist = calibrate.calibrate(prob, [0.5, 0.5, kn], iters=2, lr=0.01, log=None)
+    assert len(hist) == 2 and np.isfinite(hist[-1]["loss"])
+    ctx.close()
