@@ -188,10 +188,10 @@ void launch_bin(avf_ctx *c, int mode) {
 }
 
 template <int BS>
-void launch_force_bs(avf_ctx *c, int initial) {
+void launch_force_bs(avf_ctx *c, int initial, bool ft) {
   k_force<kMaxNeigh, BS><<<grid_for(c->n, BS), BS, 0, c->stream>>>(
-      c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_P, c->d_clist, c->d_ccnt, c->d_f, c->d_diag,
-      c->n, c->cap, initial);
+      c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_P, c->d_clist, c->d_ccnt,
+      ft ? c->d_f : nullptr, c->d_diag, c->n, c->cap, initial);
 }
 
 void launch_detect(avf_ctx *c) {
@@ -200,17 +200,18 @@ void launch_detect(avf_ctx *c) {
       c->d_diag, 4 * c->n);
 }
 
-// k_force block shape: 64 (one wavefront) or 128 lanes (avf_options.block_clumps).
-void launch_force(avf_ctx *c, int initial) {
+// k_force block shape: 64 (one wavefront) or 128 lanes (avf_options.block_clumps).  ft:
+// keep the step's forces and torques for avf_get_forces (the last step of a sequence).
+void launch_force(avf_ctx *c, int initial, bool ft = true) {
   if (c->opt.block_clumps >= 128)
-    launch_force_bs<128>(c, initial);
+    launch_force_bs<128>(c, initial, ft);
   else
-    launch_force_bs<kForceBlock>(c, initial);
+    launch_force_bs<kForceBlock>(c, initial, ft);
 }
 
 // One full step: avalanche_hst.cu:669-710 (or :439-467) plus the regrid and cc++.
 // ev (optional, AVF_NKERNELS + 1 events) brackets every kernel for avf_step_profiled.
-void launch_step(avf_ctx *c, hipEvent_t *ev = nullptr) {
+void launch_step(avf_ctx *c, hipEvent_t *ev = nullptr, bool ft = true) {
   auto mark = [&](int k) {
     if (ev) (void)hipEventRecord(ev[k], c->stream);
   };
@@ -232,7 +233,7 @@ void launch_step(avf_ctx *c, hipEvent_t *ev = nullptr) {
   mark(AVF_K_DETECT);
   launch_detect(c);
   mark(AVF_K_FORCE);
-  launch_force(c, 0);
+  launch_force(c, 0, ft);
   mark(AVF_K_CELLREDUCE);
   k_cellreduce<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(c->d_st, c->d_inst, c->d_P,
                                                                   c->d_acc_cell, c->n, c->cap, 0);
@@ -274,7 +275,7 @@ int rebuild_grid(avf_ctx *c, bool reorder) {
 int run_steps(avf_ctx *c, int64_t k) {
   if (k <= 0) return AVF_OK;
   if (!c->opt.use_graph) {
-    for (int64_t i = 0; i < k; i++) launch_step(c);
+    for (int64_t i = 0; i < k; i++) launch_step(c, nullptr, i + 1 == k);
     HIPCHK(c, hipGetLastError());
     return AVF_OK;
   }
@@ -282,7 +283,7 @@ int run_steps(avf_ctx *c, int64_t k) {
   if (it == c->graphs.end()) {
     hipGraph_t g;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int64_t i = 0; i < k; i++) launch_step(c);
+    for (int64_t i = 0; i < k; i++) launch_step(c, nullptr, i + 1 == k);
     HIPCHK(c, hipStreamEndCapture(c->stream, &g));
     hipGraphExec_t ge;
     HIPCHK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
@@ -728,7 +729,7 @@ int avf_step_profiled(avf_ctx *c, int64_t nsteps, double *kernel_ms, double *tot
         HIPCHK(c, hipEventRecord(first, c->stream));
         started = true;
       }
-      launch_step(c, &base[i * kEv]);
+      launch_step(c, &base[i * kEv], queued + i + 1 == nsteps);
       c->steps_since_reorder++;
       c->steps_total++;
       batch_len[cur]++;

@@ -62,6 +62,17 @@ struct DevDiag {
   unsigned long long cell_below, cell_beyond, neigh_overflow, max_neighbours, max_cell, pad;
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8),
+// each with its own L2.  This maps the hardware workgroup index to a logical one so that
+// every XCD processes one contiguous range of logical workgroups: spatially close clumps
+// (the storage order is spatial) then share one L2 instead of being fetched by all eight.
+// A bijection on [0, nb) for any nb.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+  const unsigned q = nb / kXcds, r = nb % kXcds, x = b % kXcds, k = b / kXcds;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
 // Order-preserving int64 image of a double (for integer atomic min / max).
 __device__ __forceinline__ long long order_key(double v) {
   long long b = __double_as_longlong(v);

@@ -69,7 +69,7 @@ __global__ void k_predict(double *__restrict__ st, double *__restrict__ sta,
                           const int32_t *__restrict__ orig, const InstParams *__restrict__ P,
                           uint32_t *__restrict__ counts, uint32_t *__restrict__ skey,
                           uint32_t *__restrict__ srank, int64_t n, int64_t cap, int mode) {
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const InstParams &p = P[inst_of[c]];
   const double t_old = p.time;
@@ -296,7 +296,7 @@ __global__ void k_scatter(const uint32_t *__restrict__ skey, const uint32_t *__r
                           double4 *__restrict__ spos, uint32_t *__restrict__ sid,
                           InstParams *__restrict__ P, int32_t ninst, int64_t nsph,
                           int advance) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (advance && i < ninst) {  // ninst <= clumps < spheres, so every instance is covered
     InstParams &p = P[i];
     p.time = p.time + p.dt;  // avalanche_hst.cu:672 (sys_par.time += sys_par.dt)
@@ -528,7 +528,7 @@ __global__ __launch_bounds__(256) void k_detect(
     const double4 *__restrict__ spos, const uint32_t *__restrict__ sid,
     uint32_t *__restrict__ clist, int32_t *__restrict__ ccnt, DevDiag *__restrict__ diag,
     int64_t nsph) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= nsph) return;
   const int64_t c = i >> 2;
   const int s = (int)(i & 3);
@@ -597,7 +597,7 @@ __global__ __launch_bounds__(BS) void k_force(
     double *__restrict__ fout, DevDiag *__restrict__ diag, int64_t n, int64_t cap, int initial) {
   __shared__ PPShared<MAXN, BS> sh;
   const int lane = threadIdx.x;
-  const int64_t c = (int64_t)blockIdx.x * BS + lane;
+  const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * BS + lane;
   if (c >= n) return;
   const InstParams &p = P[inst_of[c]];
   const double time = p.time;
@@ -751,12 +751,14 @@ __global__ __launch_bounds__(BS) void k_force(
     if (mods & AVF_MOD_DEP) deposit_floor(p, rr, vv, ww, sph, rad, massw, ff, tq);
   }
 
-  fout[F_FX * cap + c] = ff.x;
-  fout[F_FY * cap + c] = ff.y;
-  fout[F_FZ * cap + c] = ff.z;
-  fout[F_TX * cap + c] = tq.x;
-  fout[F_TY * cap + c] = tq.y;
-  fout[F_TZ * cap + c] = tq.z;
+  if (fout) {  // F / T of the step, kept for export (last step of a launch sequence)
+    fout[F_FX * cap + c] = ff.x;
+    fout[F_FY * cap + c] = ff.y;
+    fout[F_FZ * cap + c] = ff.z;
+    fout[F_TX * cap + c] = tq.x;
+    fout[F_TY * cap + c] = tq.y;
+    fout[F_TZ * cap + c] = tq.z;
+  }
 
   if (frz) {
     if (!initial) {  // finish_Verlet.cu:28-32

@@ -107,7 +107,7 @@ __global__ __launch_bounds__(64) void k_win_list(
   __shared__ uint32_t nk[kWinMaxNeigh][64];
   __shared__ uint16_t nmask[kWinMaxNeigh][64];
   const int lane = threadIdx.x;
-  const int64_t c = (int64_t)blockIdx.x * 64 + lane;
+  const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 64 + lane;
   if (c >= n) return;
   const InstParams &p = P[inst_of[c]];
   const double *gl = stage + c * kStageDoubles;
@@ -254,7 +254,7 @@ __global__ void k_win_missed(const double *__restrict__ stage, const InstParams 
                              const double4 *__restrict__ spos, const uint32_t *__restrict__ sid,
                              const int32_t *__restrict__ list, const int32_t *__restrict__ cnt,
                              unsigned long long *__restrict__ missed, int64_t nsph) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= nsph) return;
   const int64_t c = i >> 2;
   const int s = (int)(i & 3);
@@ -365,7 +365,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_win_predict(const T *__restrict__ st, const double *__restrict__ sta,
                               const InstParams *__restrict__ P, const int32_t *__restrict__ inst_of,
                               T *__restrict__ stage, int64_t n, int64_t cap) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const InstParams &p = P[inst_of[c]];
   T S[S_NFIELDS];
@@ -515,7 +515,7 @@ __global__ __launch_bounds__(64) void k_win_force(
     const int32_t *__restrict__ list, const int32_t *__restrict__ cnt, T *__restrict__ tq_out,
     const double *__restrict__ sph0, double *__restrict__ dmax2, WinDiag *__restrict__ wd,
     int64_t n, int64_t cap) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const int inst = inst_of[c];
   const InstParams &p = P[inst];
@@ -664,7 +664,7 @@ __global__ __launch_bounds__(64) void k_win_adj_force(
     const int32_t *__restrict__ inst_of, const int32_t *__restrict__ list,
     const int32_t *__restrict__ cnt, double *__restrict__ sbar, double *__restrict__ ebuf,
     uint8_t *__restrict__ eflag, double *__restrict__ tbar, int64_t n, int64_t cap) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const int inst = inst_of[c];
   const InstParams &p = P[inst];
@@ -948,7 +948,7 @@ __global__ __launch_bounds__(64) void k_win_adj_gather(const double *__restrict_
                                  const int32_t *__restrict__ list, const int32_t *__restrict__ in,
                                  const int32_t *__restrict__ icnt, const InstParams *__restrict__ P,
                                  const int32_t *__restrict__ inst_of, int64_t n, int64_t cap) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const InstParams &p = P[inst_of[c]];
   const double dt = p.dt, hdt = p.hdt;

@@ -165,6 +165,9 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-only", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a device-to-device copy kernel of this many bytes 5 times "
+                         "(calibrates the HBM byte counters against a known traffic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding of "
@@ -172,6 +175,15 @@ def main() -> int:
         return 2
 
     import torch
+
+    if args.copy_probe:
+        x = torch.ones(args.copy_probe // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "launches": 5}), flush=True)
+        return 0
 
     from avalancheflume3d_amd import dem, workloads
 
