@@ -153,7 +153,7 @@ typedef enum avf_status {
   AVF_ERR_CELL_FULL = 5,    /* a clump cell holds ntags-1 clumps (cells.cu:96-102)      */
   AVF_ERR_NEIGH_OVERFLOW = 6,/* a clump touched more clumps than max_neighbours          */
   AVF_ERR_NOFIT = 7,        /* lattice does not fit the box (place_particles.cu:58-62)  */
-  AVF_ERR_INPUT = 8         /* malformed DataInput.dat / checkpoint                     */
+  AVF_ERR_INPUT = 8         /* malformed DataInput.dat / checkpoint; unsupported state  */
 } avf_status;
 
 /* Diagnostics accumulated by the device since the last avf_get_diag(). */
