@@ -91,7 +91,7 @@ class Options(C.Structure):
     _fields_ = [("reorder_interval", C.c_int32), ("use_graph", C.c_int32),
                 ("sphere_cell_factor", C.c_double), ("block_clumps", C.c_int32),
                 ("max_neighbours", C.c_int32), ("device", C.c_int32), ("reserved", C.c_int32),
-                ("stream", C.c_void_p)]
+                ("stream", C.c_void_p), ("verlet_skin", C.c_double)]
 
 
 class Diag(C.Structure):
@@ -212,6 +212,7 @@ def load_library() -> C.CDLL:
             "avf_get_diag": (C.c_int, [vp, vp]),
             "avf_sync": (C.c_int, [vp]),
             "avf_step_profiled": (C.c_int, [vp, i64, vp, vp]),
+            "avf_step_timed": (C.c_int, [vp, i64, i32, vp, vp]),
             "avf_observe": (C.c_int, [vp, i32, vp, vp, vp]),
             "avf_window_run": (C.c_int, [vp] * 15),
         }
@@ -232,7 +233,7 @@ EXPORTED = [
     "avf_last_error", "avf_upload", "avf_download", "avf_set_phase", "avf_get_time",
     "avf_regrid", "avf_get_cell", "avf_initial_forces", "avf_step", "avf_get_forces",
     "avf_get_clump_cells", "avf_check_cells", "avf_get_contacts", "avf_get_diag", "avf_sync",
-    "avf_step_profiled", "avf_observe", "avf_window_run",
+    "avf_step_profiled", "avf_step_timed", "avf_observe", "avf_window_run",
 ]
 KERNELS = ["predict", "scan_a", "scan_b", "scan_c", "scatter", "detect", "force", "cellreduce",
            "finalize"]

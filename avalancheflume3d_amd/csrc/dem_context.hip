@@ -53,6 +53,18 @@ struct avf_ctx {
   uint32_t *d_skey = nullptr, *d_srank = nullptr, *d_sid = nullptr, *d_clist = nullptr;
   int32_t *d_ccnt = nullptr;
   double4 *d_spos = nullptr;
+  // candidate lists (verlet_skin > 0): per clump up to kVMax neighbour clumps with their
+  // candidate sphere pairings (entry-major), displacement tracking, chunk checkpoint
+  int32_t *d_vl_m = nullptr, *d_vl_cnt = nullptr;
+  uint16_t *d_vl_mask = nullptr;
+  double *d_xb = nullptr, *d_vrot = nullptr, *d_dsp = nullptr;
+  double *d_ck_st = nullptr, *d_ck_rad = nullptr;
+  InstParams *d_ck_P = nullptr;
+  DevDiag *d_ck_diag = nullptr;
+  unsigned long long *d_umax = nullptr;
+  bool list_valid = false;
+  int64_t vchunk = 8;             // next chunk of candidate-list steps
+  int64_t since_build = 0;        // steps since the last list build
   long long *d_acc_cell = nullptr, *d_acc_sg = nullptr;
   DevDiag *d_diag = nullptr;
   unsigned long long *d_count = nullptr;
@@ -65,6 +77,7 @@ struct avf_ctx {
   std::map<int64_t, hipGraphExec_t> graphs;
   std::vector<hipGraph_t> graph_defs;
   struct WinBuf *win = nullptr;  // buffers of the differentiable window (on first use)
+  int32_t prof_kernel = -1;      // avf_step_timed: the one kernel bracketed by events
 };
 
 namespace {
@@ -178,48 +191,67 @@ void launch_scan(avf_ctx *c) {
 
 // Binning of the staged spheres: predict (or stage), scan, scatter.
 void launch_bin(avf_ctx *c, int mode) {
-  k_predict<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(
+  k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
       c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-      c->d_srank, c->n, c->cap, mode);
+      c->d_srank, c->n, c->cap, mode, ListTrack{nullptr, nullptr, nullptr});
   launch_scan(c);
   k_scatter<<<std::max(1u, grid_for(4 * c->n, kBlock)), kBlock, 0, c->stream>>>(
       c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid, c->d_P, c->ninst,
       4 * c->n, mode);
 }
 
-template <int BS>
-void launch_force_bs(avf_ctx *c, int initial, bool ft) {
-  k_force<kMaxNeigh, BS><<<grid_for(c->n, BS), BS, 0, c->stream>>>(
-      c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_P, c->d_clist, c->d_ccnt,
-      ft ? c->d_f : nullptr, c->d_diag, c->n, c->cap, initial);
+template <int BS, bool VERLET>
+void launch_force_bs(avf_ctx *c, int initial, bool ft, int time_ahead) {
+  k_force<kMaxNeigh, BS, VERLET><<<grid_for(c->n, BS), BS, 0, c->stream>>>(
+      c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_P, c->d_clist, c->d_ccnt, c->d_vl_m,
+      c->d_vl_mask, c->d_vl_cnt, ft ? c->d_f : nullptr, c->d_diag, c->n, c->cap, initial,
+      time_ahead);
 }
 
-void launch_detect(avf_ctx *c) {
-  k_detect<<<grid_for(4 * c->n, kBlock), kBlock, 0, c->stream>>>(
-      c->d_stage, c->d_P, c->d_inst, c->d_starts, c->d_spos, c->d_sid, c->d_clist, c->d_ccnt,
-      c->d_diag, 4 * c->n);
+// Touching spheres of the staged spheres (skin 0) or candidate spheres within sep + skin.
+void launch_detect(avf_ctx *c, double skin = 0.0) {
+  if (skin > 0.0)
+    k_detect<kSphListV><<<grid_for(4 * c->n, kBlock), kBlock, 0, c->stream>>>(
+        c->d_stage, c->d_P, c->d_inst, c->d_starts, c->d_spos, c->d_sid, c->d_clist, c->d_ccnt,
+        c->d_diag, 4 * c->n, skin);
+  else
+    k_detect<kSphList><<<grid_for(4 * c->n, kBlock), kBlock, 0, c->stream>>>(
+        c->d_stage, c->d_P, c->d_inst, c->d_starts, c->d_spos, c->d_sid, c->d_clist, c->d_ccnt,
+        c->d_diag, 4 * c->n, 0.0);
 }
 
 // k_force block shape: 64 (one wavefront) or 128 lanes (avf_options.block_clumps).  ft:
 // keep the step's forces and torques for avf_get_forces (the last step of a sequence).
-void launch_force(avf_ctx *c, int initial, bool ft = true) {
-  if (c->opt.block_clumps >= 128)
-    launch_force_bs<128>(c, initial, ft);
-  else
-    launch_force_bs<kForceBlock>(c, initial, ft);
+// verlet: touching clumps from the candidate lists; time_ahead: see k_force.
+void launch_force(avf_ctx *c, int initial, bool ft = true, bool verlet = false,
+                  int time_ahead = 0) {
+  if (c->opt.block_clumps >= 128) {
+    if (verlet)
+      launch_force_bs<128, true>(c, initial, ft, time_ahead);
+    else
+      launch_force_bs<128, false>(c, initial, ft, time_ahead);
+  } else {
+    if (verlet)
+      launch_force_bs<kForceBlock, true>(c, initial, ft, time_ahead);
+    else
+      launch_force_bs<kForceBlock, false>(c, initial, ft, time_ahead);
+  }
 }
+
+ListTrack list_track(avf_ctx *c) { return ListTrack{c->d_xb, c->d_vrot, c->d_dsp}; }
 
 // One full step: avalanche_hst.cu:669-710 (or :439-467) plus the regrid and cc++.
 // ev (optional, AVF_NKERNELS + 1 events) brackets every kernel for avf_step_profiled.
 void launch_step(avf_ctx *c, hipEvent_t *ev = nullptr, bool ft = true) {
   auto mark = [&](int k) {
-    if (ev) (void)hipEventRecord(ev[k], c->stream);
+    if (ev && (c->prof_kernel < 0 || k == c->prof_kernel || k == c->prof_kernel + 1))
+      (void)hipEventRecord(ev[k], c->stream);
   };
   const unsigned nb = grid_for(c->cell_cap, kScanTile);
   mark(AVF_K_PREDICT);
-  k_predict<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(
+  k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
       c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-      c->d_srank, c->n, c->cap, 1);
+      c->d_srank, c->n, c->cap, 1, ListTrack{nullptr, nullptr, nullptr});
   mark(AVF_K_SCAN_A);
   k_scan_a<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
   mark(AVF_K_SCAN_B);
@@ -236,9 +268,56 @@ void launch_step(avf_ctx *c, hipEvent_t *ev = nullptr, bool ft = true) {
   launch_force(c, 0, ft);
   mark(AVF_K_CELLREDUCE);
   k_cellreduce<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(c->d_st, c->d_inst, c->d_P,
-                                                                  c->d_acc_cell, c->n, c->cap, 0);
+                                                                  c->d_acc_cell, c->n, c->cap, 0,
+                                                                  c->ninst);
   mark(AVF_K_FINALIZE);
   k_finalize<<<grid_for(c->ninst, 64), 64, 0, c->stream>>>(c->d_P, c->d_acc_cell, c->ninst, 0, 1);
+  mark(AVF_NKERNELS);
+}
+
+// One step on candidate lists.  rebuild: bin the predicted spheres and rebuild the lists
+// (k_predict keys, scans, scatter advancing P.time, k_detect with the skin, k_vmerge), then
+// the force stage; otherwise k_predict without keys (tracking displacements) and the force
+// stage with P.time + dt, P.time advanced by k_finalize.  The kernel slots of
+// avf_step_profiled are kept (the binning slots are empty on list steps).
+void launch_vstep(avf_ctx *c, bool rebuild, hipEvent_t *ev = nullptr, bool ft = true) {
+  auto mark = [&](int k) {
+    if (ev && (c->prof_kernel < 0 || k == c->prof_kernel || k == c->prof_kernel + 1))
+      (void)hipEventRecord(ev[k], c->stream);
+  };
+  const unsigned nb = grid_for(c->cell_cap, kScanTile);
+  mark(AVF_K_PREDICT);
+  k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
+      c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
+      c->d_srank, c->n, c->cap, rebuild ? 1 : 2, list_track(c));
+  mark(AVF_K_SCAN_A);
+  if (rebuild) k_scan_a<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
+  mark(AVF_K_SCAN_B);
+  if (rebuild) k_scan_b<<<1, 1024, 0, c->stream>>>(c->d_bsum, c->d_ncells, c->d_starts);
+  mark(AVF_K_SCAN_C);
+  if (rebuild)
+    k_scan_c<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_starts, c->d_ncells);
+  mark(AVF_K_SCATTER);
+  if (rebuild)
+    k_scatter<<<std::max(1u, grid_for(4 * c->n, kBlock)), kBlock, 0, c->stream>>>(
+        c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid, c->d_P, c->ninst,
+        4 * c->n, 1);
+  mark(AVF_K_DETECT);
+  if (rebuild) {
+    launch_detect(c, c->opt.verlet_skin);
+    k_vmerge<kForceBlock><<<grid_for(c->n, kForceBlock), kForceBlock, 0, c->stream>>>(
+        c->d_stage, c->d_P, c->d_inst, c->d_clist, c->d_ccnt, c->d_vl_m, c->d_vl_mask,
+        c->d_vl_cnt, list_track(c), c->d_diag, c->n, c->cap);
+  }
+  mark(AVF_K_FORCE);
+  launch_force(c, 0, ft, true, rebuild ? 0 : 1);
+  mark(AVF_K_CELLREDUCE);
+  k_cellreduce<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(c->d_st, c->d_inst, c->d_P,
+                                                                  c->d_acc_cell, c->n, c->cap, 0,
+                                                                  c->ninst);
+  mark(AVF_K_FINALIZE);
+  k_finalize<<<grid_for(c->ninst, 64), 64, 0, c->stream>>>(c->d_P, c->d_acc_cell, c->ninst, 0, 1,
+                                                           rebuild ? 0 : 1);
   mark(AVF_NKERNELS);
 }
 
@@ -249,7 +328,8 @@ int rebuild_grid(avf_ctx *c, bool reorder) {
                                                                     c->d_acc_sg, c->n, c->cap);
   const double horizon = 2.0 * std::max(1, c->opt.reorder_interval);
   k_sgrid_finalize<<<1, 1, 0, c->stream>>>(c->d_P, c->d_acc_sg, c->ninst, c->cell_cap,
-                                           c->opt.sphere_cell_factor, horizon, c->d_ncells);
+                                           c->opt.sphere_cell_factor, horizon, c->d_ncells,
+                                           c->opt.verlet_skin);
   if (reorder) {
     k_reorder_key<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(
         c->d_st, c->d_inst, c->d_P, c->d_counts, c->d_skey, c->d_srank, c->n, c->cap);
@@ -268,6 +348,7 @@ int rebuild_grid(avf_ctx *c, bool reorder) {
   }
   HIPCHK(c, hipGetLastError());
   c->sgrid_valid = true;
+  c->list_valid = false;  // storage order (and the sphere grid) changed
   c->steps_since_reorder = 0;
   return AVF_OK;
 }
@@ -291,6 +372,139 @@ int run_steps(avf_ctx *c, int64_t k) {
     it = c->graphs.emplace(k, ge).first;
   }
   HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+  return AVF_OK;
+}
+
+// Per-kernel event sums of avf_step_profiled.
+struct StepProf {
+  double sum[AVF_NKERNELS] = {0};
+  int64_t launched = 0;
+};
+
+// S candidate-list (or one rebuild) steps launched eagerly with events between kernels.
+int vsteps_profiled(avf_ctx *c, int64_t S, bool rebuild, StepProf *prof) {
+  constexpr int kEv = AVF_NKERNELS + 1;
+  std::vector<hipEvent_t> ev((size_t)S * kEv, nullptr);
+  int rc = AVF_OK;
+  for (auto &e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      rc = fail(c, AVF_ERR_HIP, "hipEventCreate failed");
+      break;
+    }
+  if (rc == AVF_OK) {
+    for (int64_t i = 0; i < S; i++) launch_vstep(c, rebuild, &ev[(size_t)i * kEv], i + 1 == S);
+    const int jlast = c->prof_kernel >= 0 ? c->prof_kernel + 1 : AVF_NKERNELS;
+    hipError_t he = hipEventSynchronize(ev[(size_t)(S - 1) * kEv + jlast]);
+    for (int64_t i = 0; i < S && he == hipSuccess; i++)
+      for (int j = 0; j < AVF_NKERNELS && he == hipSuccess; j++) {
+        if (c->prof_kernel >= 0 && j != c->prof_kernel) continue;
+        float ms = 0.f;
+        he = hipEventElapsedTime(&ms, ev[(size_t)i * kEv + j], ev[(size_t)i * kEv + j + 1]);
+        prof->sum[j] += ms;
+      }
+    if (he != hipSuccess) rc = fail(c, AVF_ERR_HIP, "event timing: %s", hipGetErrorString(he));
+    prof->launched += S;
+  }
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+  return rc;
+}
+
+// Steps on candidate lists (avf_options.verlet_skin > 0).  A rebuild step bins the spheres
+// and lists every sphere pair within sep + skin; chunks of list steps follow, each from a
+// checkpoint of the state.  After a chunk the largest displacement bound of any sphere since
+// the build decides: beyond 0.49 skin a pair outside the list could have touched, so the
+// chunk is undone from its checkpoint and redone after a rebuild; otherwise the next chunk
+// length is set from the bound's growth.  The results are those of per-step binning.
+int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
+  constexpr int64_t kMaxChunk = 64;
+  const double skin = c->opt.verlet_skin;
+  const size_t SF = (size_t)c->cap * S_NFIELDS * sizeof(double);
+  int rc;
+  int64_t left = nsteps;
+  while (left > 0) {
+    if (!c->sgrid_valid ||
+        (c->opt.reorder_interval > 0 && c->steps_since_reorder >= c->opt.reorder_interval)) {
+      if ((rc = rebuild_grid(c, c->opt.reorder_interval > 0))) return rc;
+    }
+    if (!c->list_valid) {
+      if (prof) {
+        if ((rc = vsteps_profiled(c, 1, true, prof))) return rc;
+      } else {
+        launch_vstep(c, true);
+      }
+      c->list_valid = true;
+      c->binned = true;
+      c->since_build = 0;
+      c->steps_since_reorder++;
+      c->steps_total++;
+      left--;
+      continue;
+    }
+    int64_t S = std::min(left, std::max<int64_t>(1, c->vchunk));
+    if (c->opt.reorder_interval > 0)
+      S = std::min<int64_t>(S, c->opt.reorder_interval - c->steps_since_reorder);
+    // checkpoint: state, radii (clumps may freeze), parameters (time, cc, grids), counters
+    HIPCHK(c, hipMemcpyAsync(c->d_ck_st, c->d_st, SF, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_ck_rad, c->d_sta + (size_t)C_RAD * c->cap,
+                             c->cap * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_ck_P, c->d_P, c->ninst * sizeof(InstParams),
+                             hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_ck_diag, c->d_diag, sizeof(DevDiag), hipMemcpyDeviceToDevice,
+                             c->stream));
+    if (prof) {
+      if ((rc = vsteps_profiled(c, S, false, prof))) return rc;
+    } else if (c->opt.use_graph) {
+      const int64_t key = -S;  // list-step chunks (plain chunks use positive keys)
+      auto it = c->graphs.find(key);
+      if (it == c->graphs.end()) {
+        hipGraph_t g;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int64_t i = 0; i < S; i++) launch_vstep(c, false, nullptr, i + 1 == S);
+        HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+        hipGraphExec_t ge;
+        HIPCHK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        c->graph_defs.push_back(g);
+        it = c->graphs.emplace(key, ge).first;
+      }
+      HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+    } else {
+      for (int64_t i = 0; i < S; i++) launch_vstep(c, false, nullptr, i + 1 == S);
+    }
+    // largest displacement bound since the build
+    unsigned long long um = 0;
+    HIPCHK(c, hipMemsetAsync(c->d_umax, 0, sizeof um, c->stream));
+    k_win_max<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(c->d_dsp, c->n, c->d_umax);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(&um, c->d_umax, sizeof um, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double maxd;
+    std::memcpy(&maxd, &um, sizeof maxd);
+    if (maxd > 0.49 * skin) {
+      HIPCHK(c, hipMemcpyAsync(c->d_st, c->d_ck_st, SF, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->d_sta + (size_t)C_RAD * c->cap, c->d_ck_rad,
+                               c->cap * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->d_P, c->d_ck_P, c->ninst * sizeof(InstParams),
+                               hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->d_diag, c->d_ck_diag, sizeof(DevDiag), hipMemcpyDeviceToDevice,
+                               c->stream));
+      c->list_valid = false;
+      c->vchunk = std::max<int64_t>(1, S / 2);
+      continue;
+    }
+    left -= S;
+    c->steps_since_reorder += S;
+    c->steps_total += S;
+    c->since_build += S;
+    c->binned = false;
+    const double rate = maxd / (double)c->since_build, room = 0.49 * skin - maxd;
+    int64_t next = rate > 0.0 ? (int64_t)(0.8 * room / rate) : kMaxChunk;
+    if (next < 1) {
+      c->list_valid = false;
+      next = 1;
+    }
+    c->vchunk = std::min(next, kMaxChunk);
+  }
   return AVF_OK;
 }
 
@@ -331,6 +545,7 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
   else
     avf_options_default(&c->opt);
   if (c->opt.sphere_cell_factor < 1.0) c->opt.sphere_cell_factor = 1.0;
+  if (!(c->opt.verlet_skin > 0.0)) c->opt.verlet_skin = 0.0;
   c->opt.block_clumps = c->opt.block_clumps >= 128 ? 128 : 64;
   c->device = c->opt.device;
   c->ninst = ninst;
@@ -389,9 +604,18 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
       (rc = dalloc(c, &c->d_bsum, grid_for(c->cell_cap, kScanTile) + 1)) ||
       (rc = dalloc(c, &c->d_skey, 4 * N)) || (rc = dalloc(c, &c->d_srank, 4 * N)) ||
       (rc = dalloc(c, &c->d_sid, 4 * N)) || (rc = dalloc(c, &c->d_spos, 4 * N)) ||
-      (rc = dalloc(c, &c->d_clist, 4 * N * kSphList)) || (rc = dalloc(c, &c->d_ccnt, 4 * N)) ||
+      (rc = dalloc(c, &c->d_clist, 4 * N * (c->opt.verlet_skin > 0.0 ? kSphListV : kSphList))) ||
+      (rc = dalloc(c, &c->d_ccnt, 4 * N)) ||
       (rc = dalloc(c, &c->d_acc_cell, 8 * ninst)) || (rc = dalloc(c, &c->d_acc_sg, 8 * ninst)) ||
       (rc = dalloc(c, &c->d_diag, 1)) || (rc = dalloc(c, &c->d_count, 1)))
+    return bail(rc);
+  if (c->opt.verlet_skin > 0.0 &&
+      ((rc = dalloc(c, &c->d_vl_m, kVMax * N)) || (rc = dalloc(c, &c->d_vl_mask, kVMax * N)) ||
+       (rc = dalloc(c, &c->d_vl_cnt, N)) || (rc = dalloc(c, &c->d_xb, 3 * N)) ||
+       (rc = dalloc(c, &c->d_vrot, N)) || (rc = dalloc(c, &c->d_dsp, N)) ||
+       (rc = dalloc(c, &c->d_ck_st, N * S_NFIELDS)) || (rc = dalloc(c, &c->d_ck_rad, N)) ||
+       (rc = dalloc(c, &c->d_ck_P, ninst)) || (rc = dalloc(c, &c->d_ck_diag, 1)) ||
+       (rc = dalloc(c, &c->d_umax, 1))))
     return bail(rc);
 
   // Parameters, identity storage order, zeroed tables.
@@ -492,7 +716,9 @@ int avf_destroy(avf_ctx *c) {
                   c->d_f,      c->d_io,    c->d_io2,   c->d_orig,  c->d_orig2, c->d_inst,
                   c->d_inst2,  c->d_inv,   c->d_ncells, c->d_tmp64, c->d_counts, c->d_starts,
                   c->d_bsum,   c->d_skey,  c->d_srank, c->d_sid,   c->d_spos,  c->d_acc_cell,
-                  c->d_acc_sg, c->d_diag,  c->d_count, c->d_clist, c->d_ccnt};
+                  c->d_acc_sg, c->d_diag,  c->d_count, c->d_clist, c->d_ccnt,
+                  c->d_vl_m,   c->d_vl_cnt, c->d_vl_mask, c->d_xb, c->d_vrot, c->d_dsp,
+                  c->d_ck_st,  c->d_ck_rad, c->d_ck_P, c->d_ck_diag, c->d_umax};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   free_window(c);
@@ -549,6 +775,7 @@ int avf_upload(avf_ctx *c, int32_t inst, const avf_particle *pp, const avf_doubl
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->sgrid_valid = false;
   c->binned = false;
+  c->list_valid = false;
   return AVF_OK;
 }
 
@@ -630,7 +857,8 @@ int avf_get_time(avf_ctx *c, int32_t inst, double *time, int64_t *cc) {
 int avf_regrid(avf_ctx *c) {
   if (!c) return AVF_ERR_ARG;
   k_cellreduce<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(c->d_st, c->d_inst, c->d_P,
-                                                                  c->d_acc_cell, c->n, c->cap, 1);
+                                                                  c->d_acc_cell, c->n, c->cap, 1,
+                                                                  c->ninst);
   k_finalize<<<grid_for(c->ninst, 64), 64, 0, c->stream>>>(c->d_P, c->d_acc_cell, c->ninst, 1, 0);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -663,6 +891,7 @@ int avf_initial_forces(avf_ctx *c) {
   launch_detect(c);
   launch_force(c, 1);
   c->binned = true;
+  c->list_valid = false;
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return AVF_OK;
@@ -670,6 +899,7 @@ int avf_initial_forces(avf_ctx *c) {
 
 int avf_step(avf_ctx *c, int64_t nsteps) {
   if (!c || nsteps < 0) return AVF_ERR_ARG;
+  if (c->opt.verlet_skin > 0.0) return run_verlet(c, nsteps, nullptr);
   int rc;
   const int64_t chunk_max = c->opt.reorder_interval > 0 ? c->opt.reorder_interval : 1000;
   int64_t left = nsteps;
@@ -690,6 +920,24 @@ int avf_step(avf_ctx *c, int64_t nsteps) {
 
 int avf_step_profiled(avf_ctx *c, int64_t nsteps, double *kernel_ms, double *total_ms) {
   if (!c || nsteps < 0) return AVF_ERR_ARG;
+  if (c->opt.verlet_skin > 0.0) {
+    hipEvent_t first, last;
+    HIPCHK(c, hipEventCreate(&first));
+    HIPCHK(c, hipEventCreate(&last));
+    HIPCHK(c, hipEventRecord(first, c->stream));
+    StepProf prof;
+    int rc = run_verlet(c, nsteps, &prof);
+    HIPCHK(c, hipEventRecord(last, c->stream));
+    HIPCHK(c, hipEventSynchronize(last));
+    float tot = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&tot, first, last));
+    (void)hipEventDestroy(first);
+    (void)hipEventDestroy(last);
+    for (int j = 0; j < AVF_NKERNELS; j++)
+      if (kernel_ms) kernel_ms[j] = prof.launched ? prof.sum[j] / (double)prof.launched : 0.0;
+    if (total_ms) *total_ms = tot;
+    return rc;
+  }
   // Two batches of events: batch b+1 is queued before batch b is read, so reading the
   // timings never leaves the GPU idle.
   constexpr int kBatch = 128, kEv = AVF_NKERNELS + 1;
@@ -707,9 +955,11 @@ int avf_step_profiled(avf_ctx *c, int64_t nsteps, double *kernel_ms, double *tot
   auto read_batch = [&](int b) -> int {
     if (batch_len[b] == 0) return AVF_OK;
     hipEvent_t *base = &ev[(size_t)b * kBatch * kEv];
-    HIPCHK(c, hipEventSynchronize(base[(batch_len[b] - 1) * kEv + AVF_NKERNELS]));
+    const int jlast = c->prof_kernel >= 0 ? c->prof_kernel + 1 : AVF_NKERNELS;
+    HIPCHK(c, hipEventSynchronize(base[(batch_len[b] - 1) * kEv + jlast]));
     for (int64_t i = 0; i < batch_len[b]; i++)
       for (int j = 0; j < AVF_NKERNELS; j++) {
+        if (c->prof_kernel >= 0 && j != c->prof_kernel) continue;
         float ms = 0.f;
         HIPCHK(c, hipEventElapsedTime(&ms, base[i * kEv + j], base[i * kEv + j + 1]));
         sum[j] += ms;
@@ -751,6 +1001,17 @@ int avf_step_profiled(avf_ctx *c, int64_t nsteps, double *kernel_ms, double *tot
   for (int j = 0; j < AVF_NKERNELS; j++)
     if (kernel_ms) kernel_ms[j] = nsteps ? sum[j] / (double)nsteps : 0.0;
   if (total_ms) *total_ms = tot;
+  return rc;
+}
+
+int avf_step_timed(avf_ctx *c, int64_t nsteps, int32_t kernel, double *kernel_ms,
+                   double *total_ms) {
+  if (!c || nsteps < 0 || kernel < 0 || kernel >= AVF_NKERNELS) return AVF_ERR_ARG;
+  std::vector<double> all(AVF_NKERNELS, 0.0);
+  c->prof_kernel = kernel;
+  const int rc = avf_step_profiled(c, nsteps, all.data(), total_ms);
+  c->prof_kernel = -1;
+  if (kernel_ms) *kernel_ms = all[kernel];
   return rc;
 }
 
@@ -1061,9 +1322,9 @@ void win_step(avf_ctx *c, const InstParams *P, const double *cur, double *out, d
 void win_bin(avf_ctx *c, const double *st) {
   WinBuf *w = c->win;
   const unsigned nb = grid_for(c->cell_cap, kScanTile);
-  k_predict<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(
+  k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
       const_cast<double *>(st), c->d_sta, w->bstage, c->d_inst, c->d_orig, c->d_P, c->d_counts,
-      w->bkey, w->brank, c->n, c->cap, 0);
+      w->bkey, w->brank, c->n, c->cap, 0, ListTrack{nullptr, nullptr, nullptr});
   k_scan_a<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
   k_scan_b<<<1, 1024, 0, c->stream>>>(c->d_bsum, c->d_ncells, w->bstarts);
   k_scan_c<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, w->bstarts, c->d_ncells);
@@ -1251,7 +1512,7 @@ extern "C" int avf_window_run(avf_ctx *c, const avf_window_spec *spec, const dou
     InstParams *Pt = w->allP + (size_t)t * ni;
     win_step(c, Pt, cur, out, w->segstage, nullptr, w->sph0, w->dmax2, w->wd);
     k_cellreduce<<<grid_for(n, kBlock), kBlock, 0, c->stream>>>(out, c->d_inst, Pt, w->acc, n, cap,
-                                                                0);
+                                                                0, ni);
     k_win_finalize<<<grid_for(ni, 64), 64, 0, c->stream>>>(Pt, Pt + ni, w->acc, ni);
     cur = out;
     if ((t + 1) % K == 0 || t + 1 == W) win_check_list(c, cur);

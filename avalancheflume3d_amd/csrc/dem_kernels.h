@@ -73,6 +73,13 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
 }
 
+// Running maximum of a diagnostic counter without contending on one address: a lane only
+// issues the atomic when its value exceeds what the counter already holds (after the first
+// steps almost no lane does).
+__device__ __forceinline__ void diag_max(unsigned long long *addr, unsigned long long v) {
+  if (v > __atomic_load_n(addr, __ATOMIC_RELAXED)) atomicMax(addr, v);
+}
+
 // Order-preserving int64 image of a double (for integer atomic min / max).
 __device__ __forceinline__ long long order_key(double v) {
   long long b = __double_as_longlong(v);

@@ -62,16 +62,24 @@ __device__ __forceinline__ d3 pick(const d3 *s, int i) {
 
 // --------------------------------------------------------------------- k_predict ----
 
-// mode 1: predict (init_Verlet) then stage; mode 0: stage the current state only (initial
-// force pass, avalanche_hst.cu:365-384, whose force stage sees the unchanged time).
-__global__ void k_predict(double *__restrict__ st, double *__restrict__ sta,
-                          double *__restrict__ stage, const int32_t *__restrict__ inst_of,
-                          const int32_t *__restrict__ orig, const InstParams *__restrict__ P,
-                          uint32_t *__restrict__ counts, uint32_t *__restrict__ skey,
-                          uint32_t *__restrict__ srank, int64_t n, int64_t cap, int mode) {
-  int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  const InstParams &p = P[inst_of[c]];
+// Displacement tracking of the candidate-list step: centre at the list build, accumulated
+// rotation angle and the largest sphere displacement bound since the build, per clump.
+struct ListTrack {
+  double *xb, *vrot, *dsp;
+};
+
+// mode 1: predict (init_Verlet) then stage and sphere-cell keys; mode 2: the same predict
+// and stage without keys (candidate-list step), updating the displacement bound
+// |x' - x_b| + sqrt(3) pos * sum |dOmega| of every sphere since the list build; mode 0:
+// stage the current state only (initial force pass, avalanche_hst.cu:365-384, whose force
+// stage sees the unchanged time).
+__device__ __forceinline__ void predict_clump(double *__restrict__ st, double *__restrict__ sta,
+                                              double *__restrict__ g, const InstParams &p,
+                                              const int32_t *__restrict__ orig,
+                                              uint32_t *__restrict__ counts,
+                                              uint32_t *__restrict__ skey,
+                                              uint32_t *__restrict__ srank, int64_t c, int64_t cap,
+                                              int mode, ListTrack lt) {
   const double t_old = p.time;
   const double t_new = mode ? t_old + p.dt : t_old;
   const double dt = p.dt, hdt = p.hdt;
@@ -84,6 +92,7 @@ __global__ void k_predict(double *__restrict__ st, double *__restrict__ sta,
   const double rad0 = sta[C_RAD0 * cap + c];
   d3 vv = make_d3(st[S_VX * cap + c], st[S_VY * cap + c], st[S_VZ * cap + c]);
   d3 ww = make_d3(st[S_WX * cap + c], st[S_WY * cap + c], st[S_WZ * cap + c]);
+  double rot_step = 0.0;  // |dOmega| of this step
 
   if (mode) {
     if (frozen(t_old, rr.x, p.side_x)) {
@@ -103,6 +112,7 @@ __global__ void k_predict(double *__restrict__ st, double *__restrict__ sta,
       d3 dO = make_d3(dt * (ww.x + hdt * ang.x), dt * (ww.y + hdt * ang.y),
                       dt * (ww.z + hdt * ang.z));
       double mag = sqrt(dO.x * dO.x + dO.y * dO.y + dO.z * dO.z);
+      rot_step = mag;
       double dR[9];
       if (mag > 0.0) {
         d3 nn = make_d3(dO.x / mag, dO.y / mag, dO.z / mag);
@@ -158,7 +168,6 @@ __global__ void k_predict(double *__restrict__ st, double *__restrict__ sta,
   d3 sph[4];
   sphere_centres(rr, R, rad0 / p.sphere_div, sph);
 
-  double *g = stage + c * kStageDoubles;
   g[G_RR + 0] = rr.x;
   g[G_RR + 1] = rr.y;
   g[G_RR + 2] = rr.z;
@@ -178,6 +187,15 @@ __global__ void k_predict(double *__restrict__ st, double *__restrict__ sta,
     g[G_SPH + 3 * s + 2] = sph[s].z;
   }
 
+  if (mode == 2) {
+    const double rot = lt.vrot[c] + rot_step;
+    lt.vrot[c] = rot;
+    const double dx = rr.x - lt.xb[c], dy = rr.y - lt.xb[cap + c], dz = rr.z - lt.xb[2 * cap + c];
+    const double d = sqrt(dx * dx + dy * dy + dz * dz) + 1.7320508075688772 * (rad0 / p.sphere_div) * rot;
+    if (d > lt.dsp[c]) lt.dsp[c] = d;
+    return;
+  }
+
   // Binning at the force-stage time: clumps past side.x are left out (cells.cu:65).
   const bool out = frozen(t_new, rr.x, p.side_x);
 #pragma unroll
@@ -190,6 +208,37 @@ __global__ void k_predict(double *__restrict__ st, double *__restrict__ sta,
     skey[4 * c + s] = key;
     srank[4 * c + s] = rank;
   }
+}
+
+// The stage records of a workgroup's 256 clumps are assembled in LDS (rows padded to 25
+// doubles against bank conflicts) and written as one contiguous span, instead of 24
+// scattered 192-B-strided stores per lane.
+constexpr int kPredictBlock = 256;
+constexpr int kStagePad = kStageDoubles + 1;
+__global__ __launch_bounds__(kPredictBlock) void k_predict(
+    double *__restrict__ st, double *__restrict__ sta, double *__restrict__ stage,
+    const int32_t *__restrict__ inst_of, const int32_t *__restrict__ orig,
+    const InstParams *__restrict__ P, uint32_t *__restrict__ counts, uint32_t *__restrict__ skey,
+    uint32_t *__restrict__ srank, int64_t n, int64_t cap, int mode, ListTrack lt) {
+  __shared__ double rec[kPredictBlock * kStagePad];
+  const int64_t c0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kPredictBlock;
+  const int64_t c = c0 + threadIdx.x;
+  if (c < n) {
+    const int inst = inst_of[c];
+    while (true) {
+      const int cur = __builtin_amdgcn_readfirstlane(inst);
+      if (inst == cur) {
+        predict_clump(st, sta, rec + threadIdx.x * kStagePad, P[cur], orig, counts, skey, srank,
+                      c, cap, mode, lt);
+        break;
+      }
+    }
+  }
+  __syncthreads();
+  const int64_t rows = n - c0 < kPredictBlock ? n - c0 : kPredictBlock;
+  double *dst = stage + c0 * kStageDoubles;
+  for (int64_t k = threadIdx.x; k < rows * kStageDoubles; k += kPredictBlock)
+    dst[k] = rec[(k / kStageDoubles) * kStagePad + k % kStageDoubles];
 }
 
 // ------------------------------------------------------------------------- scan -----
@@ -522,12 +571,16 @@ constexpr int kSphList = 16;  // touching spheres of other clumps per sphere
 // up front; every touching sphere of another clump is listed as (m << 2 | sm).  The test
 // is force_part.cu:140-157's ((dx^2 + dy^2) + dz^2) < sep^2 on the same operands, so the
 // listed set is exactly the set force_part evaluates.
+//
+// With skin > 0 (candidate lists) the test is d^2 < (sep + skin)^2 and CAP ids are kept per
+// sphere; the sphere cells are then at least 2 r_max + skin wide, so 27 cells still cover.
+template <int CAP>
 __global__ __launch_bounds__(256) void k_detect(
     const double *__restrict__ stage, const InstParams *__restrict__ P,
     const int32_t *__restrict__ inst_of, const uint32_t *__restrict__ starts,
     const double4 *__restrict__ spos, const uint32_t *__restrict__ sid,
     uint32_t *__restrict__ clist, int32_t *__restrict__ ccnt, DevDiag *__restrict__ diag,
-    int64_t nsph) {
+    int64_t nsph, double skin) {
   const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= nsph) return;
   const int64_t c = i >> 2;
@@ -557,27 +610,113 @@ __global__ __launch_bounds__(256) void k_detect(
     }
   }
   int cnt = 0;
-  uint32_t *out = clist + i * kSphList;
+  uint32_t *out = clist + i * CAP;
 #pragma unroll
   for (int r = 0; r < 9; r++) {
     for (uint32_t j = lo[r]; j < hi[r]; j++) {
       const uint32_t id = sid[j];
       const double4 o = spos[j];
       if ((int64_t)(id >> 2) == c) continue;
-      const double sep = rad + o.w, sep2 = sep * sep;
+      const double sep = rad + o.w + skin, sep2 = sep * sep;
       const double dx = o.x - q.x, dy = o.y - q.y, dz = o.z - q.z;
       const double d2 = dx * dx + dy * dy + dz * dz;
       if (d2 < sep2) {
-        if (cnt < kSphList) out[cnt] = id;
+        if (cnt < CAP) out[cnt] = id;
         cnt++;
       }
     }
   }
-  if (cnt > kSphList) {
+  if (cnt > CAP) {
     atomicAdd(&diag->neigh_overflow, 1ull);
-    cnt = kSphList;
+    cnt = CAP;
   }
   ccnt[i] = cnt;
+}
+
+// Candidate list of one clump from the 4 sphere lists of k_detect (skin > 0): every clump
+// with a sphere pair within sep + skin, with the mask of those pairings, sorted by the
+// reference key at the build (clump-cell offset, reference index; clumps the visit rule
+// excludes at the build sort last, the force stage re-applies the rule every step).  Also
+// resets the displacement tracking (centre at the build, rotation, bound).
+constexpr int kVMax = 32;     // candidate clumps per clump
+constexpr int kSphListV = 24; // candidate spheres per sphere
+template <int BS>
+__global__ __launch_bounds__(BS) void k_vmerge(
+    const double *__restrict__ stage, const InstParams *__restrict__ P,
+    const int32_t *__restrict__ inst_of, const uint32_t *__restrict__ clist,
+    const int32_t *__restrict__ ccnt, int32_t *__restrict__ vl_m, uint16_t *__restrict__ vl_mask,
+    int32_t *__restrict__ vl_cnt, ListTrack lt, DevDiag *__restrict__ diag, int64_t n,
+    int64_t cap) {
+  __shared__ int32_t sm[kVMax][BS];
+  __shared__ uint32_t sk[kVMax][BS];
+  __shared__ uint16_t smask[kVMax][BS];
+  const int lane = threadIdx.x;
+  const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * BS + lane;
+  if (c >= n) return;
+  const InstParams &p = P[inst_of[c]];
+  const double *gl = stage + c * kStageDoubles;
+  const d3 rr = make_d3(gl[G_RR], gl[G_RR + 1], gl[G_RR + 2]);
+  lt.xb[c] = rr.x;
+  lt.xb[cap + c] = rr.y;
+  lt.xb[2 * cap + c] = rr.z;
+  lt.vrot[c] = 0.0;
+  lt.dsp[c] = 0.0;
+  int64_t li = (int64_t)((rr.x - p.rr_min[0]) / p.cell_side[0]);
+  if (li == p.ncell[0]) li--;
+  int64_t lj = (int64_t)((rr.y - p.rr_min[1]) / p.cell_side[1]);
+  if (lj == p.ncell[1]) lj--;
+  int64_t lk = (int64_t)((rr.z - p.rr_min[2]) / p.cell_side[2]);
+  if (lk == p.ncell[2]) lk--;
+  const int64_t nxy = p.ncell[0] * p.ncell[1];
+  int cnt = 0;
+  bool overflow = false;
+  for (int s = 0; s < 4; s++) {
+    const int k = ccnt[4 * c + s];
+    for (int e = 0; e < k; e++) {
+      const uint32_t id = clist[(4 * c + s) * kSphListV + e];
+      const int64_t cm = (int64_t)(id >> 2);
+      const uint16_t bit = (uint16_t)(1u << (((id & 3u) << 2) | (uint32_t)s));
+      int e2 = 0;
+      for (; e2 < cnt; e2++)
+        if (sm[e2][lane] == (int32_t)cm) break;
+      if (e2 < cnt) {
+        smask[e2][lane] = (uint16_t)(smask[e2][lane] | bit);
+        continue;
+      }
+      const double *g = stage + cm * kStageDoubles;
+      const int64_t L = ref_cell_locate(p, make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]));
+      uint32_t off = 27u;
+      if (L >= 0) {
+        const int64_t ak = L / nxy, rem = L - ak * nxy;
+        const int64_t aj = rem / p.ncell[0], ai = rem - aj * p.ncell[0];
+        const int64_t di = ai - li, dj = aj - lj, dk = ak - lk;
+        if (di >= -1 && di <= 1 && dj >= -1 && dj <= 1 && dk >= -1 && dk <= 1)
+          off = (uint32_t)((di + 1) * 9 + (dj + 1) * 3 + (dk + 1));
+      }
+      const uint32_t key = (off << 27) | (uint32_t)__double_as_longlong(g[G_ID]);
+      if (cnt == kVMax) {
+        overflow = true;
+        continue;
+      }
+      int pos = cnt;
+      while (pos > 0 && sk[pos - 1][lane] > key) {
+        sk[pos][lane] = sk[pos - 1][lane];
+        sm[pos][lane] = sm[pos - 1][lane];
+        smask[pos][lane] = smask[pos - 1][lane];
+        pos--;
+      }
+      sk[pos][lane] = key;
+      sm[pos][lane] = (int32_t)cm;
+      smask[pos][lane] = bit;
+      cnt++;
+    }
+  }
+  for (int e = 0; e < cnt; e++) {
+    vl_m[e * cap + c] = sm[e][lane];
+    vl_mask[e * cap + c] = smask[e][lane];
+  }
+  vl_cnt[c] = cnt;
+  if (overflow) atomicAdd(&diag->neigh_overflow, 1ull);
 }
 
 template <int MAXN, int BS>
@@ -588,19 +727,20 @@ struct PPShared {
   uint32_t pm[16][BS];       // per pairing: bit e set when entry e touches in that pairing
 };
 
-// Main per-clump force pass.  One lane per clump (storage order = spatial order).
-template <int MAXN, int BS>
-__global__ __launch_bounds__(BS) void k_force(
+// Main per-clump force pass.  One lane per clump (storage order = spatial order).  The
+// touching clumps come from k_detect's per-sphere lists or, with VERLET, from the candidate
+// list of k_vmerge, filtered every step by the same touching test and visit rule.
+// time_ahead: the step's time is P.time + dt (candidate-list steps advance P.time at the
+// end of the step instead of in k_scatter).
+template <int MAXN, int BS, bool VERLET>
+__device__ __forceinline__ void force_clump(
+    const InstParams &p, PPShared<MAXN, BS> &sh, int lane, int64_t c,
     double *__restrict__ st, const double *__restrict__ sta, const double *__restrict__ stage,
-    const int32_t *__restrict__ inst_of, const InstParams *__restrict__ P,
     const uint32_t *__restrict__ clist, const int32_t *__restrict__ ccnt,
-    double *__restrict__ fout, DevDiag *__restrict__ diag, int64_t n, int64_t cap, int initial) {
-  __shared__ PPShared<MAXN, BS> sh;
-  const int lane = threadIdx.x;
-  const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * BS + lane;
-  if (c >= n) return;
-  const InstParams &p = P[inst_of[c]];
-  const double time = p.time;
+    const int32_t *__restrict__ vl_m, const uint16_t *__restrict__ vl_mask,
+    const int32_t *__restrict__ vl_cnt, double *__restrict__ fout, DevDiag *__restrict__ diag,
+    int64_t cap, int initial, int time_ahead) {
+  const double time = time_ahead ? p.time + p.dt : p.time;
 
   const d3 rr = make_d3(st[S_RX * cap + c], st[S_RY * cap + c], st[S_RZ * cap + c]);
   d3 vv = make_d3(st[S_VX * cap + c], st[S_VY * cap + c], st[S_VZ * cap + c]);
@@ -639,6 +779,58 @@ __global__ __launch_bounds__(BS) void k_force(
     int64_t lk = (int64_t)((rr.z - p.rr_min[2]) / p.cell_side[2]);
     if (lk == p.ncell[2]) lk--;
     const int64_t nxy = p.ncell[0] * p.ncell[1];
+    if constexpr (VERLET) {
+      const int kc = vl_cnt[c];
+      for (int e = 0; e < kc; e++) {
+        const int64_t cm = vl_m[e * cap + c];
+        const double *g = stage + cm * kStageDoubles;
+        const d3 rm = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
+        if (frozen(time, rm.x, p.side_x)) continue;  // not binned at this time (cells.cu:65)
+        // the candidate pairings that touch now: k_detect's test on the same operands
+        const double sep = rad + g[G_RAD], sep2 = sep * sep;
+        uint32_t bits = vl_mask[e * cap + c], mask = 0u;
+        while (bits) {
+          const int pr = __builtin_ctz(bits);
+          bits &= bits - 1u;
+          const int sdiv = pr >> 2, smod = pr & 3;
+          const d3 q = sphere_of(rr, R, pos0, smod);
+          const double dx = g[G_SPH + 3 * sdiv] - q.x, dy = g[G_SPH + 3 * sdiv + 1] - q.y;
+          const double dz = g[G_SPH + 3 * sdiv + 2] - q.z;
+          if (dx * dx + dy * dy + dz * dz < sep2) mask |= 1u << pr;
+        }
+        if (!mask) continue;
+        const uint32_t morig = (uint32_t)__double_as_longlong(g[G_ID]);
+        const int64_t L = ref_cell_locate(p, rm);
+        if (L < 0) {
+          atomicAdd(&diag->cell_below, 1ull);
+          continue;
+        }
+        const int64_t ak = L / nxy, rem = L - ak * nxy;
+        const int64_t aj = rem / p.ncell[0], ai = rem - aj * p.ncell[0];
+        const int64_t di = ai - li, dj = aj - lj, dk = ak - lk;
+        if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) {
+          atomicAdd(&diag->cell_beyond, 1ull);
+          continue;
+        }
+        const uint32_t off = (uint32_t)((di + 1) * 9 + (dj + 1) * 3 + (dk + 1));
+        const uint32_t key = (off << 27) | morig;
+        if (cnt == MAXN) {
+          overflow = true;
+          continue;
+        }
+        int pos = cnt;
+        while (pos > 0 && sh.key[pos - 1][lane] > key) {
+          sh.key[pos][lane] = sh.key[pos - 1][lane];
+          sh.m[pos][lane] = sh.m[pos - 1][lane];
+          sh.mask[pos][lane] = sh.mask[pos - 1][lane];
+          pos--;
+        }
+        sh.key[pos][lane] = key;
+        sh.m[pos][lane] = (int32_t)cm;
+        sh.mask[pos][lane] = (uint16_t)mask;
+        cnt++;
+      }
+    } else {
     for (int s = 0; s < 4; s++) {
       const int k = ccnt[4 * c + s];
       for (int e = 0; e < k; e++) {
@@ -690,8 +882,9 @@ __global__ __launch_bounds__(BS) void k_force(
         }
       }
     }
+    }
     if (overflow) atomicAdd(&diag->neigh_overflow, 1ull);
-    atomicMax(&diag->max_neighbours, (unsigned long long)cnt);
+    diag_max(&diag->max_neighbours, (unsigned long long)cnt);
 
     // ---- fold: pairings sph_id = 1..16 (m sphere outer, l sphere inner), then clump
     //      cells in (idel, jdel, kdel) order, then ascending m (force_part.cu:40-41,88-108)
@@ -817,6 +1010,32 @@ __global__ __launch_bounds__(BS) void k_force(
   st[S_GZ * cap + c] = gf.z;
 }
 
+// Parameters of the clump's instance are wave-uniform in the waterfall below (one pass per
+// distinct instance in the wavefront, normally one), so they are read with scalar loads
+// instead of per-lane vector loads.
+template <int MAXN, int BS, bool VERLET>
+__global__ __launch_bounds__(BS) void k_force(
+    double *__restrict__ st, const double *__restrict__ sta, const double *__restrict__ stage,
+    const int32_t *__restrict__ inst_of, const InstParams *__restrict__ P,
+    const uint32_t *__restrict__ clist, const int32_t *__restrict__ ccnt,
+    const int32_t *__restrict__ vl_m, const uint16_t *__restrict__ vl_mask,
+    const int32_t *__restrict__ vl_cnt, double *__restrict__ fout, DevDiag *__restrict__ diag,
+    int64_t n, int64_t cap, int initial, int time_ahead) {
+  __shared__ PPShared<MAXN, BS> sh;
+  const int lane = threadIdx.x;
+  const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * BS + lane;
+  if (c >= n) return;
+  const int inst = inst_of[c];
+  while (true) {
+    const int cur = __builtin_amdgcn_readfirstlane(inst);
+    if (inst == cur) {
+      force_clump<MAXN, BS, VERLET>(P[cur], sh, lane, c, st, sta, stage, clist, ccnt, vl_m,
+                                    vl_mask, vl_cnt, fout, diag, cap, initial, time_ahead);
+      return;
+    }
+  }
+}
+
 // -------------------------------------------------------------- clump-cell regrid ---
 
 // Block-level min / max of order-preserving images, then one atomic per block and quantity
@@ -869,7 +1088,12 @@ __global__ __launch_bounds__(256) void k_cellreduce(const double *__restrict__ s
                                                     const int32_t *__restrict__ inst_of,
                                                     const InstParams *__restrict__ P,
                                                     long long *__restrict__ acc, int64_t n,
-                                                    int64_t cap, int force) {
+                                                    int64_t cap, int force, int32_t ninst) {
+  // nothing to do unless some instance regrids this step (every regrid_interval steps)
+  bool any = force != 0;
+  for (int32_t i = 0; i < ninst && !any; i++)
+    any = P[i].regrid_interval > 0 && P[i].cc % P[i].regrid_interval == 0;
+  if (!any) return;
   const int64_t c0 = (int64_t)blockIdx.x * blockDim.x;
   const int64_t c = c0 + threadIdx.x;
   const int inst0 = inst_of[c0];
@@ -934,11 +1158,13 @@ __device__ __forceinline__ void finalize_params(InstParams &p, long long *a, int
 }
 
 // Per instance: finalize_params (step = 0 recomputes the grid without counting a step).
+// advance_time: P.time += dt (candidate-list steps, whose force stage used P.time + dt).
 __global__ void k_finalize(InstParams *__restrict__ P, long long *__restrict__ acc, int32_t ninst,
-                           int force, int step) {
+                           int force, int step, int advance_time = 0) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ninst) return;
   finalize_params(P[i], acc + 8 * i, force, step);
+  if (advance_time) P[i].time = P[i].time + P[i].dt;
 }
 
 // --------------------------------------------------------------- sphere-cell grid ---
@@ -976,7 +1202,7 @@ __global__ __launch_bounds__(256) void k_sgrid_reduce(const double *__restrict__
 // total fits the cell capacity.  Single thread.
 __global__ void k_sgrid_finalize(InstParams *__restrict__ P, long long *__restrict__ acc,
                                  int32_t ninst, int64_t cell_cap, double factor, double horizon,
-                                 int64_t *__restrict__ ncells_out) {
+                                 int64_t *__restrict__ ncells_out, double skin) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   double scale = 1.0;
   bool fit = false;
@@ -987,7 +1213,9 @@ __global__ void k_sgrid_finalize(InstParams *__restrict__ P, long long *__restri
       long long *a = acc + 8 * i;
       const double vmax = sqrt(order_val(a[6]));
       const double margin = p.sg_reach + vmax * horizon * p.dt + 1e-9;
-      const double h = 2.0 * p.sg_rsph * factor * scale * (1.0 + 1e-9);
+      // side >= 2 r_max (touching pairs in the 27 cells around a sphere), >= 2 r_max + skin
+      // with candidate lists
+      const double h = fmax(2.0 * p.sg_rsph * factor, 2.0 * p.sg_rsph + skin) * scale * (1.0 + 1e-9);
       int64_t nd[3];
       for (int k = 0; k < 3; k++) {
         const double lo = order_val(a[k]) - margin, hi = order_val(a[3 + k]) + margin;

@@ -202,7 +202,7 @@ __global__ __launch_bounds__(64) void k_win_list(
       }
   cnt_out[c] = out < kWinMaxList ? out : kWinMaxList;
   if (overflow) atomicAdd(&wd->list_overflow, 1ull);
-  atomicMax(&wd->max_list, (unsigned long long)out);
+  diag_max(&wd->max_list, (unsigned long long)out);
 }
 
 // Transposed list: for every clump m, the entries (l * kWinMaxList + e) that name it.
@@ -229,7 +229,7 @@ __global__ void k_win_in_sort(int32_t *__restrict__ in, int32_t *__restrict__ ic
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n) return;
   int k = icnt[m];
-  atomicMax(&wd->max_in, (unsigned long long)k);
+  diag_max(&wd->max_in, (unsigned long long)k);
   if (k > kWinMaxIn) k = icnt[m] = kWinMaxIn;
   int32_t *a = in + m * kWinMaxIn;
   auto key = [&](int32_t slot) -> int64_t {

@@ -25,8 +25,10 @@ HBM_PEAK_GBS = 8000.0
 # alpha = 24 doubles, 192 B), v, w, a, alpha written (96 B), radius (8 B) and the sorted
 # sphere index of its 4 spheres (16 B).
 FORCE_BYTES_PER_CLUMP = 312.0
-# Algorithmic bytes per clump-step of the whole step (480 B).
+# Algorithmic bytes per clump-step of the whole step (480 B per clump-step).
 STEP_BYTES_PER_CLUMP = 480.0
+# Candidate-list skin (cm) of the headline runs: 5 % of the sphere radius.
+DEFAULT_SKIN = 0.01
 
 
 def cpu_baseline(inst, budget_s: float) -> dict:
@@ -48,14 +50,16 @@ def cpu_baseline(inst, budget_s: float) -> dict:
                       f"({dt:.1f} s on one host core)"}
 
 
-def batch_compare(steps: int, warmup: int = 20) -> dict:
+def batch_compare(steps: int, warmup: int = 20, skin: float = 0.0) -> dict:
     """64 independent flumes (N ~ log-uniform[5e3, 6e4], per-instance seeds, mu, eps and
     slope) stepped together in one context vs the same instances as sequential plain runs
     (one context each, synchronised between instances)."""
     from avalancheflume3d_amd import dem, workloads
     insts = workloads.batch(64)
     total = sum(len(i.particles) for i in insts)
-    ctx = dem.Context(insts)
+    opt = dem.default_options()
+    opt.verlet_skin = skin
+    ctx = dem.Context(insts, opt)
     ctx.regrid()
     ctx.initial_forces()
     ctx.step(warmup)
@@ -67,7 +71,7 @@ def batch_compare(steps: int, warmup: int = 20) -> dict:
     ctx.close()
     t_seq = 0.0
     for inst in insts:
-        c = dem.Context([inst])
+        c = dem.Context([inst], opt)
         c.regrid()
         c.initial_forces()
         c.step(warmup)
@@ -161,6 +165,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="headline", choices=["headline", "small", "large"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--skin", type=float, default=None,
+                    help=f"candidate-list skin in cm (default {DEFAULT_SKIN}; 0: sphere cells "
+                         "rebuilt every step)")
+    ap.add_argument("--compare-steps", type=int, default=2000,
+                    help="steps of the per-step-rebuild comparison run (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--adjoint-steps", type=int, default=2000)
@@ -171,7 +180,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding of "
-              "this path is design-only (see DESIGN.md)", file=sys.stderr)
+              "this path is a design only, not implemented", file=sys.stderr)
         return 2
 
     import torch
@@ -195,54 +204,75 @@ def main() -> int:
     inst = make[args.workload]()
     n = len(inst.particles)
     opt = dem.default_options()
+    opt.verlet_skin = DEFAULT_SKIN if args.skin is None else args.skin
     ctx = dem.Context([inst], opt)
     ctx.regrid()
     ctx.initial_forces()
     ctx.step(args.warmup)
     ctx.sync()
     torch.cuda.synchronize()
-    # timed region: K steps with HIP events between consecutive kernels on the library's
-    # stream (per-kernel durations for the roofline)
+    # timed region: K steps launched on the library's stream with HIP events around the
+    # dominant kernel (k_force) only, for the roofline
     t0 = time.perf_counter()
-    kern, total_ms = ctx.step_profiled(args.steps)
+    force_ms, total_ms = ctx.step_timed(args.steps, "force")
     ctx.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     diag = ctx.diag()
-    # the same number of steps through the captured-graph path, for comparison
+    # the same number of steps replayed as captured hipGraphs
     t0 = time.perf_counter()
     ctx.step(args.steps)
     ctx.sync()
     torch.cuda.synchronize()
     wall_graph = time.perf_counter() - t0
+    # per-kernel breakdown (events between all kernels: adds a few us per kernel)
+    kern, _ = ctx.step_profiled(min(args.steps, 1000))
     ctx.close()
+    # the same workload with the sphere cells rebuilt every step (no candidate lists)
+    per_step = None
+    if opt.verlet_skin > 0.0 and args.compare_steps > 0:
+        o2 = dem.default_options()
+        c2 = dem.Context([inst], o2)
+        c2.regrid()
+        c2.initial_forces()
+        c2.step(args.warmup)
+        c2.sync()
+        t0 = time.perf_counter()
+        c2.step(args.compare_steps)
+        c2.sync()
+        per_step = n * args.compare_steps / (time.perf_counter() - t0)
+        c2.close()
 
     value = n * args.steps / wall
-    force_ms = kern["force"]
     achieved = FORCE_BYTES_PER_CLUMP * n / (force_ms * 1e-3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get(args.workload, {}).get("force_bytes_per_launch")
+    lists = (f"candidate lists within sep + {opt.verlet_skin} cm rebuilt when a sphere has "
+             "moved skin/2 (same results as rebuilding every step)" if opt.verlet_skin > 0
+             else "sphere cells rebuilt every step")
     out = {
         "metric": METRIC, "value": value, "unit": "clump-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {n} clumps ({4 * n} spheres), FP64, sphere "
-                               "cells rebuilt every step, spatial reorder every "
-                               f"{opt.reorder_interval} steps", "clumps": n,
-                   "parallelism": "single GPU"},
+        "config": {"workload": f"{args.workload}: {n} clumps ({4 * n} spheres), FP64, {lists}, "
+                               f"spatial reorder every {opt.reorder_interval} steps",
+                   "clumps": n, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
-        "kernel_ms": kern,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_force", "kernel_ms": force_ms,
+                     "bytes_per_clump": FORCE_BYTES_PER_CLUMP},
+        "kernel_ms_breakdown": kern,
         "step_hbm_gbs_algorithmic": STEP_BYTES_PER_CLUMP * n / (wall / args.steps) / 1e9,
         "events_total_ms": total_ms,
         "graph_path_clump_steps_per_s": n * args.steps / wall_graph,
+        "per_step_rebuild_clump_steps_per_s": per_step,
         "diag": diag,
     }
     if args.batch_steps > 0:
-        out["batched"] = batch_compare(args.batch_steps)
+        out["batched"] = batch_compare(args.batch_steps, skin=opt.verlet_skin)
     if args.adjoint_steps > 0:
         out["adjoint"] = adjoint_bench(args.adjoint_steps)
     if not args.no_cpu:

@@ -141,6 +141,9 @@ typedef struct avf_options {
   int32_t device;           /* HIP device ordinal                                      */
   int32_t reserved;
   void *stream;             /* hipStream_t to run on; NULL = library-owned stream      */
+  double verlet_skin;       /* > 0: candidate lists of sphere pairs within sep + skin, */
+                            /* rebuilt only when some sphere has moved skin / 2 since  */
+                            /* the build (results unchanged); 0: rebuild every step    */
 } avf_options;
 
 /* Status codes.  Nothing in the library calls exit(). */
@@ -298,6 +301,10 @@ enum {
  * context's stream: kernel_ms[k] receives the mean duration of kernel k per step and
  * *total_ms the elapsed time from the first to the last kernel of the run. */
 int avf_step_profiled(avf_ctx *ctx, int64_t nsteps, double *kernel_ms, double *total_ms);
+/* Like avf_step_profiled with events around one kernel only (AVF_K_*): *kernel_ms receives
+ * its mean duration per step, *total_ms the elapsed time of the run. */
+int avf_step_timed(avf_ctx *ctx, int64_t nsteps, int32_t kernel, double *kernel_ms,
+                   double *total_ms);
 /* Waits for all work queued on the context's stream. */
 int avf_sync(avf_ctx *ctx);
 
