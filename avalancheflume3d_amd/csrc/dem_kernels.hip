@@ -365,10 +365,15 @@ __global__ void k_scatter(const uint32_t *__restrict__ skey, const uint32_t *__r
 
 // force_hop.cu:9-244 for one nw (0: x walls, 1: y walls, 2: bottom gate while time < 0).
 __device__ __forceinline__ void hop_walls(const InstParams &p, int nw, double time, d3 rr, d3 vv, d3 ww,
-                          const d3 *sph, double ri, double mass, d3 &ff, d3 &tq) {
+                          const d3 *sph, double ri, double mass, double rb, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
   const double xfront = p.hop_width, xrear = 0.0, yleft = p.yhop_l, yright = p.yhop_r;
   const double zbottom = p.hop_height;
+  // every sphere centre lies within rb of rr: skip when no sphere can pass the tests below
+  if (rr.z + rb < zbottom || rr.x - rb > p.hop_width) return;
+  if (nw == 0 && rr.x - rb - xrear > ri && xfront - rr.x - rb > ri) return;
+  if (nw == 1 && rr.y - rb - yleft > ri && yright - rr.y - rb > ri) return;
+  if (nw == 2 && (!(time < 0.0) || rr.z - rb - zbottom > ri)) return;
   for (int ns = 0; ns < 4; ns++) {
     d3 r = pick(sph, ns), rrc = make_d3(0, 0, 0), nvec = make_d3(0, 0, 0);
     double dist, comp = 0.0;
@@ -421,9 +426,16 @@ __device__ __forceinline__ void hop_walls(const InstParams &p, int nw, double ti
 
 // force_funnel.cu:9-213: converging side planes of the funnel (slope frame).
 __device__ __forceinline__ void funnel_walls(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph, double ri,
-                             double mass, d3 &ff, d3 &tq) {
+                             double mass, double rb, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
   const double sinfi = p.sinfi, cosfi = p.cosfi, tanfi = p.tanfi, cost = p.cost, sint = p.sint;
+  {  // the plane distances below are 1-Lipschitz in the sphere centre
+    const d3 qc = to_slope(rr, cost, sint);
+    if (qc.x - rb > p.lfunnel - ri * sinfi) return;
+    if (fabs(qc.y - qc.x * tanfi) * cosfi - rb > ri &&
+        fabs(p.yfunnel - qc.y - qc.x * tanfi) * cosfi - rb > ri)
+      return;
+  }
   for (int ns = 0; ns < 4; ns++) {
     d3 q = to_slope(pick(sph, ns), cost, sint), rrc = make_d3(0, 0, 0), nvec = make_d3(0, 0, 0);
     double dist, comp = 0.0;
@@ -453,10 +465,14 @@ __device__ __forceinline__ void funnel_walls(const InstParams &p, d3 rr, d3 vv, 
 
 // force_funnel.cu:216-404: vertical corner edges at the funnel exit.
 __device__ __forceinline__ void funnel_corners(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph,
-                               double ri, double mass, d3 &ff, d3 &tq) {
+                               double ri, double mass, double rb, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
   const double yfun_l = p.dy2, yfun_r = p.side_y + p.dy2, hsy = p.dy2 + p.side_y / 2.0;
   const double lf = p.lfunnel, cost = p.cost, sint = p.sint;
+  {
+    const d3 qc = to_slope(rr, cost, sint);
+    if (qc.x + rb < lf - ri * p.sinfi || qc.x - rb > lf + ri) return;
+  }
   for (int ns = 0; ns < 4; ns++) {
     d3 q = to_slope(pick(sph, ns), cost, sint);
     if (q.x < lf - ri * p.sinfi || q.x > lf + ri) continue;
@@ -480,10 +496,12 @@ __device__ __forceinline__ void funnel_corners(const InstParams &p, d3 rr, d3 vv
 
 // force_walls.cu:9-199: flume side walls beyond the funnel (caller skips frozen clumps).
 __device__ __forceinline__ void side_walls(const InstParams &p, double time, d3 rr, d3 vv, d3 ww, const d3 *sph,
-                           double ri, double mass, d3 &ff, d3 &tq) {
+                           double ri, double mass, double rb, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass;
   double gm = 0.5 * p.gamma_pw * mass, mu = p.mu_pw, cns_1 = p.fr_pw;
   const double yleft = p.dy2, yright = p.side_y + p.dy2;
+  if (to_slope(rr, p.cost, p.sint).x + rb < p.lfunnel) return;
+  if (rr.y - rb - yleft > ri && yright - rr.y - rb > ri) return;
   for (int ns = 0; ns < 4; ns++) {
     d3 r = pick(sph, ns), q = to_slope(r, p.cost, p.sint);
     d3 rrc = make_d3(0, 0, 0), nvec = make_d3(0, 0, 0);
@@ -512,9 +530,10 @@ __device__ __forceinline__ void side_walls(const InstParams &p, double time, d3 
 
 // force_flume.cu:10-182: inclined floor z = -x tan(theta).
 __device__ __forceinline__ void flume_floor(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph, double ri,
-                            double mass, d3 &ff, d3 &tq) {
+                            double mass, double rb, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
   const double sint = p.sint, cost = p.cost, tant = sint / cost;
+  if (fabs(cost * (-tant * rr.x - rr.z)) - rb > ri) return;
   for (int ns = 0; ns < 4; ns++) {
     d3 r = pick(sph, ns);
     double rect = -tant * r.x - r.z, dist = fabs(cost * rect);
@@ -528,9 +547,10 @@ __device__ __forceinline__ void flume_floor(const InstParams &p, d3 rr, d3 vv, d
 
 // force_dep.cu:10-178 with the deposit plane of avf_geometry.
 __device__ __forceinline__ void deposit_floor(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph, double ri,
-                              double mass, d3 &ff, d3 &tq) {
+                              double mass, double rb, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
   const double sind = p.dep_sin, cosd = p.dep_cos, tand = sind / cosd;
+  if (cosd * (-(tand * (rr.x - p.dep_x0 - p.dep_x1) - (rr.z - p.dep_z0))) - rb > ri) return;
   for (int ns = 0; ns < 4; ns++) {
     d3 r = pick(sph, ns);
     double rect = tand * (r.x - p.dep_x0 - p.dep_x1) - (r.z - p.dep_z0), dist = cosd * (-rect);
@@ -753,6 +773,9 @@ __device__ __forceinline__ void force_clump(
   const double pos0 = rad0 / p.sphere_div;
   d3 sph[4];
   sphere_centres(rr, R, pos0, sph);
+  // bound on |sphere centre - rr| (sqrt(3) pos0; R has unit columns) with a margin far above
+  // the rounding of the centres, for the boundary modules' early outs
+  const double rb = 1.7320508075688772 * pos0 * (1.0 + 1e-12) + 1e-9;
 
   const bool frz = frozen(time, rr.x, p.side_x);
   const int mods = p.phase ? p.modules_aval : p.modules_fill;
@@ -761,10 +784,10 @@ __device__ __forceinline__ void force_clump(
 
   if (p.phase) {  // avalanche_hst.cu:688-696
     if (mods & AVF_MOD_HOP)
-      for (int nw = 0; nw < 3; nw++) hop_walls(p, nw, time, rr, vv, ww, sph, rad, massw, ff, tq);
-    if (mods & AVF_MOD_FUNNEL) funnel_walls(p, rr, vv, ww, sph, rad, massw, ff, tq);
-    if (mods & AVF_MOD_FCORNERS) funnel_corners(p, rr, vv, ww, sph, rad, massw, ff, tq);
-    if ((mods & AVF_MOD_WALLS) && !frz) side_walls(p, time, rr, vv, ww, sph, rad, massw, ff, tq);
+      for (int nw = 0; nw < 3; nw++) hop_walls(p, nw, time, rr, vv, ww, sph, rad, massw, rb, ff, tq);
+    if (mods & AVF_MOD_FUNNEL) funnel_walls(p, rr, vv, ww, sph, rad, massw, rb, ff, tq);
+    if (mods & AVF_MOD_FCORNERS) funnel_corners(p, rr, vv, ww, sph, rad, massw, rb, ff, tq);
+    if ((mods & AVF_MOD_WALLS) && !frz) side_walls(p, time, rr, vv, ww, sph, rad, massw, rb, ff, tq);
   }
 
   if ((mods & AVF_MOD_PP) && !frz) {
@@ -938,10 +961,10 @@ __device__ __forceinline__ void force_clump(
 
   if (!p.phase) {  // avalanche_hst.cu:460-461 (pp first, then hop, while filling)
     if (mods & AVF_MOD_HOP)
-      for (int nw = 0; nw < 3; nw++) hop_walls(p, nw, time, rr, vv, ww, sph, rad, massw, ff, tq);
+      for (int nw = 0; nw < 3; nw++) hop_walls(p, nw, time, rr, vv, ww, sph, rad, massw, rb, ff, tq);
   } else {  // avalanche_hst.cu:704
-    if ((mods & AVF_MOD_FLUME) && !frz) flume_floor(p, rr, vv, ww, sph, rad, massw, ff, tq);
-    if (mods & AVF_MOD_DEP) deposit_floor(p, rr, vv, ww, sph, rad, massw, ff, tq);
+    if ((mods & AVF_MOD_FLUME) && !frz) flume_floor(p, rr, vv, ww, sph, rad, massw, rb, ff, tq);
+    if (mods & AVF_MOD_DEP) deposit_floor(p, rr, vv, ww, sph, rad, massw, rb, ff, tq);
   }
 
   if (fout) {  // F / T of the step, kept for export (last step of a launch sequence)
