@@ -33,6 +33,7 @@ STATUS = {
     4: "AVF_ERR_CELL_NEG", 5: "AVF_ERR_CELL_FULL", 6: "AVF_ERR_NEIGH_OVERFLOW",
     7: "AVF_ERR_NOFIT", 8: "AVF_ERR_INPUT",
 }
+STATUS_CODES = {v: k for k, v in STATUS.items()}
 
 
 class Double3(C.Structure):

@@ -1,30 +1,34 @@
 """Autotuner of the launch parameters that do not change results.
 
-Sphere-cell size (a multiple of the largest sphere diameter), the spatial reorder interval and
-the block shape of the force kernel change only speed: every per-clump sum is sequential in
-the reference order, so results are bit-identical across them.  The tuner times a short run
-of each candidate on the caller's instances and returns the fastest options.
+The candidate-list skin, the sphere-cell size (a multiple of the largest sphere diameter),
+the spatial reorder interval and the block shape of the force kernel change only speed:
+every per-clump sum is sequential in the reference order and the candidate lists always hold
+every touching pair, so results are bit-identical across them.  The tuner times a short run
+of each candidate on the caller's instances (through the captured-graph path) and returns the
+fastest options.
 """
 from __future__ import annotations
 
 import itertools
 import time
 
-from . import abi, dem
+from . import dem
 
-FACTORS = (1.0, 1.25, 1.6)
-REORDERS = (20, 50, 100)
+SKINS = (0.0, 0.005, 0.01, 0.02)   # cm; 0: sphere cells rebuilt every step
+FACTORS = (1.0, 1.25)
+REORDERS = (50, 100)
 BLOCKS = (64, 128)
 
 
-def autotune(instances: list[dem.Instance], steps: int = 40, warmup: int = 10,
-             factors=FACTORS, reorders=REORDERS, blocks=BLOCKS, verbose: bool = False):
-    """Returns (best avf_options, [(factor, reorder, block, seconds_per_step), ...])."""
+def autotune(instances: list[dem.Instance], steps: int = 200, warmup: int = 20,
+             skins=SKINS, factors=FACTORS, reorders=REORDERS, blocks=BLOCKS,
+             verbose: bool = False):
+    """Returns (best avf_options, [(skin, factor, reorder, block, seconds_per_step), ...])."""
     table = []
     best, best_t = None, float("inf")
-    for f, r, b in itertools.product(factors, reorders, blocks):
+    for sk, f, r, b in itertools.product(skins, factors, reorders, blocks):
         o = dem.default_options()
-        o.sphere_cell_factor, o.reorder_interval, o.block_clumps = f, r, b
+        o.verlet_skin, o.sphere_cell_factor, o.reorder_interval, o.block_clumps = sk, f, r, b
         ctx = dem.Context(instances, o)
         ctx.regrid()
         ctx.initial_forces()
@@ -35,9 +39,10 @@ def autotune(instances: list[dem.Instance], steps: int = 40, warmup: int = 10,
         ctx.sync()
         dt = (time.perf_counter() - t0) / steps
         ctx.close()
-        table.append((f, r, b, dt))
+        table.append((sk, f, r, b, dt))
         if verbose:
-            print(f"factor {f:.2f} reorder {r:4d} block {b:4d}: {dt * 1e6:9.1f} us/step", flush=True)
+            print(f"skin {sk:.3f} factor {f:.2f} reorder {r:4d} block {b:4d}: "
+                  f"{dt * 1e6:9.1f} us/step", flush=True)
         if dt < best_t:
             best, best_t = o, dt
     return best, table

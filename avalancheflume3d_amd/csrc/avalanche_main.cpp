@@ -9,7 +9,8 @@
 //   log_amd.dat, mean_velocity_amd.csv, xcm_amd.csv, snapshots_amd.xyz, velocities_amd.xyz,
 //   last_config.dat (the reference's 18-column text checkpoint) and last_config.bin (lossless).
 //
-//   avalanche_amd [--restart] [--restart-bin FILE] [--max-steps N] [--device D] < DataInput.dat
+//   avalanche_amd [--restart] [--restart-bin FILE] [--max-steps N] [--final-bin FILE]
+//                 [--device D] < DataInput.dat
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -182,6 +183,7 @@ int main(int argc, char **argv) {
   bool restart = false;
   const char *restart_bin = nullptr;
   int64_t max_steps = -1;
+  const char *final_bin = nullptr;
   avf_options opt;
   avf_options_default(&opt);
   for (int i = 1; i < argc; i++) {
@@ -189,9 +191,10 @@ int main(int argc, char **argv) {
     else if (!std::strcmp(argv[i], "--restart-bin") && i + 1 < argc) restart_bin = argv[++i];
     else if (!std::strcmp(argv[i], "--max-steps") && i + 1 < argc) max_steps = std::atoll(argv[++i]);
     else if (!std::strcmp(argv[i], "--device") && i + 1 < argc) opt.device = std::atoi(argv[++i]);
+    else if (!std::strcmp(argv[i], "--final-bin") && i + 1 < argc) final_bin = argv[++i];
     else {
       std::fprintf(stderr, "usage: %s [--restart] [--restart-bin FILE] [--max-steps N] "
-                           "[--device D] < DataInput.dat\n", argv[0]);
+                           "[--final-bin FILE] [--device D] < DataInput.dat\n", argv[0]);
       return 2;
     }
   }
@@ -338,6 +341,8 @@ int main(int argc, char **argv) {
   if (!aval_start) {
     std::printf("Filling hopper...\n");
     run_phase(true);
+  }
+  if (!aval_start && !(max_steps >= 0 && total >= max_steps)) {
     sys.side.x = xfinal;  // avalanche_hst.cu:606-613
     aval_start = 1;
     cc = 0;
@@ -345,8 +350,12 @@ int main(int argc, char **argv) {
     std::printf("Opening hopper... Avalanche starts!\n");
     CK(avf_set_phase(ctx, 0, AVF_PHASE_AVALANCHE, 0.0, xfinal, 0));
   }
-  run_phase(false);
+  if (aval_start) run_phase(false);
   CK(avf_sync(ctx));
+  if (final_bin) {  // lossless state at exit (restart with --restart-bin)
+    CK(avf_download(ctx, 0, pp.data(), aa.data(), ga.data()));
+    write_bin(final_bin, sys, aval_start, cc, jj, pp, aa, ga);
+  }
   avf_destroy(ctx);
   std::fclose(o.vmean);
   std::fclose(o.vel);

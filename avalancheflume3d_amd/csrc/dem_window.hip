@@ -1194,8 +1194,7 @@ __global__ void k_win_max(const double *__restrict__ v, int64_t n, unsigned long
     const double q = __shfl_xor(m, off);
     m = q > m ? q : m;
   }
-  if ((threadIdx.x & 63) == 0 && m > 0.0)
-    atomicMax(out, (unsigned long long)__double_as_longlong(m));
+  if ((threadIdx.x & 63) == 0 && m > 0.0) diag_max(out, (unsigned long long)__double_as_longlong(m));
 }
 
 }  // namespace avf
