@@ -47,6 +47,7 @@ struct avf_ctx {
   InstParams *d_P = nullptr;
   double *d_st = nullptr, *d_st2 = nullptr, *d_sta = nullptr, *d_sta2 = nullptr;
   double *d_stage = nullptr, *d_f = nullptr, *d_io = nullptr, *d_io2 = nullptr;
+  double *d_stage2 = nullptr;  // second stage buffer of the fused list steps
   int32_t *d_orig = nullptr, *d_orig2 = nullptr, *d_inst = nullptr, *d_inst2 = nullptr;
   int64_t *d_inv = nullptr, *d_ncells = nullptr, *d_tmp64 = nullptr;
   uint32_t *d_counts = nullptr, *d_starts = nullptr, *d_bsum = nullptr;
@@ -200,12 +201,16 @@ void launch_bin(avf_ctx *c, int mode) {
       4 * c->n, mode);
 }
 
-template <int BS, bool VERLET>
-void launch_force_bs(avf_ctx *c, int initial, bool ft, int time_ahead) {
-  k_force<kMaxNeigh, BS, VERLET><<<grid_for(c->n, BS), BS, 0, c->stream>>>(
-      c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_P, c->d_clist, c->d_ccnt, c->d_vl_m,
-      c->d_vl_mask, c->d_vl_cnt, ft ? c->d_f : nullptr, c->d_diag, c->n, c->cap, initial,
-      time_ahead);
+ListTrack list_track(avf_ctx *c) { return ListTrack{c->d_xb, c->d_vrot, c->d_dsp}; }
+
+template <int BS, bool VERLET, bool NEXT>
+void launch_force_bs(avf_ctx *c, int initial, bool ft, int time_ahead, const double *stage_in,
+                     double *stage_out) {
+  k_force<kMaxNeigh, BS, VERLET, NEXT><<<grid_for(c->n, BS), BS, 0, c->stream>>>(
+      c->d_st, c->d_sta, stage_in, c->d_inst, c->d_P, c->d_clist, c->d_ccnt, c->d_vl_m,
+      c->d_vl_mask, c->d_vl_cnt, ft ? c->d_f : nullptr, c->d_diag, c->d_acc_cell, stage_out,
+      c->d_orig, NEXT ? list_track(c) : ListTrack{nullptr, nullptr, nullptr}, c->n, c->cap,
+      initial, time_ahead);
 }
 
 // Touching spheres of the staged spheres (skin 0) or candidate spheres within sep + skin.
@@ -222,23 +227,30 @@ void launch_detect(avf_ctx *c, double skin = 0.0) {
 
 // k_force block shape: 64 (one wavefront) or 128 lanes (avf_options.block_clumps).  ft:
 // keep the step's forces and torques for avf_get_forces (the last step of a sequence).
-// verlet: touching clumps from the candidate lists; time_ahead: see k_force.
+// verlet: touching clumps from the candidate lists; time_ahead: see k_force.  stage_in: the
+// step's stage records (default d_stage); stage_out (candidate lists only): also predict the
+// next step into that buffer.
 void launch_force(avf_ctx *c, int initial, bool ft = true, bool verlet = false,
-                  int time_ahead = 0) {
+                  int time_ahead = 0, const double *stage_in = nullptr,
+                  double *stage_out = nullptr) {
+  if (!stage_in) stage_in = c->d_stage;
+  const bool next = verlet && stage_out;
   if (c->opt.block_clumps >= 128) {
-    if (verlet)
-      launch_force_bs<128, true>(c, initial, ft, time_ahead);
+    if (next)
+      launch_force_bs<128, true, true>(c, initial, ft, time_ahead, stage_in, stage_out);
+    else if (verlet)
+      launch_force_bs<128, true, false>(c, initial, ft, time_ahead, stage_in, nullptr);
     else
-      launch_force_bs<128, false>(c, initial, ft, time_ahead);
+      launch_force_bs<128, false, false>(c, initial, ft, time_ahead, stage_in, nullptr);
   } else {
-    if (verlet)
-      launch_force_bs<kForceBlock, true>(c, initial, ft, time_ahead);
+    if (next)
+      launch_force_bs<kForceBlock, true, true>(c, initial, ft, time_ahead, stage_in, stage_out);
+    else if (verlet)
+      launch_force_bs<kForceBlock, true, false>(c, initial, ft, time_ahead, stage_in, nullptr);
     else
-      launch_force_bs<kForceBlock, false>(c, initial, ft, time_ahead);
+      launch_force_bs<kForceBlock, false, false>(c, initial, ft, time_ahead, stage_in, nullptr);
   }
 }
-
-ListTrack list_track(avf_ctx *c) { return ListTrack{c->d_xb, c->d_vrot, c->d_dsp}; }
 
 // One full step: avalanche_hst.cu:669-710 (or :439-467) plus the regrid and cc++.
 // ev (optional, AVF_NKERNELS + 1 events) brackets every kernel for avf_step_profiled.
@@ -266,30 +278,32 @@ void launch_step(avf_ctx *c, hipEvent_t *ev = nullptr, bool ft = true) {
   launch_detect(c);
   mark(AVF_K_FORCE);
   launch_force(c, 0, ft);
-  mark(AVF_K_CELLREDUCE);
-  k_cellreduce<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(c->d_st, c->d_inst, c->d_P,
-                                                                  c->d_acc_cell, c->n, c->cap, 0,
-                                                                  c->ninst);
+  mark(AVF_K_CELLREDUCE);  // the regrid reduction runs in k_force
   mark(AVF_K_FINALIZE);
-  k_finalize<<<grid_for(c->ninst, 64), 64, 0, c->stream>>>(c->d_P, c->d_acc_cell, c->ninst, 0, 1);
+  k_finalize<<<grid_for(c->ninst, 64), 64, 0, c->stream>>>(c->d_P, c->d_acc_cell, c->ninst, 0, 1,
+                                                           0);
   mark(AVF_NKERNELS);
 }
 
 // One step on candidate lists.  rebuild: bin the predicted spheres and rebuild the lists
 // (k_predict keys, scans, scatter advancing P.time, k_detect with the skin, k_vmerge), then
-// the force stage; otherwise k_predict without keys (tracking displacements) and the force
-// stage with P.time + dt, P.time advanced by k_finalize.  The kernel slots of
-// avf_step_profiled are kept (the binning slots are empty on list steps).
-void launch_vstep(avf_ctx *c, bool rebuild, hipEvent_t *ev = nullptr, bool ft = true) {
+// the force stage.  Otherwise step i of a chunk of S list steps: k_predict without keys
+// (tracking displacements) for the first step only; k_force runs the force stage at
+// P.time + dt and, except on the last step, predicts the next step into the other stage
+// buffer; k_finalize advances P.time.  The kernel slots of avf_step_profiled are kept (the
+// binning slots are empty on list steps, the predict slot after the first step of a chunk).
+void launch_vstep(avf_ctx *c, bool rebuild, hipEvent_t *ev = nullptr, bool ft = true,
+                  int64_t i = 0, int64_t S = 1) {
   auto mark = [&](int k) {
     if (ev && (c->prof_kernel < 0 || k == c->prof_kernel || k == c->prof_kernel + 1))
       (void)hipEventRecord(ev[k], c->stream);
   };
   const unsigned nb = grid_for(c->cell_cap, kScanTile);
   mark(AVF_K_PREDICT);
-  k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
-      c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-      c->d_srank, c->n, c->cap, rebuild ? 1 : 2, list_track(c));
+  if (rebuild || i == 0)
+    k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
+        c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
+        c->d_srank, c->n, c->cap, rebuild ? 1 : 2, list_track(c));
   mark(AVF_K_SCAN_A);
   if (rebuild) k_scan_a<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
   mark(AVF_K_SCAN_B);
@@ -310,11 +324,13 @@ void launch_vstep(avf_ctx *c, bool rebuild, hipEvent_t *ev = nullptr, bool ft = 
         c->d_vl_cnt, list_track(c), c->d_diag, c->n, c->cap);
   }
   mark(AVF_K_FORCE);
-  launch_force(c, 0, ft, true, rebuild ? 0 : 1);
-  mark(AVF_K_CELLREDUCE);
-  k_cellreduce<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(c->d_st, c->d_inst, c->d_P,
-                                                                  c->d_acc_cell, c->n, c->cap, 0,
-                                                                  c->ninst);
+  if (rebuild) {
+    launch_force(c, 0, ft, true, 0);
+  } else {
+    double *buf[2] = {c->d_stage, c->d_stage2};
+    launch_force(c, 0, ft, true, 1, buf[i & 1], i + 1 < S ? buf[(i + 1) & 1] : nullptr);
+  }
+  mark(AVF_K_CELLREDUCE);  // the regrid reduction runs in k_force
   mark(AVF_K_FINALIZE);
   k_finalize<<<grid_for(c->ninst, 64), 64, 0, c->stream>>>(c->d_P, c->d_acc_cell, c->ninst, 0, 1,
                                                            rebuild ? 0 : 1);
@@ -392,7 +408,8 @@ int vsteps_profiled(avf_ctx *c, int64_t S, bool rebuild, StepProf *prof) {
       break;
     }
   if (rc == AVF_OK) {
-    for (int64_t i = 0; i < S; i++) launch_vstep(c, rebuild, &ev[(size_t)i * kEv], i + 1 == S);
+    for (int64_t i = 0; i < S; i++)
+      launch_vstep(c, rebuild, &ev[(size_t)i * kEv], i + 1 == S, i, S);
     const int jlast = c->prof_kernel >= 0 ? c->prof_kernel + 1 : AVF_NKERNELS;
     hipError_t he = hipEventSynchronize(ev[(size_t)(S - 1) * kEv + jlast]);
     for (int64_t i = 0; i < S && he == hipSuccess; i++)
@@ -460,7 +477,7 @@ int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
       if (it == c->graphs.end()) {
         hipGraph_t g;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (int64_t i = 0; i < S; i++) launch_vstep(c, false, nullptr, i + 1 == S);
+        for (int64_t i = 0; i < S; i++) launch_vstep(c, false, nullptr, i + 1 == S, i, S);
         HIPCHK(c, hipStreamEndCapture(c->stream, &g));
         hipGraphExec_t ge;
         HIPCHK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
@@ -469,7 +486,7 @@ int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
       }
       HIPCHK(c, hipGraphLaunch(it->second, c->stream));
     } else {
-      for (int64_t i = 0; i < S; i++) launch_vstep(c, false, nullptr, i + 1 == S);
+      for (int64_t i = 0; i < S; i++) launch_vstep(c, false, nullptr, i + 1 == S, i, S);
     }
     // largest displacement bound since the build
     unsigned long long um = 0;
@@ -615,7 +632,7 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
        (rc = dalloc(c, &c->d_vrot, N)) || (rc = dalloc(c, &c->d_dsp, N)) ||
        (rc = dalloc(c, &c->d_ck_st, N * S_NFIELDS)) || (rc = dalloc(c, &c->d_ck_rad, N)) ||
        (rc = dalloc(c, &c->d_ck_P, ninst)) || (rc = dalloc(c, &c->d_ck_diag, 1)) ||
-       (rc = dalloc(c, &c->d_umax, 1))))
+       (rc = dalloc(c, &c->d_umax, 1)) || (rc = dalloc(c, &c->d_stage2, N * kStageDoubles))))
     return bail(rc);
 
   // Parameters, identity storage order, zeroed tables.
@@ -718,7 +735,8 @@ int avf_destroy(avf_ctx *c) {
                   c->d_bsum,   c->d_skey,  c->d_srank, c->d_sid,   c->d_spos,  c->d_acc_cell,
                   c->d_acc_sg, c->d_diag,  c->d_count, c->d_clist, c->d_ccnt,
                   c->d_vl_m,   c->d_vl_cnt, c->d_vl_mask, c->d_xb, c->d_vrot, c->d_dsp,
-                  c->d_ck_st,  c->d_ck_rad, c->d_ck_P, c->d_ck_diag, c->d_umax};
+                  c->d_ck_st,  c->d_ck_rad, c->d_ck_P, c->d_ck_diag, c->d_umax,
+                  c->d_stage2};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   free_window(c);

@@ -72,15 +72,15 @@ struct ListTrack {
 // and stage without keys (candidate-list step), updating the displacement bound
 // |x' - x_b| + sqrt(3) pos * sum |dOmega| of every sphere since the list build; mode 0:
 // stage the current state only (initial force pass, avalanche_hst.cu:365-384, whose force
-// stage sees the unchanged time).
+// stage sees the unchanged time).  t_old: the time at the start of the step (P.time, or
+// P.time + dt when the previous step's k_finalize has not run yet).
 __device__ __forceinline__ void predict_clump(double *__restrict__ st, double *__restrict__ sta,
                                               double *__restrict__ g, const InstParams &p,
                                               const int32_t *__restrict__ orig,
                                               uint32_t *__restrict__ counts,
                                               uint32_t *__restrict__ skey,
                                               uint32_t *__restrict__ srank, int64_t c, int64_t cap,
-                                              int mode, ListTrack lt) {
-  const double t_old = p.time;
+                                              int mode, ListTrack lt, double t_old) {
   const double t_new = mode ? t_old + p.dt : t_old;
   const double dt = p.dt, hdt = p.hdt;
 
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(kPredictBlock) void k_predict(
       const int cur = __builtin_amdgcn_readfirstlane(inst);
       if (inst == cur) {
         predict_clump(st, sta, rec + threadIdx.x * kStagePad, P[cur], orig, counts, skey, srank,
-                      c, cap, mode, lt);
+                      c, cap, mode, lt, P[cur].time);
         break;
       }
     }
@@ -739,13 +739,87 @@ __global__ __launch_bounds__(BS) void k_vmerge(
   if (overflow) atomicAdd(&diag->neigh_overflow, 1ull);
 }
 
+// Per-lane columns only (no lane reads another lane's entries).  The pairing masks are
+// built once the entries are sorted and reuse the keys' storage, which keeps a 64-clump
+// workgroup at 20 KB of LDS (8 workgroups per CU, as many as the registers allow).
 template <int MAXN, int BS>
 struct PPShared {
-  int32_t m[MAXN][BS];       // storage index of each touching clump, sorted by key
-  uint32_t key[MAXN][BS];    // (clump-cell offset << 27) | reference index
-  uint16_t mask[MAXN][BS];   // bit (sdiv*4 + smod): sphere pairs in contact
-  uint32_t pm[16][BS];       // per pairing: bit e set when entry e touches in that pairing
+  static_assert(MAXN >= 16, "the pairing masks reuse the key rows");
+  int32_t m[MAXN][BS];         // storage index of each touching clump, sorted by key
+  union {
+    uint32_t key[MAXN][BS];    // (clump-cell offset << 27) | reference index
+    uint32_t pm[16][BS];       // per pairing: bit e set when entry e touches in that pairing
+  };
+  uint16_t mask[MAXN][BS];     // bit (sdiv*4 + smod): sphere pairs in contact
 };
+
+// cell_dim.cu:55-60 margins and the regrid arithmetic of avalanche_hst.cu:311-333 when the
+// regrid is due (or forced); then cc++ (avalanche_hst.cu:600, 852) when step is set.
+__device__ __forceinline__ void finalize_params(InstParams &p, long long *a, int force, int step) {
+  if (force || (p.regrid_interval > 0 && p.cc % p.regrid_interval == 0)) {
+    const double rgyr = 0.5 * p.gyr_max, ddt = p.regrid_ddt_steps * p.dt;
+    double rmin[3], rmax[3];
+    for (int k = 0; k < 3; k++) {
+      rmin[k] = order_val(a[k]);
+      rmax[k] = order_val(a[3 + k]);
+    }
+    const double vxmax = order_val(a[6]), vzmin = order_val(a[7]);
+    rmax[0] += 1.05 * rgyr + vxmax * ddt;
+    rmin[0] -= 1.05 * rgyr;
+    rmax[1] += 1.05 * rgyr;
+    rmin[1] -= 1.05 * rgyr;
+    rmax[2] += 1.05 * rgyr;
+    rmin[2] -= 1.05 * rgyr - vzmin * ddt;
+    const double dist = p.cell_dist;
+    for (int k = 0; k < 3; k++) {
+      p.rr_min[k] = rmin[k];
+      p.rr_max[k] = rmax[k];
+      p.length[k] = rmax[k] - rmin[k];
+      p.ncell[k] = (int64_t)(p.length[k] / (dist * p.rmax));
+      p.cell_side[k] = p.length[k] / ((double)p.ncell[k]);
+      p.cs_inv[k] = 1.0 / p.cell_side[k];
+    }
+    for (int k = 0; k < 3; k++) {
+      a[k] = 0x7fffffffffffffffll;
+      a[3 + k] = (long long)0x8000000000000000ull;
+    }
+    a[6] = (long long)0x8000000000000000ull;
+    a[7] = 0x7fffffffffffffffll;
+  }
+  if (step) p.cc += 1;
+}
+
+// Per wavefront: min / max of 8 order-preserving images over the lanes with active set,
+// separately for each instance present (normally one), one atomic per instance and quantity.
+// All 64 lanes of the wavefront must call it.  mode bit k: 1 = max, 0 = min.
+__device__ __forceinline__ void wave_minmax8(long long *acc, int inst, const long long *vals,
+                                             bool active, unsigned mode) {
+  const int lane = threadIdx.x & 63;
+  bool todo = active;
+  while (true) {
+    const unsigned long long b = __ballot(todo);
+    if (!b) break;
+    const int first = __ffsll((long long)b) - 1;
+    const int cur = __shfl(inst, first);
+    const bool mine = todo && inst == cur;
+    for (int k = 0; k < 8; k++) {
+      const bool mx = (mode >> k) & 1u;
+      const long long idle = mx ? (long long)0x8000000000000000ull : 0x7fffffffffffffffll;
+      long long x = mine ? vals[k] : idle;
+      for (int off = 32; off > 0; off >>= 1) {
+        const long long y = __shfl_xor(x, off);
+        x = mx ? (x > y ? x : y) : (x < y ? x : y);
+      }
+      if (lane == first) {
+        if (mx)
+          atomicMax(&acc[8 * cur + k], x);
+        else
+          atomicMin(&acc[8 * cur + k], x);
+      }
+    }
+    todo = todo && !mine;
+  }
+}
 
 // Main per-clump force pass.  One lane per clump (storage order = spatial order).  The
 // touching clumps come from k_detect's per-sphere lists or, with VERLET, from the candidate
@@ -1036,26 +1110,80 @@ __device__ __forceinline__ void force_clump(
 // Parameters of the clump's instance are wave-uniform in the waterfall below (one pass per
 // distinct instance in the wavefront, normally one), so they are read with scalar loads
 // instead of per-lane vector loads.
-template <int MAXN, int BS, bool VERLET>
+//
+// NEXT (candidate-list steps inside a chunk): after finishing the step, the same lane runs
+// the next step's predict on its clump (init_Verlet needs only the clump's own finished
+// state) and writes the next stage record to stage_out, so a list step is one launch that
+// reads and writes the carried state once.  Neighbours of this step read the records of
+// `stage`, never stage_out.  The next step starts at P.time + dt (k_finalize of this step
+// runs after this kernel).
+template <int MAXN, int BS, bool VERLET, bool NEXT>
 __global__ __launch_bounds__(BS) void k_force(
-    double *__restrict__ st, const double *__restrict__ sta, const double *__restrict__ stage,
+    double *__restrict__ st, double *__restrict__ sta, const double *__restrict__ stage,
     const int32_t *__restrict__ inst_of, const InstParams *__restrict__ P,
     const uint32_t *__restrict__ clist, const int32_t *__restrict__ ccnt,
     const int32_t *__restrict__ vl_m, const uint16_t *__restrict__ vl_mask,
     const int32_t *__restrict__ vl_cnt, double *__restrict__ fout, DevDiag *__restrict__ diag,
-    int64_t n, int64_t cap, int initial, int time_ahead) {
+    long long *__restrict__ acc, double *__restrict__ stage_out,
+    const int32_t *__restrict__ orig, ListTrack lt, int64_t n, int64_t cap, int initial,
+    int time_ahead) {
+  static_assert(!NEXT || VERLET, "the fused predict runs on candidate-list steps only");
+  static_assert(sizeof(PPShared<MAXN, BS>) >= sizeof(double) * BS * kStagePad,
+                "stage records of the workgroup are assembled in the pair scratch");
   __shared__ PPShared<MAXN, BS> sh;
   const int lane = threadIdx.x;
-  const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * BS + lane;
-  if (c >= n) return;
-  const int inst = inst_of[c];
-  while (true) {
-    const int cur = __builtin_amdgcn_readfirstlane(inst);
-    if (inst == cur) {
-      force_clump<MAXN, BS, VERLET>(P[cur], sh, lane, c, st, sta, stage, clist, ccnt, vl_m,
-                                    vl_mask, vl_cnt, fout, diag, cap, initial, time_ahead);
-      return;
+  const int64_t c0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * BS;
+  const int64_t c = c0 + lane;
+  const int inst = c < n ? inst_of[c] : -1;
+  if (c < n) {
+    while (true) {
+      const int cur = __builtin_amdgcn_readfirstlane(inst);
+      if (inst == cur) {
+        force_clump<MAXN, BS, VERLET>(P[cur], sh, lane, c, st, sta, stage, clist, ccnt, vl_m,
+                                      vl_mask, vl_cnt, fout, diag, cap, initial, time_ahead);
+        break;
+      }
     }
+  }
+  if (initial) return;  // the initial force pass (avalanche_hst.cu:365-384) is not a step
+  // k_cellreduce's work on the steps the clump-cell regrid is due: the AABB of the new
+  // centres and the velocity extremes (cell_dim.cu:27-52); k_finalize follows.
+  bool due = false;
+  long long v[8];
+  if (c < n) {
+    const InstParams &q = P[inst];
+    due = q.regrid_interval > 0 && q.cc % q.regrid_interval == 0;
+    if (due) {
+      v[0] = order_key(st[S_RX * cap + c]);
+      v[1] = order_key(st[S_RY * cap + c]);
+      v[2] = order_key(st[S_RZ * cap + c]);
+      v[3] = v[0];
+      v[4] = v[1];
+      v[5] = v[2];
+      v[6] = order_key(st[S_VX * cap + c]);
+      v[7] = order_key(st[S_VZ * cap + c]);
+    }
+  }
+  if (__any(due)) wave_minmax8(acc, inst, v, due, 0x78u /* max: 3,4,5,6 */);
+  if constexpr (NEXT) {
+    __syncthreads();  // the pair scratch is free: it now holds the workgroup's records
+    double *rec = reinterpret_cast<double *>(&sh);
+    if (c < n) {
+      while (true) {
+        const int cur = __builtin_amdgcn_readfirstlane(inst);
+        if (inst == cur) {
+          const InstParams &q = P[cur];
+          predict_clump(st, sta, rec + lane * kStagePad, q, orig, nullptr, nullptr, nullptr, c,
+                        cap, 2, lt, q.time + q.dt);
+          break;
+        }
+      }
+    }
+    __syncthreads();
+    const int64_t rows = n - c0 < BS ? n - c0 : BS;
+    double *dst = stage_out + c0 * kStageDoubles;
+    for (int64_t k = lane; k < rows * kStageDoubles; k += BS)
+      dst[k] = rec[(k / kStageDoubles) * kStagePad + k % kStageDoubles];
   }
 }
 
@@ -1144,41 +1272,6 @@ __global__ __launch_bounds__(256) void k_cellreduce(const double *__restrict__ s
                 0x78u /* max: 3,4,5,6 */);
 }
 
-// cell_dim.cu:55-60 margins and the regrid arithmetic of avalanche_hst.cu:311-333 when the
-// regrid is due (or forced); then cc++ (avalanche_hst.cu:600, 852) when step is set.
-__device__ __forceinline__ void finalize_params(InstParams &p, long long *a, int force, int step) {
-  if (force || (p.regrid_interval > 0 && p.cc % p.regrid_interval == 0)) {
-    const double rgyr = 0.5 * p.gyr_max, ddt = p.regrid_ddt_steps * p.dt;
-    double rmin[3], rmax[3];
-    for (int k = 0; k < 3; k++) {
-      rmin[k] = order_val(a[k]);
-      rmax[k] = order_val(a[3 + k]);
-    }
-    const double vxmax = order_val(a[6]), vzmin = order_val(a[7]);
-    rmax[0] += 1.05 * rgyr + vxmax * ddt;
-    rmin[0] -= 1.05 * rgyr;
-    rmax[1] += 1.05 * rgyr;
-    rmin[1] -= 1.05 * rgyr;
-    rmax[2] += 1.05 * rgyr;
-    rmin[2] -= 1.05 * rgyr - vzmin * ddt;
-    const double dist = p.cell_dist;
-    for (int k = 0; k < 3; k++) {
-      p.rr_min[k] = rmin[k];
-      p.rr_max[k] = rmax[k];
-      p.length[k] = rmax[k] - rmin[k];
-      p.ncell[k] = (int64_t)(p.length[k] / (dist * p.rmax));
-      p.cell_side[k] = p.length[k] / ((double)p.ncell[k]);
-      p.cs_inv[k] = 1.0 / p.cell_side[k];
-    }
-    for (int k = 0; k < 3; k++) {
-      a[k] = 0x7fffffffffffffffll;
-      a[3 + k] = (long long)0x8000000000000000ull;
-    }
-    a[6] = (long long)0x8000000000000000ull;
-    a[7] = 0x7fffffffffffffffll;
-  }
-  if (step) p.cc += 1;
-}
 
 // Per instance: finalize_params (step = 0 recomputes the grid without counting a step).
 // advance_time: P.time += dt (candidate-list steps, whose force stage used P.time + dt).
