@@ -52,6 +52,14 @@ __device__ __forceinline__ d3 sphere_of(d3 rr, const double *R, double pos, int 
                  rr.z + pos * (sx * R[6] + sy * R[7] + sz * R[8]));
 }
 
+// One of four values by a select chain on registers (no indexed array, no scratch).
+__device__ __forceinline__ d3 sel4(int i, d3 a, d3 b, d3 c, d3 d) {
+  const bool lo = i < 2, odd = i & 1;
+  return make_d3(lo ? (odd ? b.x : a.x) : (odd ? d.x : c.x),
+                 lo ? (odd ? b.y : a.y) : (odd ? d.y : c.y),
+                 lo ? (odd ? b.z : a.z) : (odd ? d.z : c.z));
+}
+
 __device__ __forceinline__ d3 pick(const d3 *s, int i) {
   d3 r = s[0];
   if (i == 1) r = s[1];
@@ -365,7 +373,7 @@ __global__ void k_scatter(const uint32_t *__restrict__ skey, const uint32_t *__r
 
 // force_hop.cu:9-244 for one nw (0: x walls, 1: y walls, 2: bottom gate while time < 0).
 __device__ __forceinline__ void hop_walls(const InstParams &p, int nw, double time, d3 rr, d3 vv, d3 ww,
-                          const d3 *sph, double ri, double mass, double rb, d3 &ff, d3 &tq) {
+                          const double *R, double pos, double ri, double mass, double rb, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
   const double xfront = p.hop_width, xrear = 0.0, yleft = p.yhop_l, yright = p.yhop_r;
   const double zbottom = p.hop_height;
@@ -374,8 +382,9 @@ __device__ __forceinline__ void hop_walls(const InstParams &p, int nw, double ti
   if (nw == 0 && rr.x - rb - xrear > ri && xfront - rr.x - rb > ri) return;
   if (nw == 1 && rr.y - rb - yleft > ri && yright - rr.y - rb > ri) return;
   if (nw == 2 && (!(time < 0.0) || rr.z - rb - zbottom > ri)) return;
+#pragma unroll 1
   for (int ns = 0; ns < 4; ns++) {
-    d3 r = pick(sph, ns), rrc = make_d3(0, 0, 0), nvec = make_d3(0, 0, 0);
+    d3 r = sphere_of(rr, R, pos, ns), rrc = make_d3(0, 0, 0), nvec = make_d3(0, 0, 0);
     double dist, comp = 0.0;
     int touch = 0;
     if (r.z < zbottom || r.x > p.hop_width) continue;
@@ -425,7 +434,7 @@ __device__ __forceinline__ void hop_walls(const InstParams &p, int nw, double ti
 }
 
 // force_funnel.cu:9-213: converging side planes of the funnel (slope frame).
-__device__ __forceinline__ void funnel_walls(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph, double ri,
+__device__ __forceinline__ void funnel_walls(const InstParams &p, d3 rr, d3 vv, d3 ww, const double *R, double pos, double ri,
                              double mass, double rb, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
   const double sinfi = p.sinfi, cosfi = p.cosfi, tanfi = p.tanfi, cost = p.cost, sint = p.sint;
@@ -436,8 +445,9 @@ __device__ __forceinline__ void funnel_walls(const InstParams &p, d3 rr, d3 vv, 
         fabs(p.yfunnel - qc.y - qc.x * tanfi) * cosfi - rb > ri)
       return;
   }
+#pragma unroll 1
   for (int ns = 0; ns < 4; ns++) {
-    d3 q = to_slope(pick(sph, ns), cost, sint), rrc = make_d3(0, 0, 0), nvec = make_d3(0, 0, 0);
+    d3 q = to_slope(sphere_of(rr, R, pos, ns), cost, sint), rrc = make_d3(0, 0, 0), nvec = make_d3(0, 0, 0);
     double dist, comp = 0.0;
     int touch = 0;
     if (q.x > p.lfunnel - ri * sinfi) continue;
@@ -464,7 +474,7 @@ __device__ __forceinline__ void funnel_walls(const InstParams &p, d3 rr, d3 vv, 
 }
 
 // force_funnel.cu:216-404: vertical corner edges at the funnel exit.
-__device__ __forceinline__ void funnel_corners(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph,
+__device__ __forceinline__ void funnel_corners(const InstParams &p, d3 rr, d3 vv, d3 ww, const double *R, double pos,
                                double ri, double mass, double rb, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
   const double yfun_l = p.dy2, yfun_r = p.side_y + p.dy2, hsy = p.dy2 + p.side_y / 2.0;
@@ -473,8 +483,9 @@ __device__ __forceinline__ void funnel_corners(const InstParams &p, d3 rr, d3 vv
     const d3 qc = to_slope(rr, cost, sint);
     if (qc.x + rb < lf - ri * p.sinfi || qc.x - rb > lf + ri) return;
   }
+#pragma unroll 1
   for (int ns = 0; ns < 4; ns++) {
-    d3 q = to_slope(pick(sph, ns), cost, sint);
+    d3 q = to_slope(sphere_of(rr, R, pos, ns), cost, sint);
     if (q.x < lf - ri * p.sinfi || q.x > lf + ri) continue;
     if (q.y > yfun_r || q.y < yfun_l) continue;
     double yc = (q.y <= hsy) ? yfun_l : yfun_r;
@@ -495,15 +506,16 @@ __device__ __forceinline__ void funnel_corners(const InstParams &p, d3 rr, d3 vv
 }
 
 // force_walls.cu:9-199: flume side walls beyond the funnel (caller skips frozen clumps).
-__device__ __forceinline__ void side_walls(const InstParams &p, double time, d3 rr, d3 vv, d3 ww, const d3 *sph,
+__device__ __forceinline__ void side_walls(const InstParams &p, double time, d3 rr, d3 vv, d3 ww, const double *R, double pos,
                            double ri, double mass, double rb, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass;
   double gm = 0.5 * p.gamma_pw * mass, mu = p.mu_pw, cns_1 = p.fr_pw;
   const double yleft = p.dy2, yright = p.side_y + p.dy2;
   if (to_slope(rr, p.cost, p.sint).x + rb < p.lfunnel) return;
   if (rr.y - rb - yleft > ri && yright - rr.y - rb > ri) return;
+#pragma unroll 1
   for (int ns = 0; ns < 4; ns++) {
-    d3 r = pick(sph, ns), q = to_slope(r, p.cost, p.sint);
+    d3 r = sphere_of(rr, R, pos, ns), q = to_slope(r, p.cost, p.sint);
     d3 rrc = make_d3(0, 0, 0), nvec = make_d3(0, 0, 0);
     double dist, comp = 0.0;
     int touch = 0;
@@ -529,13 +541,14 @@ __device__ __forceinline__ void side_walls(const InstParams &p, double time, d3 
 }
 
 // force_flume.cu:10-182: inclined floor z = -x tan(theta).
-__device__ __forceinline__ void flume_floor(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph, double ri,
+__device__ __forceinline__ void flume_floor(const InstParams &p, d3 rr, d3 vv, d3 ww, const double *R, double pos, double ri,
                             double mass, double rb, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
   const double sint = p.sint, cost = p.cost, tant = sint / cost;
   if (fabs(cost * (-tant * rr.x - rr.z)) - rb > ri) return;
+#pragma unroll 1
   for (int ns = 0; ns < 4; ns++) {
-    d3 r = pick(sph, ns);
+    d3 r = sphere_of(rr, R, pos, ns);
     double rect = -tant * r.x - r.z, dist = fabs(cost * rect);
     if (!(dist <= ri)) continue;
     d3 nvec = make_d3(-sint, 0.0, -cost);
@@ -546,13 +559,14 @@ __device__ __forceinline__ void flume_floor(const InstParams &p, d3 rr, d3 vv, d
 }
 
 // force_dep.cu:10-178 with the deposit plane of avf_geometry.
-__device__ __forceinline__ void deposit_floor(const InstParams &p, d3 rr, d3 vv, d3 ww, const d3 *sph, double ri,
+__device__ __forceinline__ void deposit_floor(const InstParams &p, d3 rr, d3 vv, d3 ww, const double *R, double pos, double ri,
                               double mass, double rb, d3 &ff, d3 &tq) {
   const double km = 0.5 * p.kappa_pw * mass, gm = 0.5 * p.gamma_pw * mass;
   const double sind = p.dep_sin, cosd = p.dep_cos, tand = sind / cosd;
   if (cosd * (-(tand * (rr.x - p.dep_x0 - p.dep_x1) - (rr.z - p.dep_z0))) - rb > ri) return;
+#pragma unroll 1
   for (int ns = 0; ns < 4; ns++) {
-    d3 r = pick(sph, ns);
+    d3 r = sphere_of(rr, R, pos, ns);
     double rect = tand * (r.x - p.dep_x0 - p.dep_x1) - (r.z - p.dep_z0), dist = cosd * (-rect);
     if (!(dist <= ri)) continue;
     d3 nvec = make_d3(sind, 0.0, -cosd);
@@ -571,17 +585,48 @@ __device__ __forceinline__ void deposit_floor(const InstParams &p, d3 rr, d3 vv,
 // cs_inv, cells.cu:69-77) as the linear index cell_locate stores it (cells.cu:91; an index
 // above ncell aliases into another cell exactly as in the reference).  Returns -1 if the
 // reference would abort (negative index) or write outside its cell table.
-__device__ __forceinline__ int64_t ref_cell_locate(const InstParams &p, d3 r) {
-  int64_t ii = (int64_t)(p.cs_inv[0] * (r.x - p.rr_min[0]));
+__device__ __forceinline__ void ref_cell_raw(const InstParams &p, d3 r, int64_t &ii, int64_t &jj,
+                                             int64_t &kk) {
+  ii = (int64_t)(p.cs_inv[0] * (r.x - p.rr_min[0]));
   if (ii == p.ncell[0]) ii--;
-  int64_t jj = (int64_t)(p.cs_inv[1] * (r.y - p.rr_min[1]));
+  jj = (int64_t)(p.cs_inv[1] * (r.y - p.rr_min[1]));
   if (jj == p.ncell[1]) jj--;
-  int64_t kk = (int64_t)(p.cs_inv[2] * (r.z - p.rr_min[2]));
+  kk = (int64_t)(p.cs_inv[2] * (r.z - p.rr_min[2]));
   if (kk == p.ncell[2]) kk--;
+}
+
+__device__ __forceinline__ int64_t ref_cell_locate(const InstParams &p, d3 r) {
+  int64_t ii, jj, kk;
+  ref_cell_raw(p, r, ii, jj, kk);
   if (ii < 0 || jj < 0 || kk < 0) return -1;
   int64_t idx = ii + p.ncell[0] * (jj + kk * p.ncell[1]);
   if (idx >= p.ncell[0] * p.ncell[1] * p.ncell[2]) return -1;
   return idx;
+}
+
+// The cell of ref_cell_locate as (i, j, k) = (L mod nx, (L / nx) mod ny, L / (nx ny)); false
+// where ref_cell_locate returns -1.  Inside the grid the raw indices are the answer, so the
+// 64-bit divisions run only for the aliased indices of clumps beyond the grid.
+__device__ __forceinline__ bool ref_cell_ijk(const InstParams &p, d3 r, int64_t &ai, int64_t &aj,
+                                             int64_t &ak) {
+  int64_t ii, jj, kk;
+  ref_cell_raw(p, r, ii, jj, kk);
+  if (ii < 0 || jj < 0 || kk < 0) return false;
+  if (ii < p.ncell[0] && jj < p.ncell[1]) {
+    if (kk >= p.ncell[2]) return false;  // L >= nx ny nz
+    ai = ii;
+    aj = jj;
+    ak = kk;
+    return true;
+  }
+  const int64_t nxy = p.ncell[0] * p.ncell[1];
+  const int64_t L = ii + p.ncell[0] * (jj + kk * p.ncell[1]);
+  if (L >= nxy * p.ncell[2]) return false;
+  ak = L / nxy;
+  const int64_t rem = L - ak * nxy;
+  aj = rem / p.ncell[0];
+  ai = rem - aj * p.ncell[0];
+  return true;
 }
 
 constexpr int kSphList = 16;  // touching spheres of other clumps per sphere
@@ -687,7 +732,6 @@ __global__ __launch_bounds__(BS) void k_vmerge(
   if (lj == p.ncell[1]) lj--;
   int64_t lk = (int64_t)((rr.z - p.rr_min[2]) / p.cell_side[2]);
   if (lk == p.ncell[2]) lk--;
-  const int64_t nxy = p.ncell[0] * p.ncell[1];
   int cnt = 0;
   bool overflow = false;
   for (int s = 0; s < 4; s++) {
@@ -704,11 +748,9 @@ __global__ __launch_bounds__(BS) void k_vmerge(
         continue;
       }
       const double *g = stage + cm * kStageDoubles;
-      const int64_t L = ref_cell_locate(p, make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]));
+      int64_t ai, aj, ak;
       uint32_t off = 27u;
-      if (L >= 0) {
-        const int64_t ak = L / nxy, rem = L - ak * nxy;
-        const int64_t aj = rem / p.ncell[0], ai = rem - aj * p.ncell[0];
+      if (ref_cell_ijk(p, make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]), ai, aj, ak)) {
         const int64_t di = ai - li, dj = aj - lj, dk = ak - lk;
         if (di >= -1 && di <= 1 && dj >= -1 && dj <= 1 && dk >= -1 && dk <= 1)
           off = (uint32_t)((di + 1) * 9 + (dj + 1) * 3 + (dk + 1));
@@ -845,8 +887,6 @@ __device__ __forceinline__ void force_clump(
   const double rad = sta[C_RAD * cap + c], rad0 = sta[C_RAD0 * cap + c];
   const double rho = sta[C_RHO * cap + c];
   const double pos0 = rad0 / p.sphere_div;
-  d3 sph[4];
-  sphere_centres(rr, R, pos0, sph);
   // bound on |sphere centre - rr| (sqrt(3) pos0; R has unit columns) with a margin far above
   // the rounding of the centres, for the boundary modules' early outs
   const double rb = 1.7320508075688772 * pos0 * (1.0 + 1e-12) + 1e-9;
@@ -858,10 +898,11 @@ __device__ __forceinline__ void force_clump(
 
   if (p.phase) {  // avalanche_hst.cu:688-696
     if (mods & AVF_MOD_HOP)
-      for (int nw = 0; nw < 3; nw++) hop_walls(p, nw, time, rr, vv, ww, sph, rad, massw, rb, ff, tq);
-    if (mods & AVF_MOD_FUNNEL) funnel_walls(p, rr, vv, ww, sph, rad, massw, rb, ff, tq);
-    if (mods & AVF_MOD_FCORNERS) funnel_corners(p, rr, vv, ww, sph, rad, massw, rb, ff, tq);
-    if ((mods & AVF_MOD_WALLS) && !frz) side_walls(p, time, rr, vv, ww, sph, rad, massw, rb, ff, tq);
+#pragma unroll 1
+      for (int nw = 0; nw < 3; nw++) hop_walls(p, nw, time, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
+    if (mods & AVF_MOD_FUNNEL) funnel_walls(p, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
+    if (mods & AVF_MOD_FCORNERS) funnel_corners(p, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
+    if ((mods & AVF_MOD_WALLS) && !frz) side_walls(p, time, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
   }
 
   if ((mods & AVF_MOD_PP) && !frz) {
@@ -875,35 +916,46 @@ __device__ __forceinline__ void force_clump(
     if (lj == p.ncell[1]) lj--;
     int64_t lk = (int64_t)((rr.z - p.rr_min[2]) / p.cell_side[2]);
     if (lk == p.ncell[2]) lk--;
-    const int64_t nxy = p.ncell[0] * p.ncell[1];
     if constexpr (VERLET) {
       const int kc = vl_cnt[c];
+      // the next entry is fetched one iteration ahead and a candidate's centre, radius, id
+      // and sphere centres are loaded together, so an entry waits for one gather
+      int64_t cm_next = kc > 0 ? vl_m[c] : 0;
+      uint32_t bits_next = kc > 0 ? vl_mask[c] : 0u;
       for (int e = 0; e < kc; e++) {
-        const int64_t cm = vl_m[e * cap + c];
+        const int64_t cm = cm_next;
+        uint32_t bits = bits_next;
+        if (e + 1 < kc) {
+          cm_next = vl_m[(e + 1) * cap + c];
+          bits_next = vl_mask[(e + 1) * cap + c];
+        }
         const double *g = stage + cm * kStageDoubles;
         const d3 rm = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
+        const double radm = g[G_RAD], idm = g[G_ID];
+        const d3 sm0 = make_d3(g[G_SPH + 0], g[G_SPH + 1], g[G_SPH + 2]);
+        const d3 sm1 = make_d3(g[G_SPH + 3], g[G_SPH + 4], g[G_SPH + 5]);
+        const d3 sm2 = make_d3(g[G_SPH + 6], g[G_SPH + 7], g[G_SPH + 8]);
+        const d3 sm3 = make_d3(g[G_SPH + 9], g[G_SPH + 10], g[G_SPH + 11]);
         if (frozen(time, rm.x, p.side_x)) continue;  // not binned at this time (cells.cu:65)
         // the candidate pairings that touch now: k_detect's test on the same operands
-        const double sep = rad + g[G_RAD], sep2 = sep * sep;
-        uint32_t bits = vl_mask[e * cap + c], mask = 0u;
+        const double sep = rad + radm, sep2 = sep * sep;
+        uint32_t mask = 0u;
         while (bits) {
           const int pr = __builtin_ctz(bits);
           bits &= bits - 1u;
           const int sdiv = pr >> 2, smod = pr & 3;
           const d3 q = sphere_of(rr, R, pos0, smod);
-          const double dx = g[G_SPH + 3 * sdiv] - q.x, dy = g[G_SPH + 3 * sdiv + 1] - q.y;
-          const double dz = g[G_SPH + 3 * sdiv + 2] - q.z;
+          const d3 o = sel4(sdiv, sm0, sm1, sm2, sm3);
+          const double dx = o.x - q.x, dy = o.y - q.y, dz = o.z - q.z;
           if (dx * dx + dy * dy + dz * dz < sep2) mask |= 1u << pr;
         }
         if (!mask) continue;
-        const uint32_t morig = (uint32_t)__double_as_longlong(g[G_ID]);
-        const int64_t L = ref_cell_locate(p, rm);
-        if (L < 0) {
+        const uint32_t morig = (uint32_t)__double_as_longlong(idm);
+        int64_t ai, aj, ak;
+        if (!ref_cell_ijk(p, rm, ai, aj, ak)) {
           atomicAdd(&diag->cell_below, 1ull);
           continue;
         }
-        const int64_t ak = L / nxy, rem = L - ak * nxy;
-        const int64_t aj = rem / p.ncell[0], ai = rem - aj * p.ncell[0];
         const int64_t di = ai - li, dj = aj - lj, dk = ak - lk;
         if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) {
           atomicAdd(&diag->cell_beyond, 1ull);
@@ -946,13 +998,11 @@ __device__ __forceinline__ void force_clump(
             const double *g = stage + cm * kStageDoubles;
             const d3 rm = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
             const uint32_t morig = (uint32_t)__double_as_longlong(g[G_ID]);
-            const int64_t L = ref_cell_locate(p, rm);
-            if (L < 0) {
+            int64_t ai, aj, ak;
+            if (!ref_cell_ijk(p, rm, ai, aj, ak)) {
               atomicAdd(&diag->cell_below, 1ull);
               continue;
             }
-            const int64_t ak = L / nxy, rem = L - ak * nxy;
-            const int64_t aj = rem / p.ncell[0], ai = rem - aj * p.ncell[0];
             const int64_t di = ai - li, dj = aj - lj, dk = ak - lk;
             if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) {
               // outside the 27 clump cells the reference visits (force_part.cu:88-96)
@@ -988,18 +1038,32 @@ __device__ __forceinline__ void force_clump(
     const double massl = kSphFact * rho * rad * rad * rad;
     const double kappa = p.kappa_pp, gamma = p.gamma_pp, mu = p.mu_pp, cns_1 = p.fr_pp;
     const double dt = p.dt;
+    // transpose the entry masks into per-pairing entry sets (registers, then LDS) and note
+    // which pairings have a contact at all
+    uint32_t nz = 0u;
+    {
+      uint32_t pmr[16];
 #pragma unroll
-    for (int pr = 0; pr < 16; pr++) {
-      uint32_t bits = 0;
-      for (int e = 0; e < cnt; e++) bits |= (uint32_t)((sh.mask[e][lane] >> pr) & 1u) << e;
-      sh.pm[pr][lane] = bits;
+      for (int k = 0; k < 16; k++) pmr[k] = 0u;
+      for (int e = 0; e < cnt; e++) {
+        const uint32_t mk = sh.mask[e][lane];
+        nz |= mk;
+#pragma unroll
+        for (int k = 0; k < 16; k++) pmr[k] |= ((mk >> k) & 1u) << e;
+      }
+#pragma unroll
+      for (int k = 0; k < 16; k++) sh.pm[k][lane] = pmr[k];
     }
     // walk the set bits lane by lane so every lane evaluates only its own contacts
     int pr = 0;
-    uint32_t cur = sh.pm[0][lane];
+    uint32_t cur = 0u;
     while (true) {
-      while (cur == 0u && pr < 15) cur = sh.pm[++pr][lane];
-      if (cur == 0u) break;
+      if (cur == 0u) {
+        if (nz == 0u) break;
+        pr = __builtin_ctz(nz);
+        nz &= nz - 1u;
+        cur = sh.pm[pr][lane];
+      }
       const int e = __builtin_ctz(cur);
       cur &= cur - 1u;
       const int sdiv = pr >> 2, smod = pr & 3;
@@ -1035,10 +1099,11 @@ __device__ __forceinline__ void force_clump(
 
   if (!p.phase) {  // avalanche_hst.cu:460-461 (pp first, then hop, while filling)
     if (mods & AVF_MOD_HOP)
-      for (int nw = 0; nw < 3; nw++) hop_walls(p, nw, time, rr, vv, ww, sph, rad, massw, rb, ff, tq);
+#pragma unroll 1
+      for (int nw = 0; nw < 3; nw++) hop_walls(p, nw, time, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
   } else {  // avalanche_hst.cu:704
-    if ((mods & AVF_MOD_FLUME) && !frz) flume_floor(p, rr, vv, ww, sph, rad, massw, rb, ff, tq);
-    if (mods & AVF_MOD_DEP) deposit_floor(p, rr, vv, ww, sph, rad, massw, rb, ff, tq);
+    if ((mods & AVF_MOD_FLUME) && !frz) flume_floor(p, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
+    if (mods & AVF_MOD_DEP) deposit_floor(p, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
   }
 
   if (fout) {  // F / T of the step, kept for export (last step of a launch sequence)

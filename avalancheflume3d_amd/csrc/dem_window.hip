@@ -70,11 +70,8 @@ __device__ __forceinline__ void ref_cell_of(const InstParams &p, double x, doubl
 // (force_part.cu:88-96).  Clumps below or beyond the clump-cell grid are not visited.
 __device__ __forceinline__ bool ref_visits(const InstParams &p, int64_t li, int64_t lj, int64_t lk,
                                            double mx, double my, double mz) {
-  const int64_t L = ref_cell_locate(p, make_d3(mx, my, mz));
-  if (L < 0) return false;
-  const int64_t nxy = p.ncell[0] * p.ncell[1];
-  const int64_t ak = L / nxy, rem = L - ak * nxy;
-  const int64_t aj = rem / p.ncell[0], ai = rem - aj * p.ncell[0];
+  int64_t ai, aj, ak;
+  if (!ref_cell_ijk(p, make_d3(mx, my, mz), ai, aj, ak)) return false;
   const int64_t di = ai - li, dj = aj - lj, dk = ak - lk;
   return di >= -1 && di <= 1 && dj >= -1 && dj <= 1 && dk >= -1 && dk <= 1;
 }
@@ -127,7 +124,6 @@ __global__ __launch_bounds__(64) void k_win_list(
   if (lj == p.ncell[1]) lj--;
   int64_t lk = (int64_t)((rr.z - p.rr_min[2]) / p.cell_side[2]);
   if (lk == p.ncell[2]) lk--;
-  const int64_t nxy = p.ncell[0] * p.ncell[1];
   const int nx = p.sg_n[0], ny = p.sg_n[1], nz = p.sg_n[2];
   // sphere cells (side >= 2 r_max) must cover sep + skin: widen the sweep by the skin
   const int reach = 1 + (int)ceil(skin * p.sg_inv_h);
@@ -160,12 +156,10 @@ __global__ __launch_bounds__(64) void k_win_list(
             continue;
           }
           const double *g = stage + cm * kStageDoubles;
-          const int64_t L = ref_cell_locate(p, make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]));
           // clumps outside the reference's 27 clump cells (or out of its grid) go last
+          int64_t ai, aj, ak;
           uint32_t off = 27u;
-          if (L >= 0) {
-            const int64_t ak = L / nxy, rem = L - ak * nxy;
-            const int64_t aj = rem / p.ncell[0], ai = rem - aj * p.ncell[0];
+          if (ref_cell_ijk(p, make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]), ai, aj, ak)) {
             const int64_t di = ai - li, dj = aj - lj, dk = ak - lk;
             if (di >= -1 && di <= 1 && dj >= -1 && dj <= 1 && dk >= -1 && dk <= 1)
               off = (uint32_t)((di + 1) * 9 + (dj + 1) * 3 + (dk + 1));
