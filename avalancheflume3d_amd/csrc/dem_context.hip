@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <string>
 #include <vector>
@@ -205,12 +206,12 @@ ListTrack list_track(avf_ctx *c) { return ListTrack{c->d_xb, c->d_vrot, c->d_dsp
 
 template <int BS, bool VERLET, bool NEXT>
 void launch_force_bs(avf_ctx *c, int initial, bool ft, int time_ahead, const double *stage_in,
-                     double *stage_out) {
+                     double *stage_out, int si) {
   k_force<kMaxNeigh, BS, VERLET, NEXT><<<grid_for(c->n, BS), BS, 0, c->stream>>>(
       c->d_st, c->d_sta, stage_in, c->d_inst, c->d_P, c->d_clist, c->d_ccnt, c->d_vl_m,
       c->d_vl_mask, c->d_vl_cnt, ft ? c->d_f : nullptr, c->d_diag, c->d_acc_cell, stage_out,
       c->d_orig, NEXT ? list_track(c) : ListTrack{nullptr, nullptr, nullptr}, c->n, c->cap,
-      initial, time_ahead);
+      initial, time_ahead, si);
 }
 
 // Touching spheres of the staged spheres (skin 0) or candidate spheres within sep + skin.
@@ -229,26 +230,29 @@ void launch_detect(avf_ctx *c, double skin = 0.0) {
 // keep the step's forces and torques for avf_get_forces (the last step of a sequence).
 // verlet: touching clumps from the candidate lists; time_ahead: see k_force.  stage_in: the
 // step's stage records (default d_stage); stage_out (candidate lists only): also predict the
-// next step into that buffer.
+// next step into that buffer.  si: step index within a chunk of candidate-list steps.
 void launch_force(avf_ctx *c, int initial, bool ft = true, bool verlet = false,
                   int time_ahead = 0, const double *stage_in = nullptr,
-                  double *stage_out = nullptr) {
+                  double *stage_out = nullptr, int si = 0) {
   if (!stage_in) stage_in = c->d_stage;
   const bool next = verlet && stage_out;
   if (c->opt.block_clumps >= 128) {
     if (next)
-      launch_force_bs<128, true, true>(c, initial, ft, time_ahead, stage_in, stage_out);
+      launch_force_bs<128, true, true>(c, initial, ft, time_ahead, stage_in, stage_out, si);
     else if (verlet)
-      launch_force_bs<128, true, false>(c, initial, ft, time_ahead, stage_in, nullptr);
+      launch_force_bs<128, true, false>(c, initial, ft, time_ahead, stage_in, nullptr, si);
     else
-      launch_force_bs<128, false, false>(c, initial, ft, time_ahead, stage_in, nullptr);
+      launch_force_bs<128, false, false>(c, initial, ft, time_ahead, stage_in, nullptr, si);
   } else {
     if (next)
-      launch_force_bs<kForceBlock, true, true>(c, initial, ft, time_ahead, stage_in, stage_out);
+      launch_force_bs<kForceBlock, true, true>(c, initial, ft, time_ahead, stage_in, stage_out,
+                                               si);
     else if (verlet)
-      launch_force_bs<kForceBlock, true, false>(c, initial, ft, time_ahead, stage_in, nullptr);
+      launch_force_bs<kForceBlock, true, false>(c, initial, ft, time_ahead, stage_in, nullptr,
+                                                si);
     else
-      launch_force_bs<kForceBlock, false, false>(c, initial, ft, time_ahead, stage_in, nullptr);
+      launch_force_bs<kForceBlock, false, false>(c, initial, ft, time_ahead, stage_in, nullptr,
+                                                 si);
   }
 }
 
@@ -288,10 +292,12 @@ void launch_step(avf_ctx *c, hipEvent_t *ev = nullptr, bool ft = true) {
 // One step on candidate lists.  rebuild: bin the predicted spheres and rebuild the lists
 // (k_predict keys, scans, scatter advancing P.time, k_detect with the skin, k_vmerge), then
 // the force stage.  Otherwise step i of a chunk of S list steps: k_predict without keys
-// (tracking displacements) for the first step only; k_force runs the force stage at
-// P.time + dt and, except on the last step, predicts the next step into the other stage
-// buffer; k_finalize advances P.time.  The kernel slots of avf_step_profiled are kept (the
-// binning slots are empty on list steps, the predict slot after the first step of a chunk).
+// (tracking displacements) for the first step only; k_force runs the force stage at the
+// step's time + dt and, except on the last step, predicts the next step into the other
+// stage buffer; after the last step one k_finalize advances P by the chunk's S steps (the
+// caller ends a chunk on any step whose regrid is due).  The kernel slots of
+// avf_step_profiled are kept (the binning slots are empty on list steps, the predict slot
+// after the first step of a chunk, the finalize slot before its last).
 void launch_vstep(avf_ctx *c, bool rebuild, hipEvent_t *ev = nullptr, bool ft = true,
                   int64_t i = 0, int64_t S = 1) {
   auto mark = [&](int k) {
@@ -328,12 +334,16 @@ void launch_vstep(avf_ctx *c, bool rebuild, hipEvent_t *ev = nullptr, bool ft = 
     launch_force(c, 0, ft, true, 0);
   } else {
     double *buf[2] = {c->d_stage, c->d_stage2};
-    launch_force(c, 0, ft, true, 1, buf[i & 1], i + 1 < S ? buf[(i + 1) & 1] : nullptr);
+    launch_force(c, 0, ft, true, 1, buf[i & 1], i + 1 < S ? buf[(i + 1) & 1] : nullptr, (int)i);
   }
   mark(AVF_K_CELLREDUCE);  // the regrid reduction runs in k_force
   mark(AVF_K_FINALIZE);
-  k_finalize<<<grid_for(c->ninst, 64), 64, 0, c->stream>>>(c->d_P, c->d_acc_cell, c->ninst, 0, 1,
-                                                           rebuild ? 0 : 1);
+  if (rebuild)
+    k_finalize<<<grid_for(c->ninst, 64), 64, 0, c->stream>>>(c->d_P, c->d_acc_cell, c->ninst, 0,
+                                                             1, 0, 1);
+  else if (i + 1 == S)
+    k_finalize<<<grid_for(c->ninst, 64), 64, 0, c->stream>>>(c->d_P, c->d_acc_cell, c->ninst, 0,
+                                                             1, 1, (int)S);
   mark(AVF_NKERNELS);
 }
 
@@ -439,6 +449,24 @@ int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
   const size_t SF = (size_t)c->cap * S_NFIELDS * sizeof(double);
   int rc;
   int64_t left = nsteps;
+  // clump-cell regrid schedule: a chunk's k_finalize applies its steps at its end, so every
+  // chunk ends on a step whose regrid is due (cc % regrid_interval == 0), if it has one
+  std::vector<InstParams> hp((size_t)c->ninst);
+  HIPCHK(c, hipMemcpyAsync(hp.data(), c->d_P, c->ninst * sizeof(InstParams),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  auto to_regrid = [&hp]() {  // steps up to and including the next due one
+    int64_t lim = std::numeric_limits<int64_t>::max();
+    for (const InstParams &q : hp)
+      if (q.regrid_interval > 0) {
+        const int64_t iv = q.regrid_interval, r = ((q.cc % iv) + iv) % iv;
+        lim = std::min<int64_t>(lim, (iv - r) % iv + 1);
+      }
+    return lim;
+  };
+  auto advance = [&hp](int64_t k) {
+    for (InstParams &q : hp) q.cc += k;
+  };
   while (left > 0) {
     if (!c->sgrid_valid ||
         (c->opt.reorder_interval > 0 && c->steps_since_reorder >= c->opt.reorder_interval)) {
@@ -455,12 +483,14 @@ int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
       c->since_build = 0;
       c->steps_since_reorder++;
       c->steps_total++;
+      advance(1);
       left--;
       continue;
     }
     int64_t S = std::min(left, std::max<int64_t>(1, c->vchunk));
     if (c->opt.reorder_interval > 0)
       S = std::min<int64_t>(S, c->opt.reorder_interval - c->steps_since_reorder);
+    S = std::min(S, to_regrid());
     // checkpoint: state, radii (clumps may freeze), parameters (time, cc, grids), counters
     HIPCHK(c, hipMemcpyAsync(c->d_ck_st, c->d_st, SF, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_ck_rad, c->d_sta + (size_t)C_RAD * c->cap,
@@ -491,7 +521,8 @@ int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
     // largest displacement bound since the build
     unsigned long long um = 0;
     HIPCHK(c, hipMemsetAsync(c->d_umax, 0, sizeof um, c->stream));
-    k_win_max<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(c->d_dsp, c->n, c->d_umax);
+    k_win_max<<<std::min(kWinMaxGrid, grid_for(c->n, kWinMaxBlock)), kWinMaxBlock, 0,
+                c->stream>>>(c->d_dsp, c->n, c->d_umax);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(&um, c->d_umax, sizeof um, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -510,6 +541,7 @@ int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
       continue;
     }
     left -= S;
+    advance(S);
     c->steps_since_reorder += S;
     c->steps_total += S;
     c->since_build += S;
@@ -1673,7 +1705,8 @@ extern "C" int avf_window_run(avf_ctx *c, const avf_window_spec *spec, const dou
     HIPCHK(c, hipEventElapsedTime(&ms_bwd, b0, b1));
   }
   if (diag) {
-    k_win_max<<<grid_for(n, kBlock), kBlock, 0, c->stream>>>(w->dmax2, n, w->umax);
+    k_win_max<<<std::min(kWinMaxGrid, grid_for(n, kWinMaxBlock)), kWinMaxBlock, 0,
+                c->stream>>>(w->dmax2, n, w->umax);
     unsigned long long um = 0, miss = 0;
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(&um, w->umax, sizeof um, hipMemcpyDeviceToHost, c->stream));

@@ -81,7 +81,8 @@ struct ListTrack {
 // |x' - x_b| + sqrt(3) pos * sum |dOmega| of every sphere since the list build; mode 0:
 // stage the current state only (initial force pass, avalanche_hst.cu:365-384, whose force
 // stage sees the unchanged time).  t_old: the time at the start of the step (P.time, or
-// P.time + dt when the previous step's k_finalize has not run yet).
+// within a chunk of list steps chunk_time(), which has the same sign: only comparisons of
+// the time with 0 are made here).
 __device__ __forceinline__ void predict_clump(double *__restrict__ st, double *__restrict__ sta,
                                               double *__restrict__ g, const InstParams &p,
                                               const int32_t *__restrict__ orig,
@@ -863,11 +864,20 @@ __device__ __forceinline__ void wave_minmax8(long long *acc, int inst, const lon
   }
 }
 
+// The time of step si of a chunk whose P holds its first step, as far as the kernels use
+// it: only its comparisons with 0 enter them (frozen(), the hopper gate, the side walls'
+// friction switch).  The additions are k_finalize's, in its order, up to the point where the
+// time is positive: adding dt > 0 keeps it positive, so the sign is that of the exact time.
+__device__ __forceinline__ double chunk_time(const InstParams &q, int si) {
+  double t = q.time;
+  for (int k = 0; k < si && !(q.dt > 0.0 && t > 0.0); k++) t = t + q.dt;
+  return t;
+}
+
 // Main per-clump force pass.  One lane per clump (storage order = spatial order).  The
 // touching clumps come from k_detect's per-sphere lists or, with VERLET, from the candidate
 // list of k_vmerge, filtered every step by the same touching test and visit rule.
-// time_ahead: the step's time is P.time + dt (candidate-list steps advance P.time at the
-// end of the step instead of in k_scatter).
+// time: the force stage's time (sys.time after the step's advance).
 template <int MAXN, int BS, bool VERLET>
 __device__ __forceinline__ void force_clump(
     const InstParams &p, PPShared<MAXN, BS> &sh, int lane, int64_t c,
@@ -875,8 +885,7 @@ __device__ __forceinline__ void force_clump(
     const uint32_t *__restrict__ clist, const int32_t *__restrict__ ccnt,
     const int32_t *__restrict__ vl_m, const uint16_t *__restrict__ vl_mask,
     const int32_t *__restrict__ vl_cnt, double *__restrict__ fout, DevDiag *__restrict__ diag,
-    int64_t cap, int initial, int time_ahead) {
-  const double time = time_ahead ? p.time + p.dt : p.time;
+    int64_t cap, int initial, double time) {
 
   const d3 rr = make_d3(st[S_RX * cap + c], st[S_RY * cap + c], st[S_RZ * cap + c]);
   d3 vv = make_d3(st[S_VX * cap + c], st[S_VY * cap + c], st[S_VZ * cap + c]);
@@ -1180,8 +1189,13 @@ __device__ __forceinline__ void force_clump(
 // the next step's predict on its clump (init_Verlet needs only the clump's own finished
 // state) and writes the next stage record to stage_out, so a list step is one launch that
 // reads and writes the carried state once.  Neighbours of this step read the records of
-// `stage`, never stage_out.  The next step starts at P.time + dt (k_finalize of this step
-// runs after this kernel).
+// `stage`, never stage_out.
+//
+// si: step si of a chunk of candidate-list steps whose P still holds the chunk's first
+// step (k_finalize applies the chunk's per-step advances once, at its end): the step's
+// time is chunk_time(P, si), its counter P.cc + si.
+// time_ahead: the force stage runs at the step's time + dt (candidate-list steps advance
+// the time at the end of the step instead of in k_scatter).
 template <int MAXN, int BS, bool VERLET, bool NEXT>
 __global__ __launch_bounds__(BS) void k_force(
     double *__restrict__ st, double *__restrict__ sta, const double *__restrict__ stage,
@@ -1191,7 +1205,7 @@ __global__ __launch_bounds__(BS) void k_force(
     const int32_t *__restrict__ vl_cnt, double *__restrict__ fout, DevDiag *__restrict__ diag,
     long long *__restrict__ acc, double *__restrict__ stage_out,
     const int32_t *__restrict__ orig, ListTrack lt, int64_t n, int64_t cap, int initial,
-    int time_ahead) {
+    int time_ahead, int si) {
   static_assert(!NEXT || VERLET, "the fused predict runs on candidate-list steps only");
   static_assert(sizeof(PPShared<MAXN, BS>) >= sizeof(double) * BS * kStagePad,
                 "stage records of the workgroup are assembled in the pair scratch");
@@ -1204,8 +1218,11 @@ __global__ __launch_bounds__(BS) void k_force(
     while (true) {
       const int cur = __builtin_amdgcn_readfirstlane(inst);
       if (inst == cur) {
-        force_clump<MAXN, BS, VERLET>(P[cur], sh, lane, c, st, sta, stage, clist, ccnt, vl_m,
-                                      vl_mask, vl_cnt, fout, diag, cap, initial, time_ahead);
+        const InstParams &q = P[cur];
+        const double t = chunk_time(q, si);
+        force_clump<MAXN, BS, VERLET>(q, sh, lane, c, st, sta, stage, clist, ccnt, vl_m,
+                                      vl_mask, vl_cnt, fout, diag, cap, initial,
+                                      time_ahead ? t + q.dt : t);
         break;
       }
     }
@@ -1217,7 +1234,7 @@ __global__ __launch_bounds__(BS) void k_force(
   long long v[8];
   if (c < n) {
     const InstParams &q = P[inst];
-    due = q.regrid_interval > 0 && q.cc % q.regrid_interval == 0;
+    due = q.regrid_interval > 0 && (q.cc + si) % q.regrid_interval == 0;
     if (due) {
       v[0] = order_key(st[S_RX * cap + c]);
       v[1] = order_key(st[S_RY * cap + c]);
@@ -1237,9 +1254,9 @@ __global__ __launch_bounds__(BS) void k_force(
       while (true) {
         const int cur = __builtin_amdgcn_readfirstlane(inst);
         if (inst == cur) {
-          const InstParams &q = P[cur];
+          const InstParams &q = P[cur];  // the next step starts at this step's time + dt
           predict_clump(st, sta, rec + lane * kStagePad, q, orig, nullptr, nullptr, nullptr, c,
-                        cap, 2, lt, q.time + q.dt);
+                        cap, 2, lt, chunk_time(q, si) + q.dt);
           break;
         }
       }
@@ -1340,12 +1357,16 @@ __global__ __launch_bounds__(256) void k_cellreduce(const double *__restrict__ s
 
 // Per instance: finalize_params (step = 0 recomputes the grid without counting a step).
 // advance_time: P.time += dt (candidate-list steps, whose force stage used P.time + dt).
+// nsteps: the steps of a chunk of candidate-list steps, applied in order (a chunk ends on
+// any step whose regrid is due, so only its last step can regrid).
 __global__ void k_finalize(InstParams *__restrict__ P, long long *__restrict__ acc, int32_t ninst,
-                           int force, int step, int advance_time = 0) {
+                           int force, int step, int advance_time = 0, int nsteps = 1) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ninst) return;
-  finalize_params(P[i], acc + 8 * i, force, step);
-  if (advance_time) P[i].time = P[i].time + P[i].dt;
+  for (int s = 0; s < nsteps; s++) {  // the steps of a chunk, in order
+    finalize_params(P[i], acc + 8 * i, force, step);
+    if (advance_time) P[i].time = P[i].time + P[i].dt;
+  }
 }
 
 // --------------------------------------------------------------- sphere-cell grid ---

@@ -1181,14 +1181,28 @@ __global__ void k_win_export(const double *__restrict__ st, const double *__rest
 }
 
 // Largest value of a per-clump array (non-negative doubles) as an order-preserving integer.
-__global__ void k_win_max(const double *__restrict__ v, int64_t n, unsigned long long *out) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double m = c < n ? v[c] : 0.0;
+constexpr int kWinMaxBlock = 256;
+constexpr unsigned kWinMaxGrid = 512;
+
+// Grid-stride over v, one atomic per workgroup (launched with kWinMaxBlock threads and at
+// most kWinMaxGrid workgroups, so the atomics on *out stay few).
+__global__ __launch_bounds__(kWinMaxBlock) void k_win_max(const double *__restrict__ v,
+                                                          int64_t n, unsigned long long *out) {
+  __shared__ double red[kWinMaxBlock / 64];
+  double m = 0.0;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (int64_t)gridDim.x * blockDim.x)
+    m = v[c] > m ? v[c] : m;
   for (int off = 32; off > 0; off >>= 1) {
     const double q = __shfl_xor(m, off);
     m = q > m ? q : m;
   }
-  if ((threadIdx.x & 63) == 0 && m > 0.0) diag_max(out, (unsigned long long)__double_as_longlong(m));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kWinMaxBlock / 64; w++) m = red[w] > m ? red[w] : m;
+    if (m > 0.0) diag_max(out, (unsigned long long)__double_as_longlong(m));
+  }
 }
 
 }  // namespace avf
