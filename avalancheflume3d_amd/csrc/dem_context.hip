@@ -49,6 +49,7 @@ struct avf_ctx {
   double *d_st = nullptr, *d_st2 = nullptr, *d_sta = nullptr, *d_sta2 = nullptr;
   double *d_stage = nullptr, *d_f = nullptr, *d_io = nullptr, *d_io2 = nullptr;
   double *d_stage2 = nullptr;  // second stage buffer of the fused list steps
+  bool lean = false;           // avf_options.lean_state resolved at create
   int32_t *d_orig = nullptr, *d_orig2 = nullptr, *d_inst = nullptr, *d_inst2 = nullptr;
   int64_t *d_inv = nullptr, *d_ncells = nullptr, *d_tmp64 = nullptr;
   uint32_t *d_counts = nullptr, *d_starts = nullptr, *d_bsum = nullptr;
@@ -204,10 +205,10 @@ void launch_bin(avf_ctx *c, int mode) {
 
 ListTrack list_track(avf_ctx *c) { return ListTrack{c->d_xb, c->d_vrot, c->d_dsp}; }
 
-template <int BS, bool VERLET, bool NEXT>
+template <int BS, bool VERLET, bool NEXT, bool LEAN = false>
 void launch_force_bs(avf_ctx *c, int initial, bool ft, int time_ahead, const double *stage_in,
                      double *stage_out, int si) {
-  k_force<kMaxNeigh, BS, VERLET, NEXT><<<grid_for(c->n, BS), BS, 0, c->stream>>>(
+  k_force<kMaxNeigh, BS, VERLET, NEXT, LEAN><<<grid_for(c->n, BS), BS, 0, c->stream>>>(
       c->d_st, c->d_sta, stage_in, c->d_inst, c->d_P, c->d_clist, c->d_ccnt, c->d_vl_m,
       c->d_vl_mask, c->d_vl_cnt, ft ? c->d_f : nullptr, c->d_diag, c->d_acc_cell, stage_out,
       c->d_orig, NEXT ? list_track(c) : ListTrack{nullptr, nullptr, nullptr}, c->n, c->cap,
@@ -237,14 +238,20 @@ void launch_force(avf_ctx *c, int initial, bool ft = true, bool verlet = false,
   if (!stage_in) stage_in = c->d_stage;
   const bool next = verlet && stage_out;
   if (c->opt.block_clumps >= 128) {
-    if (next)
+    if (next && c->lean)
+      launch_force_bs<128, true, true, true>(c, initial, ft, time_ahead, stage_in, stage_out,
+                                             si);
+    else if (next)
       launch_force_bs<128, true, true>(c, initial, ft, time_ahead, stage_in, stage_out, si);
     else if (verlet)
       launch_force_bs<128, true, false>(c, initial, ft, time_ahead, stage_in, nullptr, si);
     else
       launch_force_bs<128, false, false>(c, initial, ft, time_ahead, stage_in, nullptr, si);
   } else {
-    if (next)
+    if (next && c->lean)
+      launch_force_bs<kForceBlock, true, true, true>(c, initial, ft, time_ahead, stage_in,
+                                                     stage_out, si);
+    else if (next)
       launch_force_bs<kForceBlock, true, true>(c, initial, ft, time_ahead, stage_in, stage_out,
                                                si);
     else if (verlet)
@@ -615,6 +622,10 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
   }
   c->n = total;
   c->cap = total;
+  // lean chunks once the step's working set (state, constants, both stage buffers, lists,
+  // tracking: about 870 B per clump) outgrows the 256 MiB last-level cache of the MI355X
+  c->lean = c->opt.lean_state > 0 ||
+            (c->opt.lean_state == 0 && (double)total * 870.0 > 256.0 * 1024.0 * 1024.0);
   c->cell_cap = std::max<int64_t>(8 * 4 * total, 1 << 16);
   if (c->cell_cap > (int64_t)0xfffffff0) c->cell_cap = 0xfffffff0;
 

@@ -76,9 +76,98 @@ struct ListTrack {
   double *xb, *vrot, *dsp;
 };
 
+// init_Verlet.cu:46-114 on one clump in registers: position and velocity by the old
+// accelerations, Rodrigues rotation of R by dOmega with column renormalisation, angular
+// velocity.  Returns |dOmega|.
+__device__ __forceinline__ double verlet_advance(d3 &rr, d3 &vv, d3 &ww, double *R, d3 aa, d3 ang,
+                                                 double dt, double hdt) {
+  // init_Verlet.cu:46-53
+  rr.x += dt * (vv.x + hdt * aa.x);
+  rr.y += dt * (vv.y + hdt * aa.y);
+  rr.z += dt * (vv.z + hdt * aa.z);
+  vv.x += dt * aa.x;
+  vv.y += dt * aa.y;
+  vv.z += dt * aa.z;
+  // init_Verlet.cu:57-89: Rodrigues rotation by dOmega
+  d3 dO = make_d3(dt * (ww.x + hdt * ang.x), dt * (ww.y + hdt * ang.y),
+                  dt * (ww.z + hdt * ang.z));
+  double mag = sqrt(dO.x * dO.x + dO.y * dO.y + dO.z * dO.z);
+  double dR[9];
+  if (mag > 0.0) {
+    d3 nn = make_d3(dO.x / mag, dO.y / mag, dO.z / mag);
+    double cdO = cos(mag), sdO = sin(mag), om = 1.0 - cdO;
+    dR[0] = om * nn.x * nn.x + cdO;
+    dR[1] = om * nn.x * nn.y - sdO * nn.z;
+    dR[2] = om * nn.x * nn.z + sdO * nn.y;
+    dR[3] = om * nn.y * nn.x + sdO * nn.z;
+    dR[4] = om * nn.y * nn.y + cdO;
+    dR[5] = om * nn.y * nn.z - sdO * nn.x;
+    dR[6] = om * nn.z * nn.x - sdO * nn.y;
+    dR[7] = om * nn.z * nn.y + sdO * nn.x;
+    dR[8] = om * nn.z * nn.z + cdO;
+  } else {
+    dR[0] = 1.0; dR[1] = 0.0; dR[2] = 0.0;
+    dR[3] = 0.0; dR[4] = 1.0; dR[5] = 0.0;
+    dR[6] = 0.0; dR[7] = 0.0; dR[8] = 1.0;
+  }
+  // init_Verlet.cu:92-100: T = dR R (rows)
+  double T[9];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      T[3 * i + j] = dR[3 * i] * R[j] + dR[3 * i + 1] * R[3 + j] + dR[3 * i + 2] * R[6 + j];
+  // init_Verlet.cu:103-114: normalise each column
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    double norm = sqrt(T[j] * T[j] + T[3 + j] * T[3 + j] + T[6 + j] * T[6 + j]);
+    R[j] = T[j] / norm;
+    R[3 + j] = T[3 + j] / norm;
+    R[6 + j] = T[6 + j] / norm;
+  }
+  ww.x += dt * ang.x;
+  ww.y += dt * ang.y;
+  ww.z += dt * ang.z;
+  return mag;
+}
+
+// The 24-double stage record of a clump (what neighbours gather): centre, radius, velocity,
+// density, angular velocity, reference index, 4 sphere centres.
+__device__ __forceinline__ void fill_stage(double *__restrict__ g, d3 rr, double rad, d3 vv,
+                                           double rho, d3 ww, uint32_t id, const d3 *sph) {
+  g[G_RR + 0] = rr.x;
+  g[G_RR + 1] = rr.y;
+  g[G_RR + 2] = rr.z;
+  g[G_RAD] = rad;
+  g[G_VV + 0] = vv.x;
+  g[G_VV + 1] = vv.y;
+  g[G_VV + 2] = vv.z;
+  g[G_RHO] = rho;
+  g[G_WW + 0] = ww.x;
+  g[G_WW + 1] = ww.y;
+  g[G_WW + 2] = ww.z;
+  g[G_ID] = __longlong_as_double((long long)id);
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    g[G_SPH + 3 * s + 0] = sph[s].x;
+    g[G_SPH + 3 * s + 1] = sph[s].y;
+    g[G_SPH + 3 * s + 2] = sph[s].z;
+  }
+}
+
+// Candidate-list displacement bound of clump c since the list build:
+// |x' - x_b| + sqrt(3) pos * sum |dOmega| over its spheres.
+__device__ __forceinline__ void track_update(ListTrack lt, int64_t c, int64_t cap, d3 rr,
+                                             double pos, double rot_step) {
+  const double rot = lt.vrot[c] + rot_step;
+  lt.vrot[c] = rot;
+  const double dx = rr.x - lt.xb[c], dy = rr.y - lt.xb[cap + c], dz = rr.z - lt.xb[2 * cap + c];
+  const double d = sqrt(dx * dx + dy * dy + dz * dz) + 1.7320508075688772 * pos * rot;
+  if (d > lt.dsp[c]) lt.dsp[c] = d;
+}
+
 // mode 1: predict (init_Verlet) then stage and sphere-cell keys; mode 2: the same predict
-// and stage without keys (candidate-list step), updating the displacement bound
-// |x' - x_b| + sqrt(3) pos * sum |dOmega| of every sphere since the list build; mode 0:
+// and stage without keys (candidate-list step), updating the displacement bound; mode 0:
 // stage the current state only (initial force pass, avalanche_hst.cu:365-384, whose force
 // stage sees the unchanged time).  t_old: the time at the start of the step (P.time, or
 // within a chunk of list steps chunk_time(), which has the same sign: only comparisons of
@@ -91,7 +180,6 @@ __device__ __forceinline__ void predict_clump(double *__restrict__ st, double *_
                                               uint32_t *__restrict__ srank, int64_t c, int64_t cap,
                                               int mode, ListTrack lt, double t_old) {
   const double t_new = mode ? t_old + p.dt : t_old;
-  const double dt = p.dt, hdt = p.hdt;
 
   d3 rr = make_d3(st[S_RX * cap + c], st[S_RY * cap + c], st[S_RZ * cap + c]);
   double R[9];
@@ -108,56 +196,9 @@ __device__ __forceinline__ void predict_clump(double *__restrict__ st, double *_
       rad = 0.0;
       sta[C_RAD * cap + c] = 0.0;
     } else {
-      d3 aa = make_d3(st[S_AX * cap + c], st[S_AY * cap + c], st[S_AZ * cap + c]);
-      d3 ang = make_d3(st[S_GX * cap + c], st[S_GY * cap + c], st[S_GZ * cap + c]);
-      // init_Verlet.cu:46-53
-      rr.x += dt * (vv.x + hdt * aa.x);
-      rr.y += dt * (vv.y + hdt * aa.y);
-      rr.z += dt * (vv.z + hdt * aa.z);
-      vv.x += dt * aa.x;
-      vv.y += dt * aa.y;
-      vv.z += dt * aa.z;
-      // init_Verlet.cu:57-89: Rodrigues rotation by dOmega
-      d3 dO = make_d3(dt * (ww.x + hdt * ang.x), dt * (ww.y + hdt * ang.y),
-                      dt * (ww.z + hdt * ang.z));
-      double mag = sqrt(dO.x * dO.x + dO.y * dO.y + dO.z * dO.z);
-      rot_step = mag;
-      double dR[9];
-      if (mag > 0.0) {
-        d3 nn = make_d3(dO.x / mag, dO.y / mag, dO.z / mag);
-        double cdO = cos(mag), sdO = sin(mag), om = 1.0 - cdO;
-        dR[0] = om * nn.x * nn.x + cdO;
-        dR[1] = om * nn.x * nn.y - sdO * nn.z;
-        dR[2] = om * nn.x * nn.z + sdO * nn.y;
-        dR[3] = om * nn.y * nn.x + sdO * nn.z;
-        dR[4] = om * nn.y * nn.y + cdO;
-        dR[5] = om * nn.y * nn.z - sdO * nn.x;
-        dR[6] = om * nn.z * nn.x - sdO * nn.y;
-        dR[7] = om * nn.z * nn.y + sdO * nn.x;
-        dR[8] = om * nn.z * nn.z + cdO;
-      } else {
-        dR[0] = 1.0; dR[1] = 0.0; dR[2] = 0.0;
-        dR[3] = 0.0; dR[4] = 1.0; dR[5] = 0.0;
-        dR[6] = 0.0; dR[7] = 0.0; dR[8] = 1.0;
-      }
-      // init_Verlet.cu:92-100: T = dR R (rows)
-      double T[9];
-#pragma unroll
-      for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-          T[3 * i + j] = dR[3 * i] * R[j] + dR[3 * i + 1] * R[3 + j] + dR[3 * i + 2] * R[6 + j];
-      // init_Verlet.cu:103-114: normalise each column
-#pragma unroll
-      for (int j = 0; j < 3; j++) {
-        double norm = sqrt(T[j] * T[j] + T[3 + j] * T[3 + j] + T[6 + j] * T[6 + j]);
-        R[j] = T[j] / norm;
-        R[3 + j] = T[3 + j] / norm;
-        R[6 + j] = T[6 + j] / norm;
-      }
-      ww.x += dt * ang.x;
-      ww.y += dt * ang.y;
-      ww.z += dt * ang.z;
+      const d3 aa = make_d3(st[S_AX * cap + c], st[S_AY * cap + c], st[S_AZ * cap + c]);
+      const d3 ang = make_d3(st[S_GX * cap + c], st[S_GY * cap + c], st[S_GZ * cap + c]);
+      rot_step = verlet_advance(rr, vv, ww, R, aa, ang, p.dt, p.hdt);
       st[S_RX * cap + c] = rr.x;
       st[S_RY * cap + c] = rr.y;
       st[S_RZ * cap + c] = rr.z;
@@ -176,32 +217,10 @@ __device__ __forceinline__ void predict_clump(double *__restrict__ st, double *_
   // centres are the stale ones of the step it crossed side.x (rr, R unchanged since).
   d3 sph[4];
   sphere_centres(rr, R, rad0 / p.sphere_div, sph);
-
-  g[G_RR + 0] = rr.x;
-  g[G_RR + 1] = rr.y;
-  g[G_RR + 2] = rr.z;
-  g[G_RAD] = rad;
-  g[G_VV + 0] = vv.x;
-  g[G_VV + 1] = vv.y;
-  g[G_VV + 2] = vv.z;
-  g[G_RHO] = sta[C_RHO * cap + c];
-  g[G_WW + 0] = ww.x;
-  g[G_WW + 1] = ww.y;
-  g[G_WW + 2] = ww.z;
-  g[G_ID] = __longlong_as_double((long long)(uint32_t)orig[c]);
-#pragma unroll
-  for (int s = 0; s < 4; s++) {
-    g[G_SPH + 3 * s + 0] = sph[s].x;
-    g[G_SPH + 3 * s + 1] = sph[s].y;
-    g[G_SPH + 3 * s + 2] = sph[s].z;
-  }
+  fill_stage(g, rr, rad, vv, sta[C_RHO * cap + c], ww, (uint32_t)orig[c], sph);
 
   if (mode == 2) {
-    const double rot = lt.vrot[c] + rot_step;
-    lt.vrot[c] = rot;
-    const double dx = rr.x - lt.xb[c], dy = rr.y - lt.xb[cap + c], dz = rr.z - lt.xb[2 * cap + c];
-    const double d = sqrt(dx * dx + dy * dy + dz * dz) + 1.7320508075688772 * (rad0 / p.sphere_div) * rot;
-    if (d > lt.dsp[c]) lt.dsp[c] = d;
+    track_update(lt, c, cap, rr, rad0 / p.sphere_div, rot_step);
     return;
   }
 
@@ -217,6 +236,37 @@ __device__ __forceinline__ void predict_clump(double *__restrict__ st, double *_
     skey[4 * c + s] = key;
     srank[4 * c + s] = rank;
   }
+}
+
+// A clump's finished state kept in registers from the force stage to the next step's
+// predict inside a chunk of candidate-list steps.
+struct ClumpRegs {
+  d3 rr, vv, ww, aa, ang;
+  double R[9];
+  bool frz;  // frozen at the force stage's time: no finish, no accelerations
+};
+
+// predict_clump mode 2 on a finished state in registers.  Only R goes to st: inside a
+// chunk the list steps read a clump's own centre and velocities from its stage record,
+// and the chunk's last step writes them back to st.
+__device__ __forceinline__ void predict_regs(ClumpRegs &s, double *__restrict__ st,
+                                             double *__restrict__ sta, double *__restrict__ g,
+                                             const InstParams &p, uint32_t id, double rad,
+                                             double rad0, double rho, int64_t c, int64_t cap,
+                                             ListTrack lt, double t_old) {
+  double rot_step = 0.0;
+  if (frozen(t_old, s.rr.x, p.side_x)) {
+    rad = 0.0;
+    sta[C_RAD * cap + c] = 0.0;
+  } else {
+    rot_step = verlet_advance(s.rr, s.vv, s.ww, s.R, s.aa, s.ang, p.dt, p.hdt);
+#pragma unroll
+    for (int k = 0; k < 9; k++) st[(S_R00 + k) * cap + c] = s.R[k];
+  }
+  d3 sph[4];
+  sphere_centres(s.rr, s.R, rad0 / p.sphere_div, sph);
+  fill_stage(g, s.rr, rad, s.vv, rho, s.ww, id, sph);
+  track_update(lt, c, cap, s.rr, rad0 / p.sphere_div, rot_step);
 }
 
 // The stage records of a workgroup's 256 clumps are assembled in LDS (rows padded to 25
@@ -878,18 +928,31 @@ __device__ __forceinline__ double chunk_time(const InstParams &q, int si) {
 // touching clumps come from k_detect's per-sphere lists or, with VERLET, from the candidate
 // list of k_vmerge, filtered every step by the same touching test and visit rule.
 // time: the force stage's time (sys.time after the step's advance).
-template <int MAXN, int BS, bool VERLET>
+// State I/O of a clump (bit-identical results in every variant):
+// * plain (not VERLET): own centre and velocities from st; the finish writes v, w, a, alpha.
+// * VERLET, not NEXT (a rebuild step or the last step of a chunk): own centre and
+//   velocities from the stage record (own), which always holds them; the finish writes
+//   rr, v, w, a, alpha to st (in a LEAN chunk st holds rr, v, w only at the chunk start).
+// * NEXT, not LEAN: as plain; the next step's predict then reads st and writes all of it.
+// * NEXT and LEAN: own from the stage record; only a, alpha go to st and the finished state
+//   is left in out for the next step's predict (predict_regs), which writes only R to st:
+//   144 B less HBM traffic per clump-step, for working sets beyond the last-level cache.
+template <int MAXN, int BS, bool VERLET, bool NEXT, bool LEAN>
 __device__ __forceinline__ void force_clump(
     const InstParams &p, PPShared<MAXN, BS> &sh, int lane, int64_t c,
     double *__restrict__ st, const double *__restrict__ sta, const double *__restrict__ stage,
     const uint32_t *__restrict__ clist, const int32_t *__restrict__ ccnt,
     const int32_t *__restrict__ vl_m, const uint16_t *__restrict__ vl_mask,
     const int32_t *__restrict__ vl_cnt, double *__restrict__ fout, DevDiag *__restrict__ diag,
-    int64_t cap, int initial, double time) {
-
-  const d3 rr = make_d3(st[S_RX * cap + c], st[S_RY * cap + c], st[S_RZ * cap + c]);
-  d3 vv = make_d3(st[S_VX * cap + c], st[S_VY * cap + c], st[S_VZ * cap + c]);
-  d3 ww = make_d3(st[S_WX * cap + c], st[S_WY * cap + c], st[S_WZ * cap + c]);
+    int64_t cap, int initial, double time, const ClumpRegs &own, ClumpRegs &out) {
+  static_assert(!NEXT || VERLET, "the fused predict runs on candidate-list steps only");
+  constexpr bool kOwnStage = VERLET && (LEAN || !NEXT);
+  constexpr bool kRegsOut = NEXT && LEAN;
+  constexpr bool kWriteRR = VERLET && !NEXT;
+  const d3 rr = kOwnStage ? own.rr
+                          : make_d3(st[S_RX * cap + c], st[S_RY * cap + c], st[S_RZ * cap + c]);
+  d3 vv = kOwnStage ? own.vv : make_d3(st[S_VX * cap + c], st[S_VY * cap + c], st[S_VZ * cap + c]);
+  d3 ww = kOwnStage ? own.ww : make_d3(st[S_WX * cap + c], st[S_WY * cap + c], st[S_WZ * cap + c]);
   double R[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) R[k] = st[(S_R00 + k) * cap + c];
@@ -1125,10 +1188,28 @@ __device__ __forceinline__ void force_clump(
   }
 
   if (frz) {
-    if (!initial) {  // finish_Verlet.cu:28-32
-      st[S_VX * cap + c] = 0.0;
-      st[S_VY * cap + c] = 0.0;
-      st[S_VZ * cap + c] = 0.0;
+    if (!initial) vv = make_d3(0.0, 0.0, 0.0);  // finish_Verlet.cu:28-32
+    if constexpr (kRegsOut) {
+      out.rr = rr;
+      out.vv = vv;
+      out.ww = ww;
+#pragma unroll
+      for (int k = 0; k < 9; k++) out.R[k] = R[k];
+      out.frz = true;
+    } else {
+      if constexpr (kWriteRR) {
+        st[S_RX * cap + c] = rr.x;
+        st[S_RY * cap + c] = rr.y;
+        st[S_RZ * cap + c] = rr.z;
+        st[S_WX * cap + c] = ww.x;
+        st[S_WY * cap + c] = ww.y;
+        st[S_WZ * cap + c] = ww.z;
+      }
+      if (!initial) {
+        st[S_VX * cap + c] = 0.0;
+        st[S_VY * cap + c] = 0.0;
+        st[S_VZ * cap + c] = 0.0;
+      }
     }
     return;
   }
@@ -1165,12 +1246,29 @@ __device__ __forceinline__ void force_clump(
     ww.x += hdt * (gf.x - gi.x);
     ww.y += hdt * (gf.y - gi.y);
     ww.z += hdt * (gf.z - gi.z);
-    st[S_VX * cap + c] = vv.x;
-    st[S_VY * cap + c] = vv.y;
-    st[S_VZ * cap + c] = vv.z;
-    st[S_WX * cap + c] = ww.x;
-    st[S_WY * cap + c] = ww.y;
-    st[S_WZ * cap + c] = ww.z;
+    if constexpr (!kRegsOut) {
+      if constexpr (kWriteRR) {
+        st[S_RX * cap + c] = rr.x;
+        st[S_RY * cap + c] = rr.y;
+        st[S_RZ * cap + c] = rr.z;
+      }
+      st[S_VX * cap + c] = vv.x;
+      st[S_VY * cap + c] = vv.y;
+      st[S_VZ * cap + c] = vv.z;
+      st[S_WX * cap + c] = ww.x;
+      st[S_WY * cap + c] = ww.y;
+      st[S_WZ * cap + c] = ww.z;
+    }
+  }
+  if constexpr (kRegsOut) {
+    out.rr = rr;
+    out.vv = vv;
+    out.ww = ww;
+    out.aa = af;
+    out.ang = gf;
+#pragma unroll
+    for (int k = 0; k < 9; k++) out.R[k] = R[k];
+    out.frz = false;
   }
   // the accelerations become a_i of the next step (avalanche_hst.cu:844-850)
   st[S_AX * cap + c] = af.x;
@@ -1195,8 +1293,8 @@ __device__ __forceinline__ void force_clump(
 // step (k_finalize applies the chunk's per-step advances once, at its end): the step's
 // time is chunk_time(P, si), its counter P.cc + si.
 // time_ahead: the force stage runs at the step's time + dt (candidate-list steps advance
-// the time at the end of the step instead of in k_scatter).
-template <int MAXN, int BS, bool VERLET, bool NEXT>
+// the time at the end of the step instead of in k_scatter).  LEAN: see force_clump.
+template <int MAXN, int BS, bool VERLET, bool NEXT, bool LEAN>
 __global__ __launch_bounds__(BS) void k_force(
     double *__restrict__ st, double *__restrict__ sta, const double *__restrict__ stage,
     const int32_t *__restrict__ inst_of, const InstParams *__restrict__ P,
@@ -1214,15 +1312,25 @@ __global__ __launch_bounds__(BS) void k_force(
   const int64_t c0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * BS;
   const int64_t c = c0 + lane;
   const int inst = c < n ? inst_of[c] : -1;
+  ClumpRegs cr, own;
+  constexpr bool kOwnStage = VERLET && (LEAN || !NEXT);
+  constexpr bool kRegsOut = NEXT && LEAN;
+  if (kOwnStage && c < n) {  // the own record's first 12 doubles in 16-byte loads
+    const double2 *o = reinterpret_cast<const double2 *>(stage + c * kStageDoubles);
+    const double2 a = o[0], b = o[1], v0 = o[2], v1 = o[3], w0 = o[4], w1 = o[5];
+    own.rr = make_d3(a.x, a.y, b.x);
+    own.vv = make_d3(v0.x, v0.y, v1.x);
+    own.ww = make_d3(w0.x, w0.y, w1.x);
+  }
   if (c < n) {
     while (true) {
       const int cur = __builtin_amdgcn_readfirstlane(inst);
       if (inst == cur) {
         const InstParams &q = P[cur];
         const double t = chunk_time(q, si);
-        force_clump<MAXN, BS, VERLET>(q, sh, lane, c, st, sta, stage, clist, ccnt, vl_m,
-                                      vl_mask, vl_cnt, fout, diag, cap, initial,
-                                      time_ahead ? t + q.dt : t);
+        force_clump<MAXN, BS, VERLET, NEXT, LEAN>(q, sh, lane, c, st, sta, stage, clist, ccnt,
+                                                  vl_m, vl_mask, vl_cnt, fout, diag, cap, initial,
+                                            time_ahead ? t + q.dt : t, own, cr);
         break;
       }
     }
@@ -1236,14 +1344,16 @@ __global__ __launch_bounds__(BS) void k_force(
     const InstParams &q = P[inst];
     due = q.regrid_interval > 0 && (q.cc + si) % q.regrid_interval == 0;
     if (due) {
-      v[0] = order_key(st[S_RX * cap + c]);
-      v[1] = order_key(st[S_RY * cap + c]);
-      v[2] = order_key(st[S_RZ * cap + c]);
+      const d3 r = kRegsOut ? cr.rr
+                            : make_d3(st[S_RX * cap + c], st[S_RY * cap + c], st[S_RZ * cap + c]);
+      v[0] = order_key(r.x);
+      v[1] = order_key(r.y);
+      v[2] = order_key(r.z);
       v[3] = v[0];
       v[4] = v[1];
       v[5] = v[2];
-      v[6] = order_key(st[S_VX * cap + c]);
-      v[7] = order_key(st[S_VZ * cap + c]);
+      v[6] = order_key(kRegsOut ? cr.vv.x : st[S_VX * cap + c]);
+      v[7] = order_key(kRegsOut ? cr.vv.z : st[S_VZ * cap + c]);
     }
   }
   if (__any(due)) wave_minmax8(acc, inst, v, due, 0x78u /* max: 3,4,5,6 */);
@@ -1255,8 +1365,13 @@ __global__ __launch_bounds__(BS) void k_force(
         const int cur = __builtin_amdgcn_readfirstlane(inst);
         if (inst == cur) {
           const InstParams &q = P[cur];  // the next step starts at this step's time + dt
-          predict_clump(st, sta, rec + lane * kStagePad, q, orig, nullptr, nullptr, nullptr, c,
-                        cap, 2, lt, chunk_time(q, si) + q.dt);
+          if constexpr (LEAN)
+            predict_regs(cr, st, sta, rec + lane * kStagePad, q, (uint32_t)orig[c],
+                         sta[C_RAD * cap + c], sta[C_RAD0 * cap + c], sta[C_RHO * cap + c], c,
+                         cap, lt, chunk_time(q, si) + q.dt);
+          else
+            predict_clump(st, sta, rec + lane * kStagePad, q, orig, nullptr, nullptr, nullptr, c,
+                          cap, 2, lt, chunk_time(q, si) + q.dt);
           break;
         }
       }

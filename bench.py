@@ -3,7 +3,7 @@
     python bench.py [--steps K] [--warmup W] [--workload headline|small|large]
 
 Times K steps of the headline configuration (250,000 tetrahedral 4-sphere clumps, FP64,
-sphere-cell list rebuilt every step, spatial reorder every 50 steps) with inputs resident
+candidate lists with a 0.01 cm skin, spatial reorder every 100 steps) with inputs resident
 in HBM, and prints one JSON line: clump-steps/s, the roofline of the dominant kernel
 (k_force, from HIP events over the timed region) and the CPU baseline (the oracle on one
 host core, bounded sample of the same workload).
@@ -21,14 +21,20 @@ sys.path.insert(0, ROOT)
 
 METRIC = "clump-steps/s + HBM GB/s at 250k clumps; batched vs plain; adjoint/forward cost"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per clump and launch of k_force: carried state read (rr, R, v, w, a,
-# alpha = 24 doubles, 192 B), v, w, a, alpha written (96 B), radius (8 B) and the sorted
-# sphere index of its 4 spheres (16 B).
-FORCE_BYTES_PER_CLUMP = 312.0
+# Algorithmic bytes per clump and launch of k_force on a candidate-list step (which also
+# predicts the next step): reads the carried state (rr, v, w, R, a, alpha: 24 doubles,
+# 192 B), the clump constants (radius, pre-freeze radius, density, mass, I: 56 B), instance
+# and reference index (8 B), the list length (4 B) and the displacement tracking (build
+# centre, rotation sum, bound: 40 B); writes the carried state (192 B), the next stage
+# record (192 B) and the tracking (16 B).  The list entries (6 B each) and the neighbours'
+# stage records (L2 hits) are not counted.
+FORCE_BYTES_PER_CLUMP = 700.0
 # Algorithmic bytes per clump-step of the whole step (480 B per clump-step).
 STEP_BYTES_PER_CLUMP = 480.0
 # Candidate-list skin (cm) of the headline runs: 5 % of the sphere radius.
 DEFAULT_SKIN = 0.01
+# Spatial reorder interval (steps) of the headline runs; a reorder also rebuilds the lists.
+DEFAULT_REORDER = 100
 
 
 def cpu_baseline(inst, budget_s: float) -> dict:
@@ -50,7 +56,7 @@ def cpu_baseline(inst, budget_s: float) -> dict:
                       f"({dt:.1f} s on one host core)"}
 
 
-def batch_compare(steps: int, warmup: int = 20, skin: float = 0.0) -> dict:
+def batch_compare(steps: int, warmup: int = 20, skin: float = DEFAULT_SKIN) -> dict:
     """64 independent flumes (N ~ log-uniform[5e3, 6e4], per-instance seeds, mu, eps and
     slope) stepped together in one context vs the same instances as sequential plain runs
     (one context each, synchronised between instances)."""
@@ -59,6 +65,7 @@ def batch_compare(steps: int, warmup: int = 20, skin: float = 0.0) -> dict:
     total = sum(len(i.particles) for i in insts)
     opt = dem.default_options()
     opt.verlet_skin = skin
+    opt.reorder_interval = DEFAULT_REORDER
     ctx = dem.Context(insts, opt)
     ctx.regrid()
     ctx.initial_forces()
@@ -81,7 +88,7 @@ def batch_compare(steps: int, warmup: int = 20, skin: float = 0.0) -> dict:
         c.sync()
         t_seq += time.perf_counter() - t0
         c.close()
-    return {"instances": len(insts), "clumps": total, "steps": steps,
+    return {"instances": len(insts), "clumps": total, "steps": steps, "skin_cm": skin,
             "batched_clump_steps_per_s": total * steps / t_batch,
             "sequential_clump_steps_per_s": total * steps / t_seq,
             "speedup": t_seq / t_batch}
@@ -168,6 +175,8 @@ def main() -> int:
     ap.add_argument("--skin", type=float, default=None,
                     help=f"candidate-list skin in cm (default {DEFAULT_SKIN}; 0: sphere cells "
                          "rebuilt every step)")
+    ap.add_argument("--reorder", type=int, default=None,
+                    help=f"spatial reorder interval in steps (default {DEFAULT_REORDER})")
     ap.add_argument("--compare-steps", type=int, default=2000,
                     help="steps of the per-step-rebuild comparison run (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -205,6 +214,7 @@ def main() -> int:
     n = len(inst.particles)
     opt = dem.default_options()
     opt.verlet_skin = DEFAULT_SKIN if args.skin is None else args.skin
+    opt.reorder_interval = DEFAULT_REORDER if args.reorder is None else args.reorder
     ctx = dem.Context([inst], opt)
     ctx.regrid()
     ctx.initial_forces()

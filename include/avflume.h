@@ -139,7 +139,11 @@ typedef struct avf_options {
   int32_t block_clumps;     /* lanes per block of the force kernel: 64 or 128          */
   int32_t max_neighbours;   /* informational: touching-clump list capacity (32)        */
   int32_t device;           /* HIP device ordinal                                      */
-  int32_t reserved;
+  int32_t lean_state;       /* candidate-list chunks: 1 = a clump's centre and velocities */
+                            /* live only in its stage record inside a chunk (less HBM   */
+                            /* traffic), -1 = also in the state arrays every step (fewer */
+                            /* strided reads, for working sets the last-level cache   */
+                            /* holds), 0 = chosen by working-set size; same results     */
   void *stream;             /* hipStream_t to run on; NULL = library-owned stream      */
   double verlet_skin;       /* > 0: candidate lists of sphere pairs within sep + skin, */
                             /* rebuilt only when some sphere has moved skin / 2 since  */

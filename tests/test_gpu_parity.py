@@ -18,12 +18,13 @@ T_RTOL = 1e-9    # trajectories over 100 steps
 STATE = ["rr", "vv", "ww", "Rmatx", "Rmaty", "Rmatz", "rr_a", "rr_b", "rr_c", "rr_d", "radius"]
 
 
-def opts(graph=1, reorder=50, factor=1.0, skin=0.0):
+def opts(graph=1, reorder=50, factor=1.0, skin=0.0, lean=0):
     o = dem.default_options()
     o.use_graph = graph
     o.reorder_interval = reorder
     o.sphere_cell_factor = factor
     o.verlet_skin = skin
+    o.lean_state = lean
     return o
 
 
@@ -137,16 +138,19 @@ def test_trajectory_avalanche_100_steps(gpu):
     print("bit-exact 100-step avalanche trajectory:", exact)
 
 
-def test_trajectory_candidate_lists_100_steps(gpu):
-    """The candidate-list path (verlet_skin > 0) against the oracle."""
-    exact = run_pair(scenarios.avalanche_state(16, 23), 100, 50, skin=0.02)
-    print("bit-exact 100-step avalanche trajectory on candidate lists:", exact)
+@pytest.mark.parametrize("lean", [-1, 1])
+def test_trajectory_candidate_lists_100_steps(gpu, lean):
+    """The candidate-list path (verlet_skin > 0) against the oracle, with the clump state
+    carried in the state arrays every step (-1) or only in the stage records (1)."""
+    exact = run_pair(scenarios.avalanche_state(16, 23), 100, 50, skin=0.02, lean=lean)
+    print(f"bit-exact 100-step avalanche trajectory on candidate lists (lean {lean}):", exact)
 
 
-@pytest.mark.parametrize("skin", [0.03, 0.002])
-def test_candidate_lists_equal_per_step_binning(gpu, skin):
+@pytest.mark.parametrize("skin,lean", [(0.03, -1), (0.002, -1), (0.03, 1), (0.002, 1)])
+def test_candidate_lists_equal_per_step_binning(gpu, skin, lean):
     """verlet_skin > 0 changes no bit of the state or of the last step's forces, with list
-    rebuilds, chunk undo (the small skin forces them) and spatial reorders in between."""
+    rebuilds, chunk undo (the small skin forces them) and spatial reorders in between, in
+    both state layouts of the list chunks."""
     from avalancheflume3d_amd import workloads
     cases = [scenarios.hopper_state(500, 61), scenarios.avalanche_state(16, 62)]
     flow = workloads.build(workloads.FlumeSpec(n=300, width=2.0, length=4.0, box=(0.2, 2.0, 1.0),
@@ -155,7 +159,7 @@ def test_candidate_lists_equal_per_step_binning(gpu, skin):
     cases.append(flow)
     for inst in cases:
         a = gpu_ctx(inst)
-        b = gpu_ctx(inst, skin=skin, reorder=40)
+        b = gpu_ctx(inst, skin=skin, reorder=40, lean=lean)
         a.initial_forces()
         b.initial_forces()
         for chunk in (1, 37, 120, 300):
@@ -175,7 +179,8 @@ def test_candidate_lists_equal_per_step_binning(gpu, skin):
         b.close()
 
 
-def test_candidate_list_chunks_across_regrids(gpu):
+@pytest.mark.parametrize("lean", [-1, 1])
+def test_candidate_list_chunks_across_regrids(gpu, lean):
     """Chunks of list steps apply their per-step parameter advances at their end and end on
     the steps whose clump-cell regrid is due; with a regrid every 7 steps the candidate-list
     path still equals per-step binning bit for bit."""
@@ -189,7 +194,7 @@ def test_candidate_list_chunks_across_regrids(gpu):
         inst.geo = abi.Geometry.from_buffer_copy(inst.geo)
         inst.geo.regrid_interval = 7
         a = gpu_ctx(inst)
-        b = gpu_ctx(inst, skin=0.02, reorder=40)
+        b = gpu_ctx(inst, skin=0.02, reorder=40, lean=lean)
         a.initial_forces()
         b.initial_forces()
         for chunk in (3, 40, 101):
