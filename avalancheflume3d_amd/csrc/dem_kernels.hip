@@ -1478,10 +1478,20 @@ __global__ void k_finalize(InstParams *__restrict__ P, long long *__restrict__ a
                            int force, int step, int advance_time = 0, int nsteps = 1) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ninst) return;
+  InstParams &p = P[i];
+  // cc % regrid_interval kept as a running remainder (one 64-bit division per launch)
+  const int64_t ri = p.regrid_interval;
+  int64_t r = ri > 0 ? ((p.cc % ri) + ri) % ri : 1;
+  double t = p.time;
   for (int s = 0; s < nsteps; s++) {  // the steps of a chunk, in order
-    finalize_params(P[i], acc + 8 * i, force, step);
-    if (advance_time) P[i].time = P[i].time + P[i].dt;
+    if (force || (ri > 0 && r == 0)) finalize_params(p, acc + 8 * i, 1, 0);
+    if (step) {
+      p.cc += 1;
+      if (ri > 0 && ++r == ri) r = 0;
+    }
+    if (advance_time) t = t + p.dt;
   }
+  p.time = t;
 }
 
 // --------------------------------------------------------------- sphere-cell grid ---

@@ -102,8 +102,9 @@ def adjoint_bench(steps: int, checkpoint: int = 100, count: int = 16, n: int = 2
     L2 misfit of a 64-bin deposit profile against the profile the forward window gives at
     mu = 0.45, eps = 0.55; gradients wrt (mu, eps, kappa) for both contact laws and the
     per-clump start velocities (dv ~ N(0, 1 cm/s)).  Cost = reverse pass (segment
-    recompute + adjoint steps) / forward window, both from HIP events inside the library;
-    plus central differences of the forward window for three parameters."""
+    recompute + adjoint steps) / forward window, both from HIP events inside the library.
+    Checks: the dual-number tangent of the same window along each parameter (same scale),
+    and central differences of the forward window on a 2,000-clump, 200-step window."""
     import numpy as np
 
     from avalancheflume3d_amd import abi, dem, workloads
@@ -125,28 +126,18 @@ def adjoint_bench(steps: int, checkpoint: int = 100, count: int = 16, n: int = 2
     dv = np.random.default_rng(5).normal(scale=1.0, size=(count * n, 3))
     out = ctx.window_run(steps, theta=base, dv=dv, target=target, **kw)
     d = out["diag"]
-    # central differences of the forward window (same frozen list): step sizes from the
-    # 1e-6 relative of the specification up to 1e-3 (small gradients relative to the loss
-    # hit the double-precision noise floor at the smallest steps)
-    fd = {}
-    for k in (1, 3, 4, 5):
-        g = float(out["grad_theta"][0, k])
-        row = {"adjoint": g}
-        best = None
-        for hr in (1e-6, 1e-5, 1e-4, 1e-3):
-            tp, tm = base.copy(), base.copy()
-            h = hr * base[0, k]
-            tp[:, k] += h
-            tm[:, k] -= h
-            lp = ctx.window_run(steps, theta=tp, dv=dv, target=target, grad=False, **kw)["loss"][0]
-            lm = ctx.window_run(steps, theta=tm, dv=dv, target=target, grad=False, **kw)["loss"][0]
-            f = (lp - lm) / (2 * h)
-            rel = abs(f - g) / max(abs(g), 1e-300)
-            row[f"fd_h{hr:.0e}"] = f
-            row[f"rel_h{hr:.0e}"] = rel
-            best = rel if best is None else min(best, rel)
-        row["best_rel"] = best
-        fd[abi.THETA[k]] = row
+    # dual-number tangent along each parameter (all instances at once)
+    tangent = {}
+    for k in (1, 2, 3, 4, 5):
+        e = np.zeros_like(base)
+        e[:, k] = base[:, k] * 1e-2
+        o2 = ctx.window_run(steps, theta=base, dv=dv, target=target, grad=False, tangent=True,
+                            tan_theta=e, **kw)
+        want = out["grad_theta"][:, k] * e[:, k]
+        got = o2["dloss"]
+        rel = np.abs(got - want) / np.maximum(np.maximum(np.abs(got), np.abs(want)), 1e-300)
+        tangent[abi.THETA[k]] = {"adjoint_inst0": float(want[0]), "dual_inst0": float(got[0]),
+                                 "max_rel": float(rel.max())}
     # the plain forward over the same number of steps
     t0 = time.perf_counter()
     ctx.step(steps)
@@ -162,7 +153,55 @@ def adjoint_bench(steps: int, checkpoint: int = 100, count: int = 16, n: int = 2
             "max_disp_cm": d["max_disp"], "max_list": d["max_list"],
             "loss_mean": float(np.mean(out["loss"])),
             "grad_theta_inst0": out["grad_theta"][0].tolist(),
-            "grad_dv_rms": float(np.sqrt(np.mean(out["grad_dv"] ** 2))), "fd": fd}
+            "grad_dv_rms": float(np.sqrt(np.mean(out["grad_dv"] ** 2))),
+            "tangent": tangent, "fd": adjoint_fd_check()}
+
+
+def adjoint_fd_check(n: int = 2000, pre: int = 8000, steps: int = 200) -> dict:
+    """Central differences of the GPU forward window against the adjoint on one slope
+    release of n clumps (a window short enough that no contact switches on or off within
+    the step sizes used), step sizes 1e-6 .. 1e-4 relative to each parameter."""
+    import numpy as np
+
+    from avalancheflume3d_amd import abi, dem, workloads
+    inst = workloads.slope_release(n=n, seed=11, width=4.0)
+    ctx = dem.Context([inst])
+    ctx.regrid()
+    ctx.initial_forces()
+    ctx.step(pre)
+    ctx.sync()
+    ps = abi.ProfileSpec()
+    ps.x0, ps.x1, ps.width, ps.sigma, ps.nbins = 0.0, 40.0, 4.0, 0.0, 32
+    s0 = inst.sys
+    base = np.array([[s0.mu_pp, s0.eps_pp, s0.kappa_pp, s0.mu_pw, s0.eps_pw, s0.kappa_pw]])
+    kw = dict(checkpoint=50, skin=0.05, w_runout=1.0, w_profile=10.0, profile=ps)
+    target = np.linspace(0.05, 0.01, ps.nbins)[None, :]
+    dv = np.random.default_rng(6).normal(scale=1.0, size=(n, 3))
+    out = ctx.window_run(steps, theta=base, dv=dv, target=target, **kw)
+    g_all = out["grad_theta"][0]
+    scale = max(abs(g_all[k] * base[0, k]) for k in range(6))
+    res = {"clumps": n, "window": steps}
+    for k in (1, 2, 3, 4, 5):
+        g = float(g_all[k])
+        row = {"adjoint": g}
+        best = None
+        for hr in (1e-6, 1e-5, 1e-4):
+            tp, tm = base.copy(), base.copy()
+            h = hr * base[0, k]
+            tp[:, k] += h
+            tm[:, k] -= h
+            lp = ctx.window_run(steps, theta=tp, dv=dv, target=target, grad=False, **kw)["loss"][0]
+            lm = ctx.window_run(steps, theta=tm, dv=dv, target=target, grad=False, **kw)["loss"][0]
+            f = (lp - lm) / (2 * h)
+            row[f"fd_h{hr:.0e}"] = f
+            err = abs(f - g)
+            best = err if best is None else min(best, err)
+        row["best_rel"] = best / max(abs(g), 1e-300)
+        # components far below the largest parameter sensitivity are compared on that scale
+        row["best_rel_to_largest"] = best * base[0, k] / scale
+        res[abi.THETA[k]] = row
+    ctx.close()
+    return res
 
 
 def main() -> int:
