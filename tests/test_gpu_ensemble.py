@@ -1,0 +1,81 @@
+"""Long runs: the seed ensemble of the GPU path against the host oracle.
+
+Trajectories are chaotic, so after many steps runs are compared through the statistics the
+reference's users read off a run: the runout (mass-weighted mean slope coordinate x' of the
+clumps, slope frame of rotate_vectors.cu:7-56), the front, and the deposit height profile
+over x'.  Six seeds of the same small flume run 20,000 steps (0.29 s of flow: the layer
+leaves the release box, runs down the slope and piles onto the deposit) on the GPU in one
+batched context on candidate lists, and on the oracle (one host thread per seed).  The
+ensemble means of the GPU runs must lie within the seed-ensemble spread of the oracle runs.
+"""
+import math
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+from avalancheflume3d_amd import dem, workloads
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+SEEDS = (1, 2, 3, 4, 5, 6)
+STEPS = 20_000
+BINS = np.linspace(0.0, 10.0, 21)   # x' bins (cm) over the slope end and the deposit
+
+
+def instance(seed: int) -> dem.Instance:
+    # contacts 100x softer than the reference stiffness: a 10x larger step, so the flow
+    # develops within a run the oracle finishes in seconds; 24 clumps per clump cell
+    spec = workloads.FlumeSpec(n=120, width=2.0, length=3.0, box=(2.0, 2.0, 2.01),
+                               deposit_length=6.0, spacing=0.4, jitter=0.02, seed=seed,
+                               k_n=workloads.K_N / 100)
+    inst = workloads.build(spec)
+    inst.ntags = 24
+    return inst
+
+
+def statistics(pp: np.ndarray, inst: dem.Instance) -> dict:
+    s, c = inst.sys.sint, inst.sys.cost
+    xs = pp["rr"][:, 0] * c - pp["rr"][:, 2] * s
+    m = pp["mass"]
+    vol = m / pp["density"]
+    hist, _ = np.histogram(xs, bins=BINS, weights=vol)
+    return {"runout": float((m * xs).sum() / m.sum()), "front": float(xs.max()),
+            "profile": hist / (np.diff(BINS) * inst.sys.side.y)}
+
+
+def oracle_run(inst: dem.Instance) -> dict:
+    o = oracle.OracleSim(inst.sys, inst.geo, inst.fun, inst.ntags, inst.phase, inst.particles)
+    o.initial()
+    o.step(STEPS)
+    return statistics(o.get()["pp"], inst)
+
+
+def test_long_run_statistics_within_seed_spread(gpu):
+    insts = [instance(s) for s in SEEDS]
+    with ThreadPoolExecutor(max_workers=len(insts)) as pool:   # ctypes releases the GIL
+        ref = list(pool.map(oracle_run, insts))
+    o = dem.default_options()
+    o.verlet_skin = 0.01
+    ctx = dem.Context(insts, o)
+    ctx.regrid()
+    ctx.initial_forces()
+    ctx.step(STEPS)
+    got = [statistics(ctx.download(k)[0], inst) for k, inst in enumerate(insts)]
+    ctx.close()
+    for key in ("runout", "front"):
+        a = np.array([g[key] for g in got])
+        b = np.array([r[key] for r in ref])
+        spread = b.std(ddof=1)
+        print(f"{key}: GPU {a.mean():.4f} +- {a.std(ddof=1):.4f}, oracle {b.mean():.4f} "
+              f"+- {spread:.4f}; largest per-seed difference {np.abs(a - b).max():.3e}")
+        assert spread > 0.0
+        assert abs(a.mean() - b.mean()) <= spread / math.sqrt(len(SEEDS))
+    pa = np.array([g["profile"] for g in got])
+    pb = np.array([r["profile"] for r in ref])
+    diff = np.linalg.norm(pa.mean(0) - pb.mean(0))
+    spread = math.sqrt((pb.std(0, ddof=1) ** 2).sum())
+    print(f"deposit profile: |mean difference| {diff:.3e}, oracle seed spread {spread:.3e}")
+    assert spread > 0.0
+    assert diff <= spread / math.sqrt(len(SEEDS))
