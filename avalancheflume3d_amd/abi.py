@@ -107,6 +107,11 @@ class Observables(C.Structure):
         [("live", C.c_int64)]
 
 
+class SphereGrid(C.Structure):
+    _fields_ = [("lo", C.c_double * 3), ("h", C.c_double), ("inv_h", C.c_double),
+                ("n", C.c_int32 * 3), ("reserved", C.c_int32)]
+
+
 class ProfileSpec(C.Structure):
     _fields_ = [("x0", C.c_double), ("x1", C.c_double), ("width", C.c_double),
                 ("sigma", C.c_double), ("nbins", C.c_int32), ("reserved", C.c_int32)]
@@ -210,6 +215,8 @@ def load_library() -> C.CDLL:
             "avf_get_clump_cells": (C.c_int, [vp, i32, vp]),
             "avf_check_cells": (C.c_int, [vp, i32, vp]),
             "avf_get_contacts": (C.c_int, [vp, i32, vp, i64, vp]),
+            "avf_get_cell_table": (C.c_int, [vp, i32, vp, vp, i64, vp]),
+            "avf_get_sphere_keys": (C.c_int, [vp, i32, vp, vp, vp, vp]),
             "avf_get_diag": (C.c_int, [vp, vp]),
             "avf_sync": (C.c_int, [vp]),
             "avf_step_profiled": (C.c_int, [vp, i64, vp, vp]),
@@ -233,7 +240,8 @@ EXPORTED = [
     "avf_place_lattice", "avf_sphere_centres", "avf_cell_setup", "avf_create", "avf_destroy",
     "avf_last_error", "avf_upload", "avf_download", "avf_set_phase", "avf_get_time",
     "avf_regrid", "avf_get_cell", "avf_initial_forces", "avf_step", "avf_get_forces",
-    "avf_get_clump_cells", "avf_check_cells", "avf_get_contacts", "avf_get_diag", "avf_sync",
+    "avf_get_clump_cells", "avf_check_cells", "avf_get_contacts", "avf_get_cell_table",
+    "avf_get_sphere_keys", "avf_get_diag", "avf_sync",
     "avf_step_profiled", "avf_step_timed", "avf_observe", "avf_window_run",
 ]
 KERNELS = ["predict", "scan_a", "scan_b", "scan_c", "scatter", "detect", "force", "cellreduce",

@@ -382,6 +382,7 @@ int rebuild_grid(avf_ctx *c, bool reorder) {
   HIPCHK(c, hipGetLastError());
   c->sgrid_valid = true;
   c->list_valid = false;  // storage order (and the sphere grid) changed
+  c->binned = false;      // the sorted sphere arrays describe the old grid and order
   c->steps_since_reorder = 0;
   return AVF_OK;
 }
@@ -1119,6 +1120,91 @@ int avf_check_cells(avf_ctx *c, int32_t inst, int64_t *bad) {
     o++;
   }
   if (bad) *bad = -1;
+  return AVF_OK;
+}
+
+// clear_cell / clear_ocup / cell_locate (cells.cu:11-111) on the current state, from the
+// device's clump-cell indices, clumps in ascending reference index.
+int avf_get_cell_table(avf_ctx *c, int32_t inst, int64_t *cell_vec, int32_t *nocup,
+                       int64_t ncell3, int64_t *bad) {
+  int rc = check_inst(c, inst);
+  if (rc) return rc;
+  if (!cell_vec || !nocup) return fail(c, AVF_ERR_ARG, "avf_get_cell_table: null table");
+  InstParams p;
+  HIPCHK(c, hipMemcpy(&p, c->d_P + inst, sizeof p, hipMemcpyDeviceToHost));
+  const int64_t want = p.ncell[0] * p.ncell[1] * p.ncell[2];
+  if (ncell3 != want)
+    return fail(c, AVF_ERR_ARG, "avf_get_cell_table: ncell3 %lld, the grid has %lld cells",
+                (long long)ncell3, (long long)want);
+  const int64_t cnt = c->desc[inst].sys.npart, ntags = c->desc[inst].ntags;
+  std::vector<int64_t> cells((size_t)cnt);
+  if ((rc = avf_get_clump_cells(c, inst, cells.data()))) return rc;
+  for (int64_t i = 0; i < ncell3 * ntags; i++) cell_vec[i] = -1;
+  for (int64_t i = 0; i < ncell3; i++) nocup[i] = 0;
+  if (bad) *bad = -1;
+  for (int64_t m = 0; m < cnt; m++) {
+    const int64_t idx = cells[(size_t)m];
+    if (idx == -1) continue;  // frozen (cells.cu:65)
+    if (idx == -2 || idx >= ncell3) {  // exits, or writes past the table (idx > ncell)
+      if (bad) *bad = m;
+      return fail(c, AVF_ERR_CELL_NEG, "clump %lld outside the clump-cell grid", (long long)m);
+    }
+    if (nocup[idx] == ntags - 1) {
+      if (bad) *bad = m;
+      return fail(c, AVF_ERR_CELL_FULL, "clump cell %lld full at clump %lld", (long long)idx,
+                  (long long)m);
+    }
+    cell_vec[idx * ntags + nocup[idx]] = m;
+    nocup[idx]++;
+  }
+  return AVF_OK;
+}
+
+// Sphere keys of the last binning, instance-relative, and the order of the counting sort.
+int avf_get_sphere_keys(avf_ctx *c, int32_t inst, uint32_t *keys, int64_t *order,
+                        int64_t *nbinned, avf_sphere_grid *grid) {
+  int rc = check_inst(c, inst);
+  if (rc) return rc;
+  if (!c->binned)
+    return fail(c, AVF_ERR_ARG,
+                "no sphere binning of the current state (call avf_initial_forces, or step with "
+                "verlet_skin = 0)");
+  InstParams p;
+  HIPCHK(c, hipMemcpy(&p, c->d_P + inst, sizeof p, hipMemcpyDeviceToHost));
+  if (grid) {
+    for (int k = 0; k < 3; k++) {
+      grid->lo[k] = p.sg_lo[k];
+      grid->n[k] = p.sg_n[k];
+    }
+    grid->h = p.sg_h;
+    grid->inv_h = p.sg_inv_h;
+    grid->reserved = 0;
+  }
+  const int64_t cnt = c->desc[inst].sys.npart, b = c->begin[inst];
+  std::vector<uint32_t> sk((size_t)(4 * cnt));
+  std::vector<int32_t> orig((size_t)cnt);
+  HIPCHK(c, hipMemcpy(sk.data(), c->d_skey + 4 * b, sk.size() * sizeof(uint32_t),
+                      hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(orig.data(), c->d_orig + b, orig.size() * sizeof(int32_t),
+                      hipMemcpyDeviceToHost));
+  std::vector<uint32_t> kref((size_t)(4 * cnt), 0xffffffffu);
+  for (int64_t k = 0; k < cnt; k++) {
+    const int64_t o = orig[(size_t)k];
+    if (o < 0 || o >= cnt) return fail(c, AVF_ERR_HIP, "clump order table corrupt");
+    for (int s = 0; s < 4; s++) {
+      const uint32_t key = sk[(size_t)(4 * k + s)];
+      kref[(size_t)(4 * o + s)] = key == kInvalidKey ? 0xffffffffu : key - (uint32_t)p.sg_base;
+    }
+  }
+  if (keys) std::memcpy(keys, kref.data(), kref.size() * sizeof(uint32_t));
+  std::vector<int64_t> ord;
+  ord.reserve(kref.size());
+  for (int64_t i = 0; i < 4 * cnt; i++)
+    if (kref[(size_t)i] != 0xffffffffu) ord.push_back(i);
+  std::stable_sort(ord.begin(), ord.end(),
+                   [&kref](int64_t a, int64_t b2) { return kref[(size_t)a] < kref[(size_t)b2]; });
+  if (order) std::memcpy(order, ord.data(), ord.size() * sizeof(int64_t));
+  if (nbinned) *nbinned = (int64_t)ord.size();
   return AVF_OK;
 }
 

@@ -255,6 +255,28 @@ class Context:
             self._ck(rc)
         return rc, bad.value
 
+    def cell_table(self, inst: int = 0) -> tuple[np.ndarray, np.ndarray]:
+        """The reference's (cell_vec, nocup) of the current state (cells.cu:11-111)."""
+        cl = self.cell(inst)
+        ncell3 = cl.ncell.x * cl.ncell.y * cl.ncell.z
+        cell_vec = np.zeros(ncell3 * self.instances[inst].ntags, dtype=np.int64)
+        nocup = np.zeros(ncell3, dtype=np.int32)
+        bad = C.c_int64(-1)
+        self._ck(lib().avf_get_cell_table(self.h, inst, ptr(cell_vec), ptr(nocup),
+                                          C.c_int64(ncell3), C.byref(bad)))
+        return cell_vec, nocup
+
+    def sphere_keys(self, inst: int = 0):
+        """(keys per sphere 4 m + s, sort order of the binned sphere ids, sphere grid)."""
+        n = self.n(inst)
+        keys = np.zeros(4 * n, dtype=np.uint32)
+        order = np.zeros(4 * n, dtype=np.int64)
+        nb = C.c_int64()
+        grid = abi.SphereGrid()
+        self._ck(lib().avf_get_sphere_keys(self.h, inst, ptr(keys), ptr(order), C.byref(nb),
+                                           C.byref(grid)))
+        return keys, order[:nb.value], grid
+
     def contacts(self, inst: int = 0, cap: int | None = None) -> np.ndarray:
         cap = cap if cap is not None else 64 * self.n(inst)
         out = np.zeros(cap, dtype=abi.contact_dtype)

@@ -267,6 +267,30 @@ int avf_check_cells(avf_ctx *ctx, int32_t inst, int64_t *bad);
  * (l, sm, sl, m).  *count receives the total even when it exceeds cap. */
 int avf_get_contacts(avf_ctx *ctx, int32_t inst, avf_contact *out, int64_t cap,
                      int64_t *count);
+/* The reference's clump-cell table of the current state, as clear_cell / clear_ocup /
+ * cell_locate leave it (cells.cu:11-111; replaces the long *cell_vec, int *nocup arguments
+ * of cabecera.hu:115-157): cell_vec[ncell3 * ntags] (-1 = empty slot, else the clump index,
+ * clumps in ascending order) and nocup[ncell3]; frozen clumps are skipped.  ncell3 must be
+ * the product of avf_get_cell's ncell.  Where cell_locate would exit (or, for an index past
+ * the table, write out of bounds): AVF_ERR_CELL_NEG / AVF_ERR_CELL_FULL, clump in *bad. */
+int avf_get_cell_table(avf_ctx *ctx, int32_t inst, int64_t *cell_vec, int32_t *nocup,
+                       int64_t ncell3, int64_t *bad);
+/* Sphere-cell grid of an instance: cell (i, j, k) holds the sphere centres q with
+ * i = clamp(floor((q.x - lo[0]) * inv_h), 0, n[0] - 1) etc.; key = i + n[0] (j + n[1] k). */
+typedef struct avf_sphere_grid {
+  double lo[3];
+  double h, inv_h;
+  int32_t n[3];
+  int32_t reserved;
+} avf_sphere_grid;
+/* Debug export of the binning (no reference counterpart: the reference bins clumps, not
+ * spheres): the sphere-cell key of sphere s of clump m at keys[4 m + s] (0xffffffff: not
+ * binned, the clump was frozen), the counting sort's order as sphere ids 4 m + s sorted by
+ * key (within a cell by id: the device places a cell's spheres in arrival order, on which
+ * no result depends), *nbinned entries, and the grid.  Needs a binned state (after
+ * avf_initial_forces or a per-step-binning step): AVF_ERR_ARG otherwise. */
+int avf_get_sphere_keys(avf_ctx *ctx, int32_t inst, uint32_t *keys, int64_t *order,
+                        int64_t *nbinned, avf_sphere_grid *grid);
 int avf_get_diag(avf_ctx *ctx, avf_diag *diag);
 
 /* Observables of one instance, reduced on the device in reference clump order (so they do

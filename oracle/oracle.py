@@ -55,8 +55,23 @@ def oracle_lib() -> C.CDLL:
         lib.orc_brute_contacts.restype = i64
         lib.orc_brute_contacts.argtypes = [vp, i64, vp, vp, i64]
         lib.orc_cell_setup.argtypes = [vp, vp, vp, vp]
+        lib.orc_cell_locate.restype = C.c_int
+        lib.orc_cell_locate.argtypes = [vp, vp, vp, vp, vp, vp]
         _orc = lib
     return _orc
+
+
+def cell_table(cell: abi.Cell, sys: abi.System, pp: np.ndarray):
+    """clear_cell / clear_ocup / cell_locate (cells.cu:11-111) of the restatement:
+    (cell_vec, nocup, status, offending clump)."""
+    cell_vec = np.full(cell.ncell3 * cell.ntags, -1, dtype=np.int64)
+    nocup = np.zeros(cell.ncell3, dtype=np.int32)
+    s = abi.System.from_buffer_copy(sys)
+    s.npart = len(pp)
+    bad = C.c_int64(-1)
+    rc = oracle_lib().orc_cell_locate(ptr(cell), ptr(s), ptr(np.ascontiguousarray(pp)),
+                                      ptr(nocup), ptr(cell_vec), C.byref(bad))
+    return cell_vec, nocup, rc, bad.value
 
 
 def ref_available() -> bool:
