@@ -50,6 +50,7 @@ struct avf_ctx {
   double *d_stage = nullptr, *d_f = nullptr, *d_io = nullptr, *d_io2 = nullptr;
   double *d_stage2 = nullptr;  // second stage buffer of the fused list steps
   bool lean = false;           // avf_options.lean_state resolved at create
+  std::vector<hipEvent_t> prof_ev;  // timing events of the profiled / timed steps
   int32_t *d_orig = nullptr, *d_orig2 = nullptr, *d_inst = nullptr, *d_inst2 = nullptr;
   int64_t *d_inv = nullptr, *d_ncells = nullptr, *d_tmp64 = nullptr;
   uint32_t *d_counts = nullptr, *d_starts = nullptr, *d_bsum = nullptr;
@@ -415,16 +416,24 @@ struct StepProf {
   int64_t launched = 0;
 };
 
+// At least n timing events, created once per context and reused: timing-only, so without
+// the system-scope cache writeback / invalidate at each record.
+int prof_events(avf_ctx *c, size_t n, hipEvent_t **out) {
+  while (c->prof_ev.size() < n) {
+    hipEvent_t e;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess)
+      return fail(c, AVF_ERR_HIP, "hipEventCreateWithFlags failed");
+    c->prof_ev.push_back(e);
+  }
+  *out = c->prof_ev.data();
+  return AVF_OK;
+}
+
 // S candidate-list (or one rebuild) steps launched eagerly with events between kernels.
 int vsteps_profiled(avf_ctx *c, int64_t S, bool rebuild, StepProf *prof) {
   constexpr int kEv = AVF_NKERNELS + 1;
-  std::vector<hipEvent_t> ev((size_t)S * kEv, nullptr);
-  int rc = AVF_OK;
-  for (auto &e : ev)
-    if (hipEventCreate(&e) != hipSuccess) {
-      rc = fail(c, AVF_ERR_HIP, "hipEventCreate failed");
-      break;
-    }
+  hipEvent_t *ev = nullptr;
+  int rc = prof_events(c, (size_t)S * kEv, &ev);
   if (rc == AVF_OK) {
     for (int64_t i = 0; i < S; i++)
       launch_vstep(c, rebuild, &ev[(size_t)i * kEv], i + 1 == S, i, S);
@@ -440,8 +449,6 @@ int vsteps_profiled(avf_ctx *c, int64_t S, bool rebuild, StepProf *prof) {
     if (he != hipSuccess) rc = fail(c, AVF_ERR_HIP, "event timing: %s", hipGetErrorString(he));
     prof->launched += S;
   }
-  for (auto &e : ev)
-    if (e) (void)hipEventDestroy(e);
   return rc;
 }
 
@@ -783,6 +790,7 @@ int avf_destroy(avf_ctx *c) {
                   c->d_stage2};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
+  for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
   free_window(c);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1004,7 +1012,7 @@ int avf_step_profiled(avf_ctx *c, int64_t nsteps, double *kernel_ms, double *tot
   // timings never leaves the GPU idle.
   constexpr int kBatch = 128, kEv = AVF_NKERNELS + 1;
   std::vector<hipEvent_t> ev((size_t)2 * kBatch * kEv);
-  for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
+  for (auto &e : ev) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
   hipEvent_t first, last;
   HIPCHK(c, hipEventCreate(&first));
   HIPCHK(c, hipEventCreate(&last));
