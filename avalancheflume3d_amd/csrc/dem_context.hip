@@ -459,7 +459,7 @@ int vsteps_profiled(avf_ctx *c, int64_t S, bool rebuild, StepProf *prof) {
 // chunk is undone from its checkpoint and redone after a rebuild; otherwise the next chunk
 // length is set from the bound's growth.  The results are those of per-step binning.
 int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
-  constexpr int64_t kMaxChunk = 64;
+  constexpr int64_t kMaxChunk = 128;
   const double skin = c->opt.verlet_skin;
   const size_t SF = (size_t)c->cap * S_NFIELDS * sizeof(double);
   int rc;

@@ -3,7 +3,7 @@
     python bench.py [--steps K] [--warmup W] [--workload headline|small|large]
 
 Times K steps of the headline configuration (250,000 tetrahedral 4-sphere clumps, FP64,
-candidate lists with a 0.01 cm skin, spatial reorder every 100 steps) with inputs resident
+candidate lists with a 0.01 cm skin, spatial reorder every 200 steps) with inputs resident
 in HBM, and prints one JSON line: clump-steps/s, the roofline of the dominant kernel
 (k_force, from HIP events over the timed region) and the CPU baseline (the oracle on one
 host core, bounded sample of the same workload).
@@ -34,7 +34,7 @@ STEP_BYTES_PER_CLUMP = 480.0
 # Candidate-list skin (cm) of the headline runs: 5 % of the sphere radius.
 DEFAULT_SKIN = 0.01
 # Spatial reorder interval (steps) of the headline runs; a reorder also rebuilds the lists.
-DEFAULT_REORDER = 100
+DEFAULT_REORDER = 200
 
 
 def cpu_baseline(inst, budget_s: float) -> dict:
