@@ -1479,18 +1479,25 @@ __global__ void k_finalize(InstParams *__restrict__ P, long long *__restrict__ a
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ninst) return;
   InstParams &p = P[i];
-  // cc % regrid_interval kept as a running remainder (one 64-bit division per launch)
+  // counter, time and cc % regrid_interval (a running remainder: one 64-bit division per
+  // launch) in registers; P is touched only on a regrid and at the end
   const int64_t ri = p.regrid_interval;
-  int64_t r = ri > 0 ? ((p.cc % ri) + ri) % ri : 1;
+  int64_t cc = p.cc;
+  int64_t r = ri > 0 ? ((cc % ri) + ri) % ri : 1;
   double t = p.time;
+  const double dt = p.dt;
   for (int s = 0; s < nsteps; s++) {  // the steps of a chunk, in order
-    if (force || (ri > 0 && r == 0)) finalize_params(p, acc + 8 * i, 1, 0);
+    if (force || (ri > 0 && r == 0)) {
+      p.cc = cc;
+      finalize_params(p, acc + 8 * i, 1, 0);
+    }
     if (step) {
-      p.cc += 1;
+      cc += 1;
       if (ri > 0 && ++r == ri) r = 0;
     }
-    if (advance_time) t = t + p.dt;
+    if (advance_time) t = t + dt;
   }
+  p.cc = cc;
   p.time = t;
 }
 
