@@ -73,6 +73,18 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
 }
 
+// The same idea at a finer grain for kernels whose work per workgroup varies with the local
+// density (k_force): XCD x processes runs of L consecutive logical workgroups, runs x, x + 8,
+// x + 16, ... in turn, so dense regions are shared by all XCDs while neighbouring workgroups
+// still share one L2.  A bijection on [0, nb) (the last nb mod 8L workgroups map to
+// themselves).
+__device__ __forceinline__ unsigned xcd_block_runs(unsigned b, unsigned nb, unsigned L) {
+  const unsigned span = kXcds * L, full = nb / span * span;
+  if (b >= full) return b;
+  const unsigned x = b % kXcds, k = b / kXcds;
+  return ((k / L) * kXcds + x) * L + (k % L);
+}
+
 // Running maximum of a diagnostic counter without contending on one address: a lane only
 // issues the atomic when its value exceeds what the counter already holds (after the first
 // steps almost no lane does).

@@ -1309,7 +1309,7 @@ __global__ __launch_bounds__(BS) void k_force(
                 "stage records of the workgroup are assembled in the pair scratch");
   __shared__ PPShared<MAXN, BS> sh;
   const int lane = threadIdx.x;
-  const int64_t c0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * BS;
+  const int64_t c0 = (int64_t)xcd_block_runs(blockIdx.x, gridDim.x, 16) * BS;
   const int64_t c = c0 + lane;
   const int inst = c < n ? inst_of[c] : -1;
   ClumpRegs cr, own;
