@@ -5,9 +5,12 @@
 //               record and the sphere-cell keys (integer atomic ranks: counting sort);
 //   k_scan_*    exclusive scan of the sphere-cell counts;
 //   k_scatter   places sphere records (x, y, z, r) + id in cell order;
-//   k_force     boundary modules + force_part (27-cell sweep of the sphere grid, contacts
-//               folded in the reference's summation order) + get_accel + finish_Verlet;
-//   k_cellreduce / k_finalize   cell_dim + regrid every regrid_interval steps, cc++.
+//   k_detect    the touching (or, with a skin, candidate) spheres of every sphere from the
+//               27 sphere cells around it; k_vmerge turns them into per-clump lists;
+//   k_force     boundary modules + force_part (contacts folded in the reference's summation
+//               order) + get_accel + finish_Verlet + the cell_dim reduction when the regrid
+//               is due; on candidate-list steps also the next step's predict;
+//   k_finalize  the regrid arithmetic and cc++ (per step, or once per chunk of list steps).
 // No float atomics anywhere: per-clump sums are sequential in the reference order.
 #include "dem_kernels.h"
 
@@ -58,14 +61,6 @@ __device__ __forceinline__ d3 sel4(int i, d3 a, d3 b, d3 c, d3 d) {
   return make_d3(lo ? (odd ? b.x : a.x) : (odd ? d.x : c.x),
                  lo ? (odd ? b.y : a.y) : (odd ? d.y : c.y),
                  lo ? (odd ? b.z : a.z) : (odd ? d.z : c.z));
-}
-
-__device__ __forceinline__ d3 pick(const d3 *s, int i) {
-  d3 r = s[0];
-  if (i == 1) r = s[1];
-  if (i == 2) r = s[2];
-  if (i == 3) r = s[3];
-  return r;
 }
 
 // --------------------------------------------------------------------- k_predict ----

@@ -320,7 +320,11 @@ typedef struct avf_profile_spec {
 int avf_observe(avf_ctx *ctx, int32_t inst, avf_observables *obs, const avf_profile_spec *spec,
                 double *profile);
 
-/* Kernels of one step, in launch order (indices of avf_step_profiled's kernel_ms). */
+/* Kernels of one step, in launch order (indices of avf_step_profiled's kernel_ms).  Slots
+ * of kernels a step does not launch read as empty (near 0): the binning kernels on
+ * candidate-list steps, the predict after the first step of a chunk (it runs inside the
+ * previous step's force kernel), the regrid reduction (inside the force kernel) and the
+ * finalize before the last step of a chunk. */
 enum {
   AVF_K_PREDICT, AVF_K_SCAN_A, AVF_K_SCAN_B, AVF_K_SCAN_C, AVF_K_SCATTER, AVF_K_DETECT,
   AVF_K_FORCE, AVF_K_CELLREDUCE, AVF_K_FINALIZE, AVF_NKERNELS
