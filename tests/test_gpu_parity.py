@@ -280,3 +280,27 @@ def test_batch_equals_separate_runs(gpu):
             assert np.array_equal(a[0][fld].view(np.uint64), b[0][fld].view(np.uint64)), (k, fld)
         one.close()
     ctx.close()
+
+
+def test_headline_size_lists_equal_per_step_binning(gpu):
+    """At the headline size (250,000 clumps, 1M spheres): 150 steps on candidate lists, in
+    both state layouts and with a list rebuild forced by a reorder every 64 steps, equal
+    per-step binning bit for bit (state, accelerations, last forces, time and counter)."""
+    from avalancheflume3d_amd import workloads
+    inst = workloads.headline()
+    outs = []
+    for kw in ({}, {"skin": 0.01, "reorder": 64, "lean": -1}, {"skin": 0.01, "reorder": 64, "lean": 1}):
+        ctx = gpu_ctx(inst, **kw)
+        ctx.initial_forces()
+        ctx.step(150)
+        pp, aa, ga = ctx.download()
+        outs.append((pp, aa, ga, ctx.forces()[0], ctx.time()))
+        assert ctx.diag()["neigh_overflow"] == 0
+        ctx.close()
+    ref = outs[0]
+    for got in outs[1:]:
+        for fld in STATE:
+            assert np.array_equal(ref[0][fld].view(np.uint64), got[0][fld].view(np.uint64)), fld
+        for k in (1, 2, 3):
+            assert np.array_equal(ref[k].view(np.uint64), got[k].view(np.uint64))
+        assert ref[4] == got[4]
