@@ -10,7 +10,9 @@
 //   last_config.dat (the reference's 18-column text checkpoint) and last_config.bin (lossless).
 //
 //   avalanche_amd [--restart] [--restart-bin FILE] [--max-steps N] [--final-bin FILE]
-//                 [--device D] < DataInput.dat
+//                 [--device D] [--skin CM] [--reorder N] < DataInput.dat
+// --skin: candidate-list skin (default 0.01 cm; 0 bins the spheres every step), --reorder:
+// spatial reorder interval (default 200 steps).  Neither changes any result.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -186,15 +188,21 @@ int main(int argc, char **argv) {
   const char *final_bin = nullptr;
   avf_options opt;
   avf_options_default(&opt);
+  opt.verlet_skin = 0.01;
+  opt.reorder_interval = 200;
   for (int i = 1; i < argc; i++) {
     if (!std::strcmp(argv[i], "--restart")) restart = true;
     else if (!std::strcmp(argv[i], "--restart-bin") && i + 1 < argc) restart_bin = argv[++i];
     else if (!std::strcmp(argv[i], "--max-steps") && i + 1 < argc) max_steps = std::atoll(argv[++i]);
     else if (!std::strcmp(argv[i], "--device") && i + 1 < argc) opt.device = std::atoi(argv[++i]);
     else if (!std::strcmp(argv[i], "--final-bin") && i + 1 < argc) final_bin = argv[++i];
+    else if (!std::strcmp(argv[i], "--skin") && i + 1 < argc) opt.verlet_skin = std::atof(argv[++i]);
+    else if (!std::strcmp(argv[i], "--reorder") && i + 1 < argc)
+      opt.reorder_interval = std::atoi(argv[++i]);
     else {
       std::fprintf(stderr, "usage: %s [--restart] [--restart-bin FILE] [--max-steps N] "
-                           "[--final-bin FILE] [--device D] < DataInput.dat\n", argv[0]);
+                           "[--final-bin FILE] [--device D] [--skin CM] [--reorder N] "
+                           "< DataInput.dat\n", argv[0]);
       return 2;
     }
   }

@@ -100,6 +100,12 @@ def test_driver_binary_restart_is_lossless(gpu, tmp_path):
     assert ha[1] == hb[1] == 14
     assert pa.tobytes() == pb.tobytes()
     assert aa_.tobytes() == ab_.tobytes() and ga_.tobytes() == gb_.tobytes()
+    # the default candidate lists and per-step binning (--skin 0) write the same state
+    c = tmp_path / "c"
+    c.mkdir()
+    assert run(c, ["--skin", "0", "--max-steps", "14", "--final-bin", "end.bin"], text).returncode == 0
+    hc, pc, ac_, gc_ = read_bin(c / "end.bin", n)
+    assert pc.tobytes() == pa.tobytes() and ac_.tobytes() == aa_.tobytes()
     # the text checkpoint (6 decimals) is read back by --restart
     r = run(b, ["--restart", "--max-steps", "2"], text)
     assert r.returncode == 0, r.stderr
