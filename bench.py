@@ -56,7 +56,8 @@ def cpu_baseline(inst, budget_s: float) -> dict:
                       f"({dt:.1f} s on one host core)"}
 
 
-def batch_compare(steps: int, warmup: int = 20, skin: float = DEFAULT_SKIN) -> dict:
+def batch_compare(steps: int, warmup: int = 20, skin: float = DEFAULT_SKIN,
+                  sequential: bool = True) -> dict:
     """64 independent flumes (N ~ log-uniform[5e3, 6e4], per-instance seeds, mu, eps and
     slope) stepped together in one context vs the same instances as sequential plain runs
     (one context each, synchronised between instances)."""
@@ -76,6 +77,9 @@ def batch_compare(steps: int, warmup: int = 20, skin: float = DEFAULT_SKIN) -> d
     ctx.sync()
     t_batch = time.perf_counter() - t0
     ctx.close()
+    if not sequential:
+        return {"instances": len(insts), "clumps": total, "steps": steps, "skin_cm": skin,
+                "batched_clump_steps_per_s": total * steps / t_batch}
     t_seq = 0.0
     for inst in insts:
         c = dem.Context([inst], opt)
@@ -224,6 +228,8 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-only", action="store_true")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batched run of the ensemble (no sequential comparison)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy kernel of this many bytes 5 times "
                          "(calibrates the HBM byte counters against a known traffic)")
@@ -245,6 +251,11 @@ def main() -> int:
         return 0
 
     from avalancheflume3d_amd import dem, workloads
+
+    if args.batch_only:
+        print(json.dumps({"batched": batch_compare(args.batch_steps, sequential=False)}),
+              flush=True)
+        return 0
 
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_bench(args.adjoint_steps)}), flush=True)
