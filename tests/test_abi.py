@@ -43,6 +43,21 @@ def test_create_without_device_reports_error():
     assert e.value.code == 2 and "hip" in str(e.value).lower()
 
 
+def test_create_rejects_empty_and_bad_batches():
+    """Argument errors come back before any device call: an instance without clumps, an
+    empty batch."""
+    sys, geo, fun = dem_parts()
+    sys.npart = 0
+    desc = (abi.InstanceDesc * 1)()
+    desc[0].sys, desc[0].geo, desc[0].fun, desc[0].ntags = sys, geo, fun, 9
+    h = C.c_void_p()
+    lib = abi.load_library()
+    assert lib.avf_create(C.byref(h), desc, 1, None) == abi.STATUS_CODES["AVF_ERR_ARG"]
+    assert b"npart" in lib.avf_last_error(None)
+    assert lib.avf_create(C.byref(h), desc, 0, None) == abi.STATUS_CODES["AVF_ERR_ARG"]
+    assert lib.avf_create(None, desc, 1, None) == abi.STATUS_CODES["AVF_ERR_ARG"]
+
+
 def dem_parts():
     from tests import scenarios
     sys, geo, fun, nt = scenarios.reference_system(4)

@@ -304,3 +304,27 @@ def test_headline_size_lists_equal_per_step_binning(gpu):
         for k in (1, 2, 3):
             assert np.array_equal(ref[k].view(np.uint64), got[k].view(np.uint64))
         assert ref[4] == got[4]
+
+
+def test_tiny_and_ragged_batch_on_candidate_lists(gpu):
+    """Instances of 1, 2 and 7 clumps batched with a 300-clump flume, stepped on candidate
+    lists, equal each instance stepped alone with per-step binning bit for bit."""
+    from avalancheflume3d_amd import workloads
+    insts = [workloads.build(workloads.FlumeSpec(n=k, width=2.0, length=4.0,
+                                                 box=(2.0, 2.0, 0.4 * max(1, k // 25 + 1) + 0.01),
+                                                 deposit_length=4.0, seed=70 + k))
+             for k in (1, 2, 7, 300)]
+    batch = gpu_ctx(insts, skin=0.01, reorder=30)
+    batch.initial_forces()
+    batch.step(250)
+    for k, inst in enumerate(insts):
+        one = gpu_ctx(inst)
+        one.initial_forces()
+        one.step(250)
+        pa = one.download()[0]
+        pb = batch.download(k)[0]
+        for fld in STATE:
+            assert np.array_equal(pa[fld].view(np.uint64), pb[fld].view(np.uint64)), (k, fld)
+        assert one.time() == batch.time(k)
+        one.close()
+    batch.close()
