@@ -220,7 +220,7 @@ def load_library() -> C.CDLL:
             "avf_get_diag": (C.c_int, [vp, vp]),
             "avf_sync": (C.c_int, [vp]),
             "avf_step_profiled": (C.c_int, [vp, i64, vp, vp]),
-            "avf_step_timed": (C.c_int, [vp, i64, i32, vp, vp]),
+            "avf_step_timed": (C.c_int, [vp, i64, i32, i32, vp, vp]),
             "avf_observe": (C.c_int, [vp, i32, vp, vp, vp]),
             "avf_window_run": (C.c_int, [vp] * 15),
         }

@@ -82,6 +82,8 @@ struct avf_ctx {
   std::vector<hipGraph_t> graph_defs;
   struct WinBuf *win = nullptr;  // buffers of the differentiable window (on first use)
   int32_t prof_kernel = -1;      // avf_step_timed: the one kernel bracketed by events
+  int32_t prof_every = 1;        // avf_step_timed: events on every prof_every-th step
+  int64_t prof_seq = 0;          // steps launched under the profiler
 };
 
 namespace {
@@ -435,19 +437,26 @@ int vsteps_profiled(avf_ctx *c, int64_t S, bool rebuild, StepProf *prof) {
   hipEvent_t *ev = nullptr;
   int rc = prof_events(c, (size_t)S * kEv, &ev);
   if (rc == AVF_OK) {
-    for (int64_t i = 0; i < S; i++)
-      launch_vstep(c, rebuild, &ev[(size_t)i * kEv], i + 1 == S, i, S);
+    std::vector<int64_t> timed;  // steps of this chunk carrying events
+    for (int64_t i = 0; i < S; i++) {
+      const bool rec = c->prof_every <= 1 || c->prof_seq % c->prof_every == 0;
+      c->prof_seq++;
+      if (rec) timed.push_back(i);
+      launch_vstep(c, rebuild, rec ? &ev[(size_t)i * kEv] : nullptr, i + 1 == S, i, S);
+    }
     const int jlast = c->prof_kernel >= 0 ? c->prof_kernel + 1 : AVF_NKERNELS;
-    hipError_t he = hipEventSynchronize(ev[(size_t)(S - 1) * kEv + jlast]);
-    for (int64_t i = 0; i < S && he == hipSuccess; i++)
+    hipError_t he = hipSuccess;
+    if (!timed.empty()) he = hipEventSynchronize(ev[(size_t)timed.back() * kEv + jlast]);
+    for (size_t t = 0; t < timed.size() && he == hipSuccess; t++)
       for (int j = 0; j < AVF_NKERNELS && he == hipSuccess; j++) {
         if (c->prof_kernel >= 0 && j != c->prof_kernel) continue;
+        const size_t i = (size_t)timed[t];
         float ms = 0.f;
-        he = hipEventElapsedTime(&ms, ev[(size_t)i * kEv + j], ev[(size_t)i * kEv + j + 1]);
+        he = hipEventElapsedTime(&ms, ev[i * kEv + j], ev[i * kEv + j + 1]);
         prof->sum[j] += ms;
       }
     if (he != hipSuccess) rc = fail(c, AVF_ERR_HIP, "event timing: %s", hipGetErrorString(he));
-    prof->launched += S;
+    prof->launched += (int64_t)timed.size();
   }
   return rc;
 }
@@ -1074,13 +1083,16 @@ int avf_step_profiled(avf_ctx *c, int64_t nsteps, double *kernel_ms, double *tot
   return rc;
 }
 
-int avf_step_timed(avf_ctx *c, int64_t nsteps, int32_t kernel, double *kernel_ms,
-                   double *total_ms) {
-  if (!c || nsteps < 0 || kernel < 0 || kernel >= AVF_NKERNELS) return AVF_ERR_ARG;
+int avf_step_timed(avf_ctx *c, int64_t nsteps, int32_t kernel, int32_t every,
+                   double *kernel_ms, double *total_ms) {
+  if (!c || nsteps < 0 || kernel < 0 || kernel >= AVF_NKERNELS || every < 1) return AVF_ERR_ARG;
   std::vector<double> all(AVF_NKERNELS, 0.0);
   c->prof_kernel = kernel;
+  c->prof_every = every;
+  c->prof_seq = 0;
   const int rc = avf_step_profiled(c, nsteps, all.data(), total_ms);
   c->prof_kernel = -1;
+  c->prof_every = 1;
   if (kernel_ms) *kernel_ms = all[kernel];
   return rc;
 }

@@ -227,11 +227,12 @@ class Context:
         self._ck(lib().avf_step_profiled(self.h, C.c_int64(n), ptr(ks), C.byref(tot)))
         return dict(zip(abi.KERNELS, ks.tolist())), tot.value
 
-    def step_timed(self, n: int, kernel: str) -> tuple[float, float]:
-        """n steps with HIP events around one kernel only: (its mean ms, total ms)."""
+    def step_timed(self, n: int, kernel: str, every: int = 1) -> tuple[float, float]:
+        """n steps with HIP events around one kernel on every `every`-th step only:
+        (its mean ms per timed launch, total ms)."""
         km, tot = C.c_double(), C.c_double()
         self._ck(lib().avf_step_timed(self.h, C.c_int64(n), abi.KERNELS.index(kernel),
-                                      C.byref(km), C.byref(tot)))
+                                      C.c_int32(every), C.byref(km), C.byref(tot)))
         return km.value, tot.value
 
     def sync(self) -> None:

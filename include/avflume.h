@@ -333,10 +333,12 @@ enum {
  * context's stream: kernel_ms[k] receives the mean duration of kernel k per step and
  * *total_ms the elapsed time from the first to the last kernel of the run. */
 int avf_step_profiled(avf_ctx *ctx, int64_t nsteps, double *kernel_ms, double *total_ms);
-/* Like avf_step_profiled with events around one kernel only (AVF_K_*): *kernel_ms receives
- * its mean duration per step, *total_ms the elapsed time of the run. */
-int avf_step_timed(avf_ctx *ctx, int64_t nsteps, int32_t kernel, double *kernel_ms,
-                   double *total_ms);
+/* Like avf_step_profiled with events around one kernel only (AVF_K_*), on every
+ * `every`-th step (1: all; on candidate-list steps, fewer events disturb the run less):
+ * *kernel_ms receives its mean duration per timed launch, *total_ms the elapsed time of the
+ * run. */
+int avf_step_timed(avf_ctx *ctx, int64_t nsteps, int32_t kernel, int32_t every,
+                   double *kernel_ms, double *total_ms);
 /* Waits for all work queued on the context's stream. */
 int avf_sync(avf_ctx *ctx);
 
