@@ -246,12 +246,21 @@ def main() -> int:
     import torch
 
     if args.copy_probe:
+        # a device-to-device copy of known size: calibrates the byte counters, and its rate is
+        # the practical read + write bandwidth a streaming kernel of that size reaches
         x = torch.ones(args.copy_probe // 4, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)
-        for _ in range(5):
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for k in range(5):
+            if k == 1:
+                t0.record()
             y.copy_(x)
+        t1.record()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": args.copy_probe, "launches": 5}), flush=True)
+        ms = t0.elapsed_time(t1) / 4
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "launches": 5, "ms_per_copy": ms,
+                          "read_plus_write_gbs": 2 * args.copy_probe / (ms * 1e-3) / 1e9}),
+              flush=True)
         return 0
 
     from avalancheflume3d_amd import dem, workloads
