@@ -959,11 +959,15 @@ __device__ __forceinline__ void force_clump(
   const double rb = 1.7320508075688772 * pos0 * (1.0 + 1e-12) + 1e-9;
 
   const bool frz = frozen(time, rr.x, p.side_x);
-  const int mods = p.phase ? p.modules_aval : p.modules_fill;
+  // the initial force pass (avalanche_hst.cu:365-384) runs force_part and then force_hop in
+  // either phase, as the filling step does; the other boundary modules join on the steps
+  const bool aval = p.phase && !initial;
+  const int mods = (p.phase ? p.modules_aval : p.modules_fill) &
+                   (initial ? (AVF_MOD_PP | AVF_MOD_HOP) : ~0);
   const double massw = kSphFact * rad * rad * rad * rho;  // force_hop.cu:45 order
   d3 ff = make_d3(0.0, 0.0, 0.0), tq = make_d3(0.0, 0.0, 0.0);
 
-  if (p.phase) {  // avalanche_hst.cu:688-696
+  if (aval) {  // avalanche_hst.cu:688-696
     if (mods & AVF_MOD_HOP)
 #pragma unroll 1
       for (int nw = 0; nw < 3; nw++) hop_walls(p, nw, time, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
@@ -1164,7 +1168,7 @@ __device__ __forceinline__ void force_clump(
     }
   }
 
-  if (!p.phase) {  // avalanche_hst.cu:460-461 (pp first, then hop, while filling)
+  if (!aval) {  // avalanche_hst.cu:460-461 (pp first, then hop, while filling)
     if (mods & AVF_MOD_HOP)
 #pragma unroll 1
       for (int nw = 0; nw < 3; nw++) hop_walls(p, nw, time, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);

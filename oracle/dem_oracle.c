@@ -926,9 +926,13 @@ static void orc_pp_passes(orc_sim *s) {
 
 /* Fill: avalanche_hst.cu:445-461.  Avalanche: avalanche_hst.cu:678-704 (+ the deposit
  * floor after the flume when enabled). */
-int orc_sim_forces(orc_sim *s) {
+/* One force stage.  initial != 0: the initial force pass of avalanche_hst.cu:365-384, which
+ * runs force_part and then force_hop in either phase (the filling composition). */
+static int orc_sim_forces_pass(orc_sim *s, int initial) {
   int rc;
   int32_t mods = s->phase == AVF_PHASE_FILL ? s->geo.modules_fill : s->geo.modules_avalanche;
+  const int fill = s->phase == AVF_PHASE_FILL || initial;
+  if (initial) mods &= AVF_MOD_PP | AVF_MOD_HOP;
   orc_clear_forces(s->n, s->ff, s->tq);
   orc_clear_cell(&s->cell, s->cell_vec);
   orc_clear_ocup(&s->cell, s->nocup);
@@ -938,7 +942,7 @@ int orc_sim_forces(orc_sim *s) {
     return rc;
   }
   if (s->rec) s->rec->count = 0;
-  if (s->phase == AVF_PHASE_FILL) {
+  if (fill) {
     if (mods & AVF_MOD_PP) orc_pp_passes(s);
     if (mods & AVF_MOD_HOP)
       for (int nw = 0; nw < 3; nw++) orc_force_hop(s->ff, s->tq, &s->sys, s->pp, &s->geo, nw);
@@ -956,8 +960,10 @@ int orc_sim_forces(orc_sim *s) {
   return 0;
 }
 
+int orc_sim_forces(orc_sim *s) { return orc_sim_forces_pass(s, 0); }
+
 int orc_sim_initial(orc_sim *s) {
-  int rc = orc_sim_forces(s);
+  int rc = orc_sim_forces_pass(s, 1);
   if (rc) return rc;
   orc_get_accel(s->ff, s->tq, s->aa_i, s->anga_i, &s->sys, s->pp, &s->geo);
   return 0;

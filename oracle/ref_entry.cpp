@@ -161,12 +161,14 @@ void ref_sim_destroy(void *h) {
   free(s->anga_f); free(s->cell_vec); free(s->nocup); free(s);
 }
 
-static void ref_forces(ref_sim *s) {
+/* initial != 0: the initial force pass of avalanche_hst.cu:365-384 (force_part, then
+ * force_hop, in either phase) */
+static void ref_forces(ref_sim *s, int initial = 0) {
   clear_forces_hst(s->sys, s->ff, s->tq);
   clear_cell_hst(s->cell, s->cell_vec);
   clear_ocup_hst(s->cell, s->nocup);
   cell_locate_hst(s->cell, s->sys, s->pp, s->nocup, s->cell_vec);
-  if (s->phase == 0) {
+  if (s->phase == 0 || initial) {
     for (int i = 1; i <= 16; i++)
       force_part_hst(s->ff, s->tq, s->cell_vec, s->nocup, s->pp, s->sys, s->cell, i);
     for (int i = 0; i < 3; i++) force_hop_hst(s->ff, s->tq, s->sys, s->pp, (char)i);
@@ -184,7 +186,7 @@ static void ref_forces(ref_sim *s) {
 /* avalanche_hst.cu:365-384 */
 void ref_sim_initial(void *h) {
   ref_sim *s = (ref_sim *)h;
-  ref_forces(s);
+  ref_forces(s, 1);
   get_accel_hst(s->ff, s->tq, s->aa_i, s->anga_i, s->sys, s->pp);
 }
 

@@ -328,3 +328,36 @@ def test_tiny_and_ragged_batch_on_candidate_lists(gpu):
         assert one.time() == batch.time(k)
         one.close()
     batch.close()
+
+
+@pytest.mark.parametrize("skin", [0.0, 0.01])
+def test_small_configuration_100_steps_vs_oracle(gpu, skin):
+    """The S configuration (2,000 clumps in a 4x4x8 cm box above a 35 deg flume) against the
+    oracle over 100 steps: state within 1e-9, last forces within 1e-12 (per-step binning and
+    candidate lists)."""
+    from avalancheflume3d_amd import workloads
+    exact = run_pair(workloads.small(), 100, 50, skin=skin)
+    print(f"bit-exact S trajectory (skin {skin}):", exact)
+
+
+def test_headline_configuration_vs_oracle(gpu):
+    """The headline configuration (250,000 clumps): initial forces within 1e-12 and three
+    steps on candidate lists within 1e-9 of the oracle."""
+    from avalancheflume3d_amd import workloads
+    inst = workloads.headline()
+    ctx = gpu_ctx(inst, skin=0.01, reorder=200)
+    ctx.initial_forces()
+    o = oracle_sim(inst)
+    o.initial()
+    f, t = ctx.forces()
+    r = o.get()
+    exact = [vec_close(f, r["ff"], F_RTOL, "force"), vec_close(t, r["tq"], F_RTOL, "torque")]
+    ctx.step(3)
+    o.step(3)
+    pp = ctx.download()[0]
+    r = o.get()
+    for fld in STATE:
+        exact.append(vec_close(pp[fld], r["pp"][fld], T_RTOL, f"{fld} after 3 steps"))
+    exact.append(vec_close(ctx.forces()[0], r["ff"], F_RTOL, "force after 3 steps"))
+    print("bit-exact headline initial forces and 3 steps:", all(exact))
+    ctx.close()
