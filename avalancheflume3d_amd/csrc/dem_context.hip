@@ -61,7 +61,7 @@ struct avf_ctx {
   // candidate sphere pairings (entry-major), displacement tracking, chunk checkpoint
   int32_t *d_vl_m = nullptr, *d_vl_cnt = nullptr;
   uint16_t *d_vl_mask = nullptr;
-  double *d_xb = nullptr, *d_vrot = nullptr, *d_dsp = nullptr;
+  double *d_dsp = nullptr;
   double *d_ck_st = nullptr, *d_ck_rad = nullptr;
   InstParams *d_ck_P = nullptr;
   DevDiag *d_ck_diag = nullptr;
@@ -199,14 +199,14 @@ void launch_scan(avf_ctx *c) {
 void launch_bin(avf_ctx *c, int mode) {
   k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
       c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-      c->d_srank, c->n, c->cap, mode, ListTrack{nullptr, nullptr, nullptr});
+      c->d_srank, c->n, c->cap, mode, ListTrack{nullptr});
   launch_scan(c);
   k_scatter<<<std::max(1u, grid_for(4 * c->n, kBlock)), kBlock, 0, c->stream>>>(
-      c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid, c->d_P, c->ninst,
+      c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid, c->d_inst, c->d_P, c->ninst,
       4 * c->n, mode);
 }
 
-ListTrack list_track(avf_ctx *c) { return ListTrack{c->d_xb, c->d_vrot, c->d_dsp}; }
+ListTrack list_track(avf_ctx *c) { return ListTrack{c->d_dsp}; }
 
 template <int BS, bool VERLET, bool NEXT, bool LEAN = false>
 void launch_force_bs(avf_ctx *c, int initial, bool ft, int time_ahead, const double *stage_in,
@@ -214,7 +214,7 @@ void launch_force_bs(avf_ctx *c, int initial, bool ft, int time_ahead, const dou
   k_force<kMaxNeigh, BS, VERLET, NEXT, LEAN><<<grid_for(c->n, BS), BS, 0, c->stream>>>(
       c->d_st, c->d_sta, stage_in, c->d_inst, c->d_P, c->d_clist, c->d_ccnt, c->d_vl_m,
       c->d_vl_mask, c->d_vl_cnt, ft ? c->d_f : nullptr, c->d_diag, c->d_acc_cell, stage_out,
-      c->d_orig, NEXT ? list_track(c) : ListTrack{nullptr, nullptr, nullptr}, c->n, c->cap,
+      c->d_orig, NEXT ? list_track(c) : ListTrack{nullptr}, c->n, c->cap,
       initial, time_ahead, si);
 }
 
@@ -277,7 +277,7 @@ void launch_step(avf_ctx *c, hipEvent_t *ev = nullptr, bool ft = true) {
   mark(AVF_K_PREDICT);
   k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
       c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-      c->d_srank, c->n, c->cap, 1, ListTrack{nullptr, nullptr, nullptr});
+      c->d_srank, c->n, c->cap, 1, ListTrack{nullptr});
   mark(AVF_K_SCAN_A);
   k_scan_a<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
   mark(AVF_K_SCAN_B);
@@ -286,7 +286,7 @@ void launch_step(avf_ctx *c, hipEvent_t *ev = nullptr, bool ft = true) {
   k_scan_c<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_starts, c->d_ncells);
   mark(AVF_K_SCATTER);
   k_scatter<<<std::max(1u, grid_for(4 * c->n, kBlock)), kBlock, 0, c->stream>>>(
-      c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid, c->d_P, c->ninst,
+      c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid, c->d_inst, c->d_P, c->ninst,
       4 * c->n, 1);
   mark(AVF_K_DETECT);
   launch_detect(c);
@@ -330,7 +330,7 @@ void launch_vstep(avf_ctx *c, bool rebuild, hipEvent_t *ev = nullptr, bool ft = 
   mark(AVF_K_SCATTER);
   if (rebuild)
     k_scatter<<<std::max(1u, grid_for(4 * c->n, kBlock)), kBlock, 0, c->stream>>>(
-        c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid, c->d_P, c->ninst,
+        c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid, c->d_inst, c->d_P, c->ninst,
         4 * c->n, 1);
   mark(AVF_K_DETECT);
   if (rebuild) {
@@ -639,10 +639,9 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
   }
   c->n = total;
   c->cap = total;
-  // lean chunks once the step's working set (state, constants, both stage buffers, lists,
-  // tracking: about 870 B per clump) outgrows the 256 MiB last-level cache of the MI355X
-  c->lean = c->opt.lean_state > 0 ||
-            (c->opt.lean_state == 0 && (double)total * 870.0 > 256.0 * 1024.0 * 1024.0);
+  // lean chunks unless asked otherwise: the stage record holds the whole carried state but
+  // the angular acceleration, so a list step reads and writes each clump's state once
+  c->lean = c->opt.lean_state >= 0;
   c->cell_cap = std::max<int64_t>(8 * 4 * total, 1 << 16);
   if (c->cell_cap > (int64_t)0xfffffff0) c->cell_cap = 0xfffffff0;
 
@@ -688,8 +687,7 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
     return bail(rc);
   if (c->opt.verlet_skin > 0.0 &&
       ((rc = dalloc(c, &c->d_vl_m, kVMax * N)) || (rc = dalloc(c, &c->d_vl_mask, kVMax * N)) ||
-       (rc = dalloc(c, &c->d_vl_cnt, N)) || (rc = dalloc(c, &c->d_xb, 3 * N)) ||
-       (rc = dalloc(c, &c->d_vrot, N)) || (rc = dalloc(c, &c->d_dsp, N)) ||
+       (rc = dalloc(c, &c->d_vl_cnt, N)) || (rc = dalloc(c, &c->d_dsp, N)) ||
        (rc = dalloc(c, &c->d_ck_st, N * S_NFIELDS)) || (rc = dalloc(c, &c->d_ck_rad, N)) ||
        (rc = dalloc(c, &c->d_ck_P, ninst)) || (rc = dalloc(c, &c->d_ck_diag, 1)) ||
        (rc = dalloc(c, &c->d_umax, 1)) || (rc = dalloc(c, &c->d_stage2, N * kStageDoubles))))
@@ -794,7 +792,7 @@ int avf_destroy(avf_ctx *c) {
                   c->d_inst2,  c->d_inv,   c->d_ncells, c->d_tmp64, c->d_counts, c->d_starts,
                   c->d_bsum,   c->d_skey,  c->d_srank, c->d_sid,   c->d_spos,  c->d_acc_cell,
                   c->d_acc_sg, c->d_diag,  c->d_count, c->d_clist, c->d_ccnt,
-                  c->d_vl_m,   c->d_vl_cnt, c->d_vl_mask, c->d_xb, c->d_vrot, c->d_dsp,
+                  c->d_vl_m,   c->d_vl_cnt, c->d_vl_mask, c->d_dsp,
                   c->d_ck_st,  c->d_ck_rad, c->d_ck_P, c->d_ck_diag, c->d_umax,
                   c->d_stage2};
   for (void *b : bufs)
@@ -1491,12 +1489,12 @@ void win_bin(avf_ctx *c, const double *st) {
   const unsigned nb = grid_for(c->cell_cap, kScanTile);
   k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
       const_cast<double *>(st), c->d_sta, w->bstage, c->d_inst, c->d_orig, c->d_P, c->d_counts,
-      w->bkey, w->brank, c->n, c->cap, 0, ListTrack{nullptr, nullptr, nullptr});
+      w->bkey, w->brank, c->n, c->cap, 0, ListTrack{nullptr});
   k_scan_a<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
   k_scan_b<<<1, 1024, 0, c->stream>>>(c->d_bsum, c->d_ncells, w->bstarts);
   k_scan_c<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, w->bstarts, c->d_ncells);
   k_scatter<<<std::max(1u, grid_for(4 * c->n, kBlock)), kBlock, 0, c->stream>>>(
-      w->bkey, w->brank, w->bstarts, w->bstage, w->bspos, w->bsid, c->d_P, c->ninst, 4 * c->n, 0);
+      w->bkey, w->brank, w->bstarts, w->bstage, w->bspos, w->bsid, c->d_inst, c->d_P, c->ninst, 4 * c->n, 0);
 }
 
 // Adds the touching sphere pairs of state st that the frozen list lacks to w->missed.

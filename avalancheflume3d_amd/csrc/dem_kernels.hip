@@ -55,20 +55,22 @@ __device__ __forceinline__ d3 sphere_of(d3 rr, const double *R, double pos, int 
                  rr.z + pos * (sx * R[6] + sy * R[7] + sz * R[8]));
 }
 
-// One of four values by a select chain on registers (no indexed array, no scratch).
-__device__ __forceinline__ d3 sel4(int i, d3 a, d3 b, d3 c, d3 d) {
-  const bool lo = i < 2, odd = i & 1;
-  return make_d3(lo ? (odd ? b.x : a.x) : (odd ? d.x : c.x),
-                 lo ? (odd ? b.y : a.y) : (odd ? d.y : c.y),
-                 lo ? (odd ? b.z : a.z) : (odd ? d.z : c.z));
+// Centre of sphere i of the clump whose stage record is g (its centre and rotation rows);
+// pos = the clump's radius / div (the record's radius: only records of clumps that are not
+// frozen, whose radius is the one they had before freezing, are used this way).
+__device__ __forceinline__ d3 rec_sphere(const double *__restrict__ g, double pos, int i) {
+  double R[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) R[k] = g[G_R + k];
+  return sphere_of(make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]), R, pos, i);
 }
 
 // --------------------------------------------------------------------- k_predict ----
 
-// Displacement tracking of the candidate-list step: centre at the list build, accumulated
-// rotation angle and the largest sphere displacement bound since the build, per clump.
+// Displacement tracking of the candidate-list step: per clump, a bound on how far any of
+// its spheres has moved since the list build.
 struct ListTrack {
-  double *xb, *vrot, *dsp;
+  double *dsp;
 };
 
 // init_Verlet.cu:46-114 on one clump in registers: position and velocity by the old
@@ -127,9 +129,10 @@ __device__ __forceinline__ double verlet_advance(d3 &rr, d3 &vv, d3 &ww, double 
 }
 
 // The 24-double stage record of a clump (what neighbours gather): centre, radius, velocity,
-// density, angular velocity, reference index, 4 sphere centres.
+// density, angular velocity, reference index, rotation rows, acceleration.
 __device__ __forceinline__ void fill_stage(double *__restrict__ g, d3 rr, double rad, d3 vv,
-                                           double rho, d3 ww, uint32_t id, const d3 *sph) {
+                                           double rho, d3 ww, uint32_t id, const double *R,
+                                           d3 aa) {
   g[G_RR + 0] = rr.x;
   g[G_RR + 1] = rr.y;
   g[G_RR + 2] = rr.z;
@@ -143,22 +146,21 @@ __device__ __forceinline__ void fill_stage(double *__restrict__ g, d3 rr, double
   g[G_WW + 2] = ww.z;
   g[G_ID] = __longlong_as_double((long long)id);
 #pragma unroll
-  for (int s = 0; s < 4; s++) {
-    g[G_SPH + 3 * s + 0] = sph[s].x;
-    g[G_SPH + 3 * s + 1] = sph[s].y;
-    g[G_SPH + 3 * s + 2] = sph[s].z;
-  }
+  for (int k = 0; k < 9; k++) g[G_R + k] = R[k];
+  g[G_AA + 0] = aa.x;
+  g[G_AA + 1] = aa.y;
+  g[G_AA + 2] = aa.z;
 }
 
-// Candidate-list displacement bound of clump c since the list build:
-// |x' - x_b| + sqrt(3) pos * sum |dOmega| over its spheres.
-__device__ __forceinline__ void track_update(ListTrack lt, int64_t c, int64_t cap, d3 rr,
-                                             double pos, double rot_step) {
-  const double rot = lt.vrot[c] + rot_step;
-  lt.vrot[c] = rot;
-  const double dx = rr.x - lt.xb[c], dy = rr.y - lt.xb[cap + c], dz = rr.z - lt.xb[2 * cap + c];
-  const double d = sqrt(dx * dx + dy * dy + dz * dz) + 1.7320508075688772 * pos * rot;
-  if (d > lt.dsp[c]) lt.dsp[c] = d;
+// Candidate-list displacement bound of clump c since the list build, accumulated step by
+// step: sum of |x' - x| + sqrt(3) pos |dOmega| (a sphere sits sqrt(3) pos from the centre,
+// and a rotation by |dOmega| moves it at most that times as far), which is never below
+// the distance any sphere has actually moved.  Frozen clumps do not move.
+__device__ __forceinline__ void track_update(ListTrack lt, int64_t c, d3 r0, d3 r1, double pos,
+                                             double rot_step) {
+  const double dx = r1.x - r0.x, dy = r1.y - r0.y, dz = r1.z - r0.z;
+  const double d = sqrt(dx * dx + dy * dy + dz * dz) + 1.7320508075688772 * pos * rot_step;
+  if (d > 0.0) lt.dsp[c] += d;
 }
 
 // mode 1: predict (init_Verlet) then stage and sphere-cell keys; mode 2: the same predict
@@ -181,17 +183,18 @@ __device__ __forceinline__ void predict_clump(double *__restrict__ st, double *_
 #pragma unroll
   for (int k = 0; k < 9; k++) R[k] = st[(S_R00 + k) * cap + c];
   double rad = sta[C_RAD * cap + c];
-  const double rad0 = sta[C_RAD0 * cap + c];
+  const double rad0 = rad != 0.0 ? rad : sta[C_RAD0 * cap + c];  // radius before freezing
   d3 vv = make_d3(st[S_VX * cap + c], st[S_VY * cap + c], st[S_VZ * cap + c]);
   d3 ww = make_d3(st[S_WX * cap + c], st[S_WY * cap + c], st[S_WZ * cap + c]);
   double rot_step = 0.0;  // |dOmega| of this step
+  const d3 r0 = rr;
+  const d3 aa = make_d3(st[S_AX * cap + c], st[S_AY * cap + c], st[S_AZ * cap + c]);
 
   if (mode) {
     if (frozen(t_old, rr.x, p.side_x)) {
       rad = 0.0;
       sta[C_RAD * cap + c] = 0.0;
     } else {
-      const d3 aa = make_d3(st[S_AX * cap + c], st[S_AY * cap + c], st[S_AZ * cap + c]);
       const d3 ang = make_d3(st[S_GX * cap + c], st[S_GY * cap + c], st[S_GZ * cap + c]);
       rot_step = verlet_advance(rr, vv, ww, R, aa, ang, p.dt, p.hdt);
       st[S_RX * cap + c] = rr.x;
@@ -208,17 +211,17 @@ __device__ __forceinline__ void predict_clump(double *__restrict__ st, double *_
     }
   }
 
+  fill_stage(g, rr, rad, vv, sta[C_RHO * cap + c], ww, (uint32_t)orig[c], R, aa);
+
+  if (mode == 2) {
+    track_update(lt, c, r0, rr, rad0 / p.sphere_div, rot_step);
+    return;
+  }
+
   // Sphere centres always use the radius the clump had before it froze: a frozen clump's
   // centres are the stale ones of the step it crossed side.x (rr, R unchanged since).
   d3 sph[4];
   sphere_centres(rr, R, rad0 / p.sphere_div, sph);
-  fill_stage(g, rr, rad, vv, sta[C_RHO * cap + c], ww, (uint32_t)orig[c], sph);
-
-  if (mode == 2) {
-    track_update(lt, c, cap, rr, rad0 / p.sphere_div, rot_step);
-    return;
-  }
-
   // Binning at the force-stage time: clumps past side.x are left out (cells.cu:65).
   const bool out = frozen(t_new, rr.x, p.side_x);
 #pragma unroll
@@ -233,35 +236,36 @@ __device__ __forceinline__ void predict_clump(double *__restrict__ st, double *_
   }
 }
 
-// A clump's finished state kept in registers from the force stage to the next step's
-// predict inside a chunk of candidate-list steps.
+// A clump's state in registers: its stage record as read at the start of a list step
+// (rr, vv, ww, R, aa = a_i; rad, rho, id), or its finished state handed from the force
+// stage to the next step's predict inside a chunk of candidate-list steps (aa, ang = the
+// new accelerations).
 struct ClumpRegs {
   d3 rr, vv, ww, aa, ang;
   double R[9];
+  double rad, rho;
+  uint32_t id;
   bool frz;  // frozen at the force stage's time: no finish, no accelerations
 };
 
-// predict_clump mode 2 on a finished state in registers.  Only R goes to st: inside a
-// chunk the list steps read a clump's own centre and velocities from its stage record,
-// and the chunk's last step writes them back to st.
-__device__ __forceinline__ void predict_regs(ClumpRegs &s, double *__restrict__ st,
-                                             double *__restrict__ sta, double *__restrict__ g,
-                                             const InstParams &p, uint32_t id, double rad,
-                                             double rad0, double rho, int64_t c, int64_t cap,
-                                             ListTrack lt, double t_old) {
+// predict_clump mode 2 on a finished state in registers, writing only the stage record:
+// inside a lean chunk a clump's centre, velocities, rotation and acceleration live in its
+// record (the angular acceleration in st), and the chunk's last step writes them back to st.
+__device__ __forceinline__ void predict_regs(ClumpRegs &s, double *__restrict__ sta,
+                                             double *__restrict__ g, const InstParams &p,
+                                             uint32_t id, double rad, double rad0, double rho,
+                                             int64_t c, int64_t cap, ListTrack lt,
+                                             double t_old) {
   double rot_step = 0.0;
+  const d3 r0 = s.rr;
   if (frozen(t_old, s.rr.x, p.side_x)) {
     rad = 0.0;
     sta[C_RAD * cap + c] = 0.0;
   } else {
     rot_step = verlet_advance(s.rr, s.vv, s.ww, s.R, s.aa, s.ang, p.dt, p.hdt);
-#pragma unroll
-    for (int k = 0; k < 9; k++) st[(S_R00 + k) * cap + c] = s.R[k];
   }
-  d3 sph[4];
-  sphere_centres(s.rr, s.R, rad0 / p.sphere_div, sph);
-  fill_stage(g, s.rr, rad, s.vv, rho, s.ww, id, sph);
-  track_update(lt, c, cap, s.rr, rad0 / p.sphere_div, rot_step);
+  fill_stage(g, s.rr, rad, s.vv, rho, s.ww, id, s.R, s.aa);
+  track_update(lt, c, r0, s.rr, rad0 / p.sphere_div, rot_step);
 }
 
 // The stage records of a workgroup's 256 clumps are assembled in LDS (rows padded to 25
@@ -397,8 +401,8 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_c(uint32_t *__restrict__ co
 __global__ void k_scatter(const uint32_t *__restrict__ skey, const uint32_t *__restrict__ srank,
                           const uint32_t *__restrict__ starts, const double *__restrict__ stage,
                           double4 *__restrict__ spos, uint32_t *__restrict__ sid,
-                          InstParams *__restrict__ P, int32_t ninst, int64_t nsph,
-                          int advance) {
+                          const int32_t *__restrict__ inst_of, InstParams *__restrict__ P,
+                          int32_t ninst, int64_t nsph, int advance) {
   int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (advance && i < ninst) {  // ninst <= clumps < spheres, so every instance is covered
     InstParams &p = P[i];
@@ -409,9 +413,9 @@ __global__ void k_scatter(const uint32_t *__restrict__ skey, const uint32_t *__r
   if (key == kInvalidKey) return;
   uint32_t dst = starts[key] + srank[i];
   const double *g = stage + (i >> 2) * kStageDoubles;
-  const int s = (int)(i & 3);
-  spos[dst] = make_double4(g[G_SPH + 3 * s], g[G_SPH + 3 * s + 1], g[G_SPH + 3 * s + 2],
-                           g[G_RAD]);
+  const double rad = g[G_RAD];  // binned: not frozen, the pre-freeze radius
+  const d3 q = rec_sphere(g, rad / P[inst_of[i >> 2]].sphere_div, (int)(i & 3));
+  spos[dst] = make_double4(q.x, q.y, q.z, rad);
   sid[dst] = (uint32_t)i;
 }
 
@@ -703,7 +707,7 @@ __global__ __launch_bounds__(256) void k_detect(
     return;
   }
   const double rad = g[G_RAD];
-  const d3 q = make_d3(g[G_SPH + 3 * s], g[G_SPH + 3 * s + 1], g[G_SPH + 3 * s + 2]);
+  const d3 q = rec_sphere(g, rad / p.sphere_div, s);
   const int nx = p.sg_n[0], ny = p.sg_n[1], nz = p.sg_n[2];
   const int ix = grid_index(q.x, p.sg_lo[0], p.sg_inv_h, nx);
   const int iy = grid_index(q.y, p.sg_lo[1], p.sg_inv_h, ny);
@@ -767,10 +771,6 @@ __global__ __launch_bounds__(BS) void k_vmerge(
   const InstParams &p = P[inst_of[c]];
   const double *gl = stage + c * kStageDoubles;
   const d3 rr = make_d3(gl[G_RR], gl[G_RR + 1], gl[G_RR + 2]);
-  lt.xb[c] = rr.x;
-  lt.xb[cap + c] = rr.y;
-  lt.xb[2 * cap + c] = rr.z;
-  lt.vrot[c] = 0.0;
   lt.dsp[c] = 0.0;
   int64_t li = (int64_t)((rr.x - p.rr_min[0]) / p.cell_side[0]);
   if (li == p.ncell[0]) li--;
@@ -950,9 +950,11 @@ __device__ __forceinline__ void force_clump(
   d3 ww = kOwnStage ? own.ww : make_d3(st[S_WX * cap + c], st[S_WY * cap + c], st[S_WZ * cap + c]);
   double R[9];
 #pragma unroll
-  for (int k = 0; k < 9; k++) R[k] = st[(S_R00 + k) * cap + c];
-  const double rad = sta[C_RAD * cap + c], rad0 = sta[C_RAD0 * cap + c];
-  const double rho = sta[C_RHO * cap + c];
+  for (int k = 0; k < 9; k++) R[k] = kOwnStage ? own.R[k] : st[(S_R00 + k) * cap + c];
+  const double rad = kOwnStage ? own.rad : sta[C_RAD * cap + c];
+  // the radius before freezing: the radius itself until the clump freezes
+  const double rad0 = rad != 0.0 ? rad : sta[C_RAD0 * cap + c];
+  const double rho = kOwnStage ? own.rho : sta[C_RHO * cap + c];
   const double pos0 = rad0 / p.sphere_div;
   // bound on |sphere centre - rr| (sqrt(3) pos0; R has unit columns) with a margin far above
   // the rounding of the centres, for the boundary modules' early outs
@@ -1003,11 +1005,11 @@ __device__ __forceinline__ void force_clump(
         const double *g = stage + cm * kStageDoubles;
         const d3 rm = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
         const double radm = g[G_RAD], idm = g[G_ID];
-        const d3 sm0 = make_d3(g[G_SPH + 0], g[G_SPH + 1], g[G_SPH + 2]);
-        const d3 sm1 = make_d3(g[G_SPH + 3], g[G_SPH + 4], g[G_SPH + 5]);
-        const d3 sm2 = make_d3(g[G_SPH + 6], g[G_SPH + 7], g[G_SPH + 8]);
-        const d3 sm3 = make_d3(g[G_SPH + 9], g[G_SPH + 10], g[G_SPH + 11]);
+        double Rm[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) Rm[k] = g[G_R + k];
         if (frozen(time, rm.x, p.side_x)) continue;  // not binned at this time (cells.cu:65)
+        const double posm = radm / p.sphere_div;
         // the candidate pairings that touch now: k_detect's test on the same operands
         const double sep = rad + radm, sep2 = sep * sep;
         uint32_t mask = 0u;
@@ -1016,7 +1018,7 @@ __device__ __forceinline__ void force_clump(
           bits &= bits - 1u;
           const int sdiv = pr >> 2, smod = pr & 3;
           const d3 q = sphere_of(rr, R, pos0, smod);
-          const d3 o = sel4(sdiv, sm0, sm1, sm2, sm3);
+          const d3 o = sphere_of(rm, Rm, posm, sdiv);
           const double dx = o.x - q.x, dy = o.y - q.y, dz = o.z - q.z;
           if (dx * dx + dy * dy + dz * dz < sep2) mask |= 1u << pr;
         }
@@ -1146,7 +1148,7 @@ __device__ __forceinline__ void force_clump(
         const double rad_2 = g[G_RAD], rhom = g[G_RHO];
         const d3 vvm = make_d3(g[G_VV], g[G_VV + 1], g[G_VV + 2]);
         const d3 wwm = make_d3(g[G_WW], g[G_WW + 1], g[G_WW + 2]);
-        const d3 r2 = make_d3(g[G_SPH + 3 * sdiv], g[G_SPH + 3 * sdiv + 1], g[G_SPH + 3 * sdiv + 2]);
+        const d3 r2 = rec_sphere(g, rad_2 / p.sphere_div, sdiv);
         // force_part.cu:119-242
         const double massm = kSphFact * rhom * rad_2 * rad_2 * rad_2;
         const double meff = 0.5 * (massm + massl);
@@ -1192,17 +1194,23 @@ __device__ __forceinline__ void force_clump(
       out.rr = rr;
       out.vv = vv;
       out.ww = ww;
+      out.aa = own.aa;  // no accelerations while frozen: a_i stays the last one computed
 #pragma unroll
       for (int k = 0; k < 9; k++) out.R[k] = R[k];
       out.frz = true;
     } else {
-      if constexpr (kWriteRR) {
+      if constexpr (kWriteRR) {  // the chunk's last step: the record's state back to st
         st[S_RX * cap + c] = rr.x;
         st[S_RY * cap + c] = rr.y;
         st[S_RZ * cap + c] = rr.z;
         st[S_WX * cap + c] = ww.x;
         st[S_WY * cap + c] = ww.y;
         st[S_WZ * cap + c] = ww.z;
+#pragma unroll
+        for (int k = 0; k < 9; k++) st[(S_R00 + k) * cap + c] = R[k];
+        st[S_AX * cap + c] = own.aa.x;
+        st[S_AY * cap + c] = own.aa.y;
+        st[S_AZ * cap + c] = own.aa.z;
       }
       if (!initial) {
         st[S_VX * cap + c] = 0.0;
@@ -1237,7 +1245,8 @@ __device__ __forceinline__ void force_clump(
 
   if (!initial) {  // finish_Verlet.cu:43-52
     const double hdt = p.hdt;
-    const d3 ai = make_d3(st[S_AX * cap + c], st[S_AY * cap + c], st[S_AZ * cap + c]);
+    const d3 ai = kOwnStage ? own.aa
+                            : make_d3(st[S_AX * cap + c], st[S_AY * cap + c], st[S_AZ * cap + c]);
     const d3 gi = make_d3(st[S_GX * cap + c], st[S_GY * cap + c], st[S_GZ * cap + c]);
     vv.x += hdt * (af.x - ai.x);
     vv.y += hdt * (af.y - ai.y);
@@ -1250,6 +1259,8 @@ __device__ __forceinline__ void force_clump(
         st[S_RX * cap + c] = rr.x;
         st[S_RY * cap + c] = rr.y;
         st[S_RZ * cap + c] = rr.z;
+#pragma unroll
+        for (int k = 0; k < 9; k++) st[(S_R00 + k) * cap + c] = R[k];
       }
       st[S_VX * cap + c] = vv.x;
       st[S_VY * cap + c] = vv.y;
@@ -1269,10 +1280,13 @@ __device__ __forceinline__ void force_clump(
     for (int k = 0; k < 9; k++) out.R[k] = R[k];
     out.frz = false;
   }
-  // the accelerations become a_i of the next step (avalanche_hst.cu:844-850)
-  st[S_AX * cap + c] = af.x;
-  st[S_AY * cap + c] = af.y;
-  st[S_AZ * cap + c] = af.z;
+  // the accelerations become a_i of the next step (avalanche_hst.cu:844-850); inside a
+  // lean chunk a_i travels in the stage record
+  if constexpr (!kRegsOut) {
+    st[S_AX * cap + c] = af.x;
+    st[S_AY * cap + c] = af.y;
+    st[S_AZ * cap + c] = af.z;
+  }
   st[S_GX * cap + c] = gf.x;
   st[S_GY * cap + c] = gf.y;
   st[S_GZ * cap + c] = gf.z;
@@ -1314,12 +1328,26 @@ __global__ __launch_bounds__(BS) void k_force(
   ClumpRegs cr, own;
   constexpr bool kOwnStage = VERLET && (LEAN || !NEXT);
   constexpr bool kRegsOut = NEXT && LEAN;
-  if (kOwnStage && c < n) {  // the own record's first 12 doubles in 16-byte loads
+  if (kOwnStage && c < n) {  // the own record, whole, in 16-byte loads
     const double2 *o = reinterpret_cast<const double2 *>(stage + c * kStageDoubles);
     const double2 a = o[0], b = o[1], v0 = o[2], v1 = o[3], w0 = o[4], w1 = o[5];
+    const double2 r0 = o[6], r1 = o[7], r2 = o[8], r3 = o[9], r4 = o[10], r5 = o[11];
     own.rr = make_d3(a.x, a.y, b.x);
+    own.rad = b.y;
     own.vv = make_d3(v0.x, v0.y, v1.x);
+    own.rho = v1.y;
     own.ww = make_d3(w0.x, w0.y, w1.x);
+    own.id = (uint32_t)__double_as_longlong(w1.y);
+    own.R[0] = r0.x;
+    own.R[1] = r0.y;
+    own.R[2] = r1.x;
+    own.R[3] = r1.y;
+    own.R[4] = r2.x;
+    own.R[5] = r2.y;
+    own.R[6] = r3.x;
+    own.R[7] = r3.y;
+    own.R[8] = r4.x;
+    own.aa = make_d3(r4.y, r5.x, r5.y);
   }
   if (c < n) {
     while (true) {
@@ -1365,9 +1393,9 @@ __global__ __launch_bounds__(BS) void k_force(
         if (inst == cur) {
           const InstParams &q = P[cur];  // the next step starts at this step's time + dt
           if constexpr (LEAN)
-            predict_regs(cr, st, sta, rec + lane * kStagePad, q, (uint32_t)orig[c],
-                         sta[C_RAD * cap + c], sta[C_RAD0 * cap + c], sta[C_RHO * cap + c], c,
-                         cap, lt, chunk_time(q, si) + q.dt);
+            predict_regs(cr, sta, rec + lane * kStagePad, q, own.id, own.rad,
+                         own.rad != 0.0 ? own.rad : sta[C_RAD0 * cap + c], own.rho, c, cap, lt,
+                         chunk_time(q, si) + q.dt);
           else
             predict_clump(st, sta, rec + lane * kStagePad, q, orig, nullptr, nullptr, nullptr, c,
                           cap, 2, lt, chunk_time(q, si) + q.dt);
@@ -1744,7 +1772,7 @@ __global__ void k_export_contacts(const double *__restrict__ st, const double *_
   const double massl = kSphFact * rho * rad * rad * rad;
   const int nx = p.sg_n[0], ny = p.sg_n[1], nz = p.sg_n[2];
   for (int s = 0; s < 4; s++) {
-    const d3 q = make_d3(gl[G_SPH + 3 * s], gl[G_SPH + 3 * s + 1], gl[G_SPH + 3 * s + 2]);
+    const d3 q = rec_sphere(gl, rad / p.sphere_div, s);
     const int ix = grid_index(q.x, p.sg_lo[0], p.sg_inv_h, nx);
     const int iy = grid_index(q.y, p.sg_lo[1], p.sg_inv_h, ny);
     const int iz = grid_index(q.z, p.sg_lo[2], p.sg_inv_h, nz);

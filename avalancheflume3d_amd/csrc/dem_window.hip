@@ -114,7 +114,15 @@ __global__ __launch_bounds__(64) void k_win_list(
     atomicAdd(&wd->frozen, 1ull);
   if (!(sta[C_IX * cap + c] == sta[C_IY * cap + c] && sta[C_IY * cap + c] == sta[C_IZ * cap + c]))
     atomicAdd(&wd->aniso, 1ull);
-  for (int k = 0; k < 12; k++) sph0[c * 12 + k] = gl[G_SPH + k];
+  double R0[9];
+  for (int k = 0; k < 9; k++) R0[k] = gl[G_R + k];
+  d3 sph[4];
+  sphere_centres(rr, R0, sta[C_RAD0 * cap + c] / p.sphere_div, sph);
+  for (int s = 0; s < 4; s++) {
+    sph0[c * 12 + 3 * s] = sph[s].x;
+    sph0[c * 12 + 3 * s + 1] = sph[s].y;
+    sph0[c * 12 + 3 * s + 2] = sph[s].z;
+  }
   dmax2[c] = 0.0;
   int cnt = 0;
   bool overflow = false;
@@ -128,7 +136,7 @@ __global__ __launch_bounds__(64) void k_win_list(
   // sphere cells (side >= 2 r_max) must cover sep + skin: widen the sweep by the skin
   const int reach = 1 + (int)ceil(skin * p.sg_inv_h);
   for (int s = 0; s < 4; s++) {
-    const d3 q = make_d3(gl[G_SPH + 3 * s], gl[G_SPH + 3 * s + 1], gl[G_SPH + 3 * s + 2]);
+    const d3 q = sph[s];
     const int ix = grid_index(q.x, p.sg_lo[0], p.sg_inv_h, nx);
     const int iy = grid_index(q.y, p.sg_lo[1], p.sg_inv_h, ny);
     const int iz = grid_index(q.z, p.sg_lo[2], p.sg_inv_h, nz);
@@ -255,7 +263,7 @@ __global__ void k_win_missed(const double *__restrict__ stage, const InstParams 
   const InstParams &p = P[inst_of[c]];
   const double *gl = stage + c * kStageDoubles;
   const double rad = gl[G_RAD];
-  const d3 q = make_d3(gl[G_SPH + 3 * s], gl[G_SPH + 3 * s + 1], gl[G_SPH + 3 * s + 2]);
+  const d3 q = rec_sphere(gl, rad / p.sphere_div, s);
   const int nx = p.sg_n[0], ny = p.sg_n[1], nz = p.sg_n[2];
   const int ix = grid_index(q.x, p.sg_lo[0], p.sg_inv_h, nx);
   const int iy = grid_index(q.y, p.sg_lo[1], p.sg_inv_h, ny);

@@ -21,14 +21,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "clump-steps/s + HBM GB/s at 250k clumps; batched vs plain; adjoint/forward cost"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per clump and launch of k_force on a candidate-list step (which also
-# predicts the next step): reads the carried state (rr, v, w, R, a, alpha: 24 doubles,
-# 192 B), the clump constants (radius, pre-freeze radius, density, mass, I: 56 B), instance
-# and reference index (8 B), the list length (4 B) and the displacement tracking (build
-# centre, rotation sum, bound: 40 B); writes the carried state (192 B), the next stage
-# record (192 B) and the tracking (16 B).  The list entries (6 B each) and the neighbours'
-# stage records (L2 hits) are not counted.
-FORCE_BYTES_PER_CLUMP = 700.0
+# Algorithmic bytes per clump and launch of k_force on a candidate-list step of a lean chunk
+# (which also predicts the next step): reads the clump's stage record (rr, radius, v,
+# density, w, reference index, R, a: 192 B), the angular acceleration (24 B), mass and
+# inertia (32 B), the instance index and list length (8 B) and the displacement bound
+# (8 B); writes the next stage record (192 B), the angular acceleration (24 B) and the
+# bound (8 B).  The list entries (6 B each) and the neighbours' stage records (L2 hits) are
+# not counted.
+FORCE_BYTES_PER_CLUMP = 488.0
 # Algorithmic bytes per clump-step of the whole step (480 B per clump-step).
 STEP_BYTES_PER_CLUMP = 480.0
 # Candidate-list skin (cm) of the headline runs: 5 % of the sphere radius.

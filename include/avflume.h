@@ -139,11 +139,12 @@ typedef struct avf_options {
   int32_t block_clumps;     /* lanes per block of the force kernel: 64 or 128          */
   int32_t max_neighbours;   /* informational: touching-clump list capacity (32)        */
   int32_t device;           /* HIP device ordinal                                      */
-  int32_t lean_state;       /* candidate-list chunks: 1 = a clump's centre and velocities */
-                            /* live only in its stage record inside a chunk (less HBM   */
-                            /* traffic), -1 = also in the state arrays every step (fewer */
-                            /* strided reads, for working sets the last-level cache   */
-                            /* holds), 0 = chosen by working-set size; same results     */
+  int32_t lean_state;       /* candidate-list chunks: 0 or 1 = inside a chunk a clump's */
+                            /* state (centre, velocities, rotation, acceleration) lives */
+                            /* only in its stage record, the angular acceleration in   */
+                            /* the state arrays (one read and one write of the state  */
+                            /* per step); -1 = the whole state also goes to the state  */
+                            /* arrays every step; same results                         */
   void *stream;             /* hipStream_t to run on; NULL = library-owned stream      */
   double verlet_skin;       /* > 0: candidate lists of sphere pairs within sep + skin, */
                             /* rebuilt only when some sphere has moved skin / 2 since  */
