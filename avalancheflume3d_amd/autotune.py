@@ -16,7 +16,7 @@ from . import dem
 
 SKINS = (0.0, 0.005, 0.01, 0.02)   # cm; 0: sphere cells rebuilt every step
 FACTORS = (1.0, 1.25)
-REORDERS = (50, 100, 200)
+REORDERS = (50, 200, 400)
 BLOCKS = (64, 128)
 
 

@@ -11,8 +11,8 @@
 //
 //   avalanche_amd [--restart] [--restart-bin FILE] [--max-steps N] [--final-bin FILE]
 //                 [--device D] [--skin CM] [--reorder N] < DataInput.dat
-// --skin: candidate-list skin (default 0.01 cm; 0 bins the spheres every step), --reorder:
-// spatial reorder interval (default 200 steps).  Neither changes any result.
+// --skin: candidate-list skin (default 0.02 cm; 0 bins the spheres every step), --reorder:
+// spatial reorder interval (default 400 steps).  Neither changes any result.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -188,8 +188,8 @@ int main(int argc, char **argv) {
   const char *final_bin = nullptr;
   avf_options opt;
   avf_options_default(&opt);
-  opt.verlet_skin = 0.01;
-  opt.reorder_interval = 200;
+  opt.verlet_skin = 0.02;
+  opt.reorder_interval = 400;
   for (int i = 1; i < argc; i++) {
     if (!std::strcmp(argv[i], "--restart")) restart = true;
     else if (!std::strcmp(argv[i], "--restart-bin") && i + 1 < argc) restart_bin = argv[++i];

@@ -3,7 +3,7 @@
     python bench.py [--steps K] [--warmup W] [--workload headline|small|large]
 
 Times K steps of the headline configuration (250,000 tetrahedral 4-sphere clumps, FP64,
-candidate lists with a 0.01 cm skin, spatial reorder every 200 steps) with inputs resident
+candidate lists with a 0.02 cm skin, spatial reorder every 400 steps) with inputs resident
 in HBM, and prints one JSON line: clump-steps/s, the roofline of the dominant kernel
 (k_force, from HIP events over the timed region) and the CPU baseline (the oracle on one
 host core, bounded sample of the same workload).
@@ -31,14 +31,15 @@ HBM_PEAK_GBS = 8000.0
 FORCE_BYTES_PER_CLUMP = 488.0
 # Algorithmic bytes per clump-step of the whole step (480 B per clump-step).
 STEP_BYTES_PER_CLUMP = 480.0
-# Candidate-list skin (cm) of the headline runs: 5 % of the sphere radius.
-DEFAULT_SKIN = 0.01
+# Candidate-list skin (cm) of the headline runs: 10 % of the sphere radius (a wider skin lists
+# more candidates; a narrower one ends chunks and rebuilds the lists sooner).
+DEFAULT_SKIN = 0.02
 # The timed region brackets k_force with HIP events on every TIMED_EVERY-th step (a regular
 # sample of its launches over the whole region; events on every step would add their own
 # pipeline stalls to the timed run).
 TIMED_EVERY = 4
 # Spatial reorder interval (steps) of the headline runs; a reorder also rebuilds the lists.
-DEFAULT_REORDER = 200
+DEFAULT_REORDER = 400
 
 
 def cpu_baseline(inst, budget_s: float) -> dict:
@@ -61,7 +62,7 @@ def cpu_baseline(inst, budget_s: float) -> dict:
 
 
 def batch_compare(steps: int, warmup: int = 20, skin: float = DEFAULT_SKIN,
-                  sequential: bool = True) -> dict:
+                  sequential: bool = True, reorder: int = DEFAULT_REORDER) -> dict:
     """64 independent flumes (N ~ log-uniform[5e3, 6e4], per-instance seeds, mu, eps and
     slope) stepped together in one context vs the same instances as sequential plain runs
     (one context each, synchronised between instances)."""
@@ -70,7 +71,7 @@ def batch_compare(steps: int, warmup: int = 20, skin: float = DEFAULT_SKIN,
     total = sum(len(i.particles) for i in insts)
     opt = dem.default_options()
     opt.verlet_skin = skin
-    opt.reorder_interval = DEFAULT_REORDER
+    opt.reorder_interval = reorder
     ctx = dem.Context(insts, opt)
     ctx.regrid()
     ctx.initial_forces()
@@ -266,7 +267,10 @@ def main() -> int:
     from avalancheflume3d_amd import dem, workloads
 
     if args.batch_only:
-        print(json.dumps({"batched": batch_compare(args.batch_steps, sequential=False)}),
+        print(json.dumps({"batched": batch_compare(
+            args.batch_steps, sequential=False,
+            skin=DEFAULT_SKIN if args.skin is None else args.skin,
+            reorder=DEFAULT_REORDER if args.reorder is None else args.reorder)}),
               flush=True)
         return 0
 
@@ -356,7 +360,8 @@ def main() -> int:
         "diag": diag,
     }
     if args.batch_steps > 0:
-        out["batched"] = batch_compare(args.batch_steps, skin=opt.verlet_skin)
+        out["batched"] = batch_compare(args.batch_steps, skin=opt.verlet_skin,
+                                       reorder=opt.reorder_interval)
     if args.adjoint_steps > 0:
         out["adjoint"] = adjoint_bench(args.adjoint_steps)
     if not args.no_cpu:
