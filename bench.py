@@ -61,6 +61,35 @@ def cpu_baseline(inst, budget_s: float) -> dict:
                       f"({dt:.1f} s on one host core)"}
 
 
+def cpu_batch_baseline(budget_s: float, cores: int) -> dict:
+    """The oracle on independent instances of the batched ensemble, one per host thread (one
+    core each), every thread stepping its instance for about budget_s seconds after its
+    initial force pass: the all-cores CPU figure of the batch."""
+    from concurrent.futures import ThreadPoolExecutor
+    from avalancheflume3d_amd import workloads
+    from oracle import oracle
+    insts = workloads.batch(64)[:cores]
+
+    def run(inst):
+        o = oracle.OracleSim(inst.sys, inst.geo, inst.fun, inst.ntags, inst.phase, inst.particles)
+        o.initial()
+        t0, k = time.perf_counter(), 0
+        while True:
+            o.step(1)
+            k += 1
+            if time.perf_counter() - t0 >= budget_s:
+                break
+        return len(inst.particles) * k, time.perf_counter() - t0
+
+    with ThreadPoolExecutor(max_workers=len(insts)) as pool:  # ctypes releases the GIL
+        res = list(pool.map(run, insts))
+    clumps = sum(len(i.particles) for i in insts)
+    return {"value": sum(w / t for w, t in res), "unit": "clump-steps/s", "cores": len(insts),
+            "kind": "port",
+            "sample": f"oracle, the first {len(insts)} of the 64 batch instances ({clumps} clumps), "
+                      f"one per host thread, about {budget_s:.0f} s of steps each"}
+
+
 def batch_compare(steps: int, warmup: int = 20, skin: float = DEFAULT_SKIN,
                   sequential: bool = True, reorder: int = DEFAULT_REORDER) -> dict:
     """64 independent flumes (N ~ log-uniform[5e3, 6e4], per-instance seeds, mu, eps and
@@ -367,6 +396,11 @@ def main() -> int:
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(inst, args.cpu_seconds)
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+        if "batched" in out:  # the batch against the oracle on all cores (16: one GPU's share)
+            cb = cpu_batch_baseline(0.5 * args.cpu_seconds, min(16, os.cpu_count() or 1))
+            out["batched"]["cpu_all_cores"] = cb
+            out["batched"]["speedup_vs_cpu_all_cores"] = (
+                out["batched"]["batched_clump_steps_per_s"] / cb["value"])
     print(json.dumps(out), flush=True)
     return 0
 
