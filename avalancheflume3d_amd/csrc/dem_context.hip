@@ -62,6 +62,7 @@ struct avf_ctx {
   int32_t *d_vl_m = nullptr, *d_vl_cnt = nullptr;
   uint16_t *d_vl_mask = nullptr;
   double *d_dsp = nullptr;
+  uint8_t *d_wreach = nullptr;
   double *d_ck_st = nullptr, *d_ck_rad = nullptr;
   InstParams *d_ck_P = nullptr;
   DevDiag *d_ck_diag = nullptr;
@@ -199,14 +200,14 @@ void launch_scan(avf_ctx *c) {
 void launch_bin(avf_ctx *c, int mode) {
   k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
       c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-      c->d_srank, c->n, c->cap, mode, ListTrack{nullptr});
+      c->d_srank, c->n, c->cap, mode, ListTrack{nullptr, nullptr});
   launch_scan(c);
   k_scatter<<<std::max(1u, grid_for(4 * c->n, kBlock)), kBlock, 0, c->stream>>>(
       c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid, c->d_inst, c->d_P, c->ninst,
       4 * c->n, mode);
 }
 
-ListTrack list_track(avf_ctx *c) { return ListTrack{c->d_dsp}; }
+ListTrack list_track(avf_ctx *c) { return ListTrack{c->d_dsp, c->d_wreach}; }
 
 template <int BS, bool VERLET, bool NEXT, bool LEAN = false>
 void launch_force_bs(avf_ctx *c, int initial, bool ft, int time_ahead, const double *stage_in,
@@ -214,7 +215,7 @@ void launch_force_bs(avf_ctx *c, int initial, bool ft, int time_ahead, const dou
   k_force<kMaxNeigh, BS, VERLET, NEXT, LEAN><<<grid_for(c->n, BS), BS, 0, c->stream>>>(
       c->d_st, c->d_sta, stage_in, c->d_inst, c->d_P, c->d_clist, c->d_ccnt, c->d_vl_m,
       c->d_vl_mask, c->d_vl_cnt, ft ? c->d_f : nullptr, c->d_diag, c->d_acc_cell, stage_out,
-      c->d_orig, NEXT ? list_track(c) : ListTrack{nullptr}, c->n, c->cap,
+      c->d_orig, VERLET ? list_track(c) : ListTrack{nullptr, nullptr}, c->n, c->cap,
       initial, time_ahead, si);
 }
 
@@ -277,7 +278,7 @@ void launch_step(avf_ctx *c, hipEvent_t *ev = nullptr, bool ft = true) {
   mark(AVF_K_PREDICT);
   k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
       c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-      c->d_srank, c->n, c->cap, 1, ListTrack{nullptr});
+      c->d_srank, c->n, c->cap, 1, ListTrack{nullptr, nullptr});
   mark(AVF_K_SCAN_A);
   k_scan_a<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
   mark(AVF_K_SCAN_B);
@@ -337,7 +338,7 @@ void launch_vstep(avf_ctx *c, bool rebuild, hipEvent_t *ev = nullptr, bool ft = 
     launch_detect(c, c->opt.verlet_skin);
     k_vmerge<kForceBlock><<<grid_for(c->n, kForceBlock), kForceBlock, 0, c->stream>>>(
         c->d_stage, c->d_P, c->d_inst, c->d_clist, c->d_ccnt, c->d_vl_m, c->d_vl_mask,
-        c->d_vl_cnt, list_track(c), c->d_diag, c->n, c->cap);
+        c->d_vl_cnt, list_track(c), c->d_diag, c->n, c->cap, c->opt.verlet_skin);
   }
   mark(AVF_K_FORCE);
   if (rebuild) {
@@ -687,7 +688,7 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
     return bail(rc);
   if (c->opt.verlet_skin > 0.0 &&
       ((rc = dalloc(c, &c->d_vl_m, kVMax * N)) || (rc = dalloc(c, &c->d_vl_mask, kVMax * N)) ||
-       (rc = dalloc(c, &c->d_vl_cnt, N)) || (rc = dalloc(c, &c->d_dsp, N)) ||
+       (rc = dalloc(c, &c->d_vl_cnt, N)) || (rc = dalloc(c, &c->d_dsp, N)) || (rc = dalloc(c, &c->d_wreach, N)) ||
        (rc = dalloc(c, &c->d_ck_st, N * S_NFIELDS)) || (rc = dalloc(c, &c->d_ck_rad, N)) ||
        (rc = dalloc(c, &c->d_ck_P, ninst)) || (rc = dalloc(c, &c->d_ck_diag, 1)) ||
        (rc = dalloc(c, &c->d_umax, 1)) || (rc = dalloc(c, &c->d_stage2, N * kStageDoubles))))
@@ -792,7 +793,7 @@ int avf_destroy(avf_ctx *c) {
                   c->d_inst2,  c->d_inv,   c->d_ncells, c->d_tmp64, c->d_counts, c->d_starts,
                   c->d_bsum,   c->d_skey,  c->d_srank, c->d_sid,   c->d_spos,  c->d_acc_cell,
                   c->d_acc_sg, c->d_diag,  c->d_count, c->d_clist, c->d_ccnt,
-                  c->d_vl_m,   c->d_vl_cnt, c->d_vl_mask, c->d_dsp,
+                  c->d_vl_m,   c->d_vl_cnt, c->d_vl_mask, c->d_dsp, c->d_wreach,
                   c->d_ck_st,  c->d_ck_rad, c->d_ck_P, c->d_ck_diag, c->d_umax,
                   c->d_stage2};
   for (void *b : bufs)
@@ -917,6 +918,7 @@ int avf_set_phase(avf_ctx *c, int32_t inst, int32_t phase, double time, double s
   HIPCHK(c, hipMemcpyAsync(c->d_P + inst, &p, sizeof p, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->desc[inst].phase = phase;
+  c->list_valid = false;  // a new phase, time or side.x: the next step rebuilds the lists
   return AVF_OK;
 }
 
@@ -1489,7 +1491,7 @@ void win_bin(avf_ctx *c, const double *st) {
   const unsigned nb = grid_for(c->cell_cap, kScanTile);
   k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
       const_cast<double *>(st), c->d_sta, w->bstage, c->d_inst, c->d_orig, c->d_P, c->d_counts,
-      w->bkey, w->brank, c->n, c->cap, 0, ListTrack{nullptr});
+      w->bkey, w->brank, c->n, c->cap, 0, ListTrack{nullptr, nullptr});
   k_scan_a<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
   k_scan_b<<<1, 1024, 0, c->stream>>>(c->d_bsum, c->d_ncells, w->bstarts);
   k_scan_c<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, w->bstarts, c->d_ncells);

@@ -67,10 +67,11 @@ __device__ __forceinline__ d3 rec_sphere(const double *__restrict__ g, double po
 
 // --------------------------------------------------------------------- k_predict ----
 
-// Displacement tracking of the candidate-list step: per clump, a bound on how far any of
-// its spheres has moved since the list build.
+// Per clump, kept with the candidate lists: a bound on how far any of its spheres has moved
+// since the list build, and the wall modules it can reach while the lists are valid.
 struct ListTrack {
   double *dsp;
+  uint8_t *wreach;
 };
 
 // init_Verlet.cu:46-114 on one clump in registers: position and velocity by the old
@@ -629,6 +630,42 @@ __device__ __forceinline__ void deposit_floor(const InstParams &p, d3 rr, d3 vv,
   }
 }
 
+// Wall modules a clump centred at rr can still reach while none of its spheres moves more
+// than `margin` (bit set = may touch): the early-out tests of the modules above, which are
+// 1-Lipschitz in the centre, with the sphere offset bound rb widened by the margin.  The
+// radius only ever drops (to 0 when the clump freezes), so each test takes whichever of ri
+// and 0 makes it harder to pass (the funnel-mouth tests depend on the sign of sin(phi)).
+// Time is not looked at (the hopper bottom counts as present).
+enum WallReach : uint32_t {
+  kReachHop = 1u, kReachFunnel = 2u, kReachCorners = 4u, kReachSide = 8u, kReachFlume = 16u,
+  kReachDep = 32u, kReachAll = 63u
+};
+__device__ __forceinline__ uint32_t wall_reach(const InstParams &p, d3 rr, double ri,
+                                               double rb) {
+  uint32_t w = 0u;
+  if (!(rr.z + rb < p.hop_height || rr.x - rb > p.hop_width) &&
+      (!(rr.x - rb > ri && p.hop_width - rr.x - rb > ri) ||
+       !(rr.y - rb - p.yhop_l > ri && p.yhop_r - rr.y - rb > ri) ||
+       !(rr.z - rb - p.hop_height > ri)))
+    w |= kReachHop;
+  const d3 qc = to_slope(rr, p.cost, p.sint);
+  if (!(qc.x - rb > p.lfunnel - (p.sinfi < 0.0 ? ri * p.sinfi : 0.0)) &&
+      !(fabs(qc.y - qc.x * p.tanfi) * p.cosfi - rb > ri &&
+        fabs(p.yfunnel - qc.y - qc.x * p.tanfi) * p.cosfi - rb > ri))
+    w |= kReachFunnel;
+  if (!(qc.x + rb < p.lfunnel - (p.sinfi > 0.0 ? ri * p.sinfi : 0.0) ||
+        qc.x - rb > p.lfunnel + ri))
+    w |= kReachCorners;
+  if (!(qc.x + rb < p.lfunnel) &&
+      !(rr.y - rb - p.dy2 > ri && p.side_y + p.dy2 - rr.y - rb > ri))
+    w |= kReachSide;
+  if (!(fabs(p.cost * (-(p.sint / p.cost) * rr.x - rr.z)) - rb > ri)) w |= kReachFlume;
+  const double tand = p.dep_sin / p.dep_cos;
+  if (!(p.dep_cos * (-(tand * (rr.x - p.dep_x0 - p.dep_x1) - (rr.z - p.dep_z0))) - rb > ri))
+    w |= kReachDep;
+  return w;
+}
+
 // ------------------------------------------------------------------- pp contacts ----
 
 // Reference clump-cell of a clump centre by the cell_locate arithmetic (multiply by
@@ -761,7 +798,7 @@ __global__ __launch_bounds__(BS) void k_vmerge(
     const int32_t *__restrict__ inst_of, const uint32_t *__restrict__ clist,
     const int32_t *__restrict__ ccnt, int32_t *__restrict__ vl_m, uint16_t *__restrict__ vl_mask,
     int32_t *__restrict__ vl_cnt, ListTrack lt, DevDiag *__restrict__ diag, int64_t n,
-    int64_t cap) {
+    int64_t cap, double skin) {
   __shared__ int32_t sm[kVMax][BS];
   __shared__ uint32_t sk[kVMax][BS];
   __shared__ uint16_t smask[kVMax][BS];
@@ -772,6 +809,14 @@ __global__ __launch_bounds__(BS) void k_vmerge(
   const double *gl = stage + c * kStageDoubles;
   const d3 rr = make_d3(gl[G_RR], gl[G_RR + 1], gl[G_RR + 2]);
   lt.dsp[c] = 0.0;
+  {  // the wall modules within reach while no sphere moves skin/2 (frozen clumps: all)
+    const double rad = gl[G_RAD];
+    lt.wreach[c] = (uint8_t)(rad > 0.0 ? wall_reach(p, rr, rad,
+                                                    1.7320508075688772 * (rad / p.sphere_div) *
+                                                            (1.0 + 1e-12) +
+                                                        1e-9 + skin)
+                                       : kReachAll);
+  }
   int64_t li = (int64_t)((rr.x - p.rr_min[0]) / p.cell_side[0]);
   if (li == p.ncell[0]) li--;
   int64_t lj = (int64_t)((rr.y - p.rr_min[1]) / p.cell_side[1]);
@@ -938,8 +983,9 @@ __device__ __forceinline__ void force_clump(
     double *__restrict__ st, const double *__restrict__ sta, const double *__restrict__ stage,
     const uint32_t *__restrict__ clist, const int32_t *__restrict__ ccnt,
     const int32_t *__restrict__ vl_m, const uint16_t *__restrict__ vl_mask,
-    const int32_t *__restrict__ vl_cnt, double *__restrict__ fout, DevDiag *__restrict__ diag,
-    int64_t cap, int initial, double time, const ClumpRegs &own, ClumpRegs &out) {
+    const int32_t *__restrict__ vl_cnt, const uint8_t *__restrict__ wreach,
+    double *__restrict__ fout, DevDiag *__restrict__ diag, int64_t cap, int initial,
+    double time, const ClumpRegs &own, ClumpRegs &out) {
   static_assert(!NEXT || VERLET, "the fused predict runs on candidate-list steps only");
   constexpr bool kOwnStage = VERLET && (LEAN || !NEXT);
   constexpr bool kRegsOut = NEXT && LEAN;
@@ -966,16 +1012,22 @@ __device__ __forceinline__ void force_clump(
   const bool aval = p.phase && !initial;
   const int mods = (p.phase ? p.modules_aval : p.modules_fill) &
                    (initial ? (AVF_MOD_PP | AVF_MOD_HOP) : ~0);
+  // on candidate-list steps, the wall modules this clump cannot reach before the lists are
+  // rebuilt are skipped (each would return at its own early-out test)
+  const uint32_t reach = VERLET ? (uint32_t)wreach[c] : (uint32_t)kReachAll;
   const double massw = kSphFact * rad * rad * rad * rho;  // force_hop.cu:45 order
   d3 ff = make_d3(0.0, 0.0, 0.0), tq = make_d3(0.0, 0.0, 0.0);
 
   if (aval) {  // avalanche_hst.cu:688-696
-    if (mods & AVF_MOD_HOP)
+    if ((mods & AVF_MOD_HOP) && (reach & kReachHop))
 #pragma unroll 1
       for (int nw = 0; nw < 3; nw++) hop_walls(p, nw, time, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
-    if (mods & AVF_MOD_FUNNEL) funnel_walls(p, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
-    if (mods & AVF_MOD_FCORNERS) funnel_corners(p, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
-    if ((mods & AVF_MOD_WALLS) && !frz) side_walls(p, time, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
+    if ((mods & AVF_MOD_FUNNEL) && (reach & kReachFunnel))
+      funnel_walls(p, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
+    if ((mods & AVF_MOD_FCORNERS) && (reach & kReachCorners))
+      funnel_corners(p, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
+    if ((mods & AVF_MOD_WALLS) && !frz && (reach & kReachSide))
+      side_walls(p, time, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
   }
 
   if ((mods & AVF_MOD_PP) && !frz) {
@@ -1171,12 +1223,14 @@ __device__ __forceinline__ void force_clump(
   }
 
   if (!aval) {  // avalanche_hst.cu:460-461 (pp first, then hop, while filling)
-    if (mods & AVF_MOD_HOP)
+    if ((mods & AVF_MOD_HOP) && (reach & kReachHop))
 #pragma unroll 1
       for (int nw = 0; nw < 3; nw++) hop_walls(p, nw, time, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
   } else {  // avalanche_hst.cu:704
-    if ((mods & AVF_MOD_FLUME) && !frz) flume_floor(p, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
-    if (mods & AVF_MOD_DEP) deposit_floor(p, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
+    if ((mods & AVF_MOD_FLUME) && !frz && (reach & kReachFlume))
+      flume_floor(p, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
+    if ((mods & AVF_MOD_DEP) && (reach & kReachDep))
+      deposit_floor(p, rr, vv, ww, R, pos0, rad, massw, rb, ff, tq);
   }
 
   if (fout) {  // F / T of the step, kept for export (last step of a launch sequence)
@@ -1356,7 +1410,8 @@ __global__ __launch_bounds__(BS) void k_force(
         const InstParams &q = P[cur];
         const double t = chunk_time(q, si);
         force_clump<MAXN, BS, VERLET, NEXT, LEAN>(q, sh, lane, c, st, sta, stage, clist, ccnt,
-                                                  vl_m, vl_mask, vl_cnt, fout, diag, cap, initial,
+                                                  vl_m, vl_mask, vl_cnt, lt.wreach, fout, diag,
+                                                  cap, initial,
                                             time_ahead ? t + q.dt : t, own, cr);
         break;
       }
@@ -1624,6 +1679,29 @@ __global__ void k_sgrid_finalize(InstParams *__restrict__ P, long long *__restri
 // ---------------------------------------------------------------------- reorder -----
 
 // Key of each clump for the spatial reorder: the sphere cell of its centre.
+// Storage order of the spatial reorder: the instance's sphere cells taken in tiles of
+// TILE^3 cells (tiles in row-major order, cells row-major inside a tile, tiles at the upper
+// edges truncated).  A bijection on the instance's cell range, so the counting sort over
+// the binning's cell counts applies.  Consecutive clumps then fill compact blocks instead of
+// rows of the box, so the wavefronts away from the walls skip the wall modules entirely.
+template <int TILE>
+__device__ __forceinline__ uint32_t tiled_cell_key(const InstParams &p, d3 q) {
+  const int nx = p.sg_n[0], ny = p.sg_n[1], nz = p.sg_n[2];
+  const int ix = grid_index(q.x, p.sg_lo[0], p.sg_inv_h, nx);
+  const int iy = grid_index(q.y, p.sg_lo[1], p.sg_inv_h, ny);
+  const int iz = grid_index(q.z, p.sg_lo[2], p.sg_inv_h, nz);
+  const int tx = ix / TILE, ty = iy / TILE, tz = iz / TILE;
+  const int wx = min(TILE, nx - tx * TILE), hy = min(TILE, ny - ty * TILE);
+  const int hz = min(TILE, nz - tz * TILE);
+  // cells of the full tile slabs below, the full tile rows before in this slab, the tiles
+  // before in this row, then the cell inside its tile
+  const int64_t base = (int64_t)tz * TILE * nx * ny + (int64_t)ty * TILE * nx * hz +
+                       (int64_t)tx * TILE * hy * hz;
+  return (uint32_t)(p.sg_base + base + (ix - tx * TILE) +
+                    (int64_t)wx * ((iy - ty * TILE) + (int64_t)hy * (iz - tz * TILE)));
+}
+constexpr int kReorderTile = 8;
+
 __global__ void k_reorder_key(const double *__restrict__ st, const int32_t *__restrict__ inst_of,
                               const InstParams *__restrict__ P, uint32_t *__restrict__ counts,
                               uint32_t *__restrict__ ckey, uint32_t *__restrict__ crank,
@@ -1631,8 +1709,8 @@ __global__ void k_reorder_key(const double *__restrict__ st, const int32_t *__re
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const InstParams &p = P[inst_of[c]];
-  const uint32_t key =
-      sphere_key(p, make_d3(st[S_RX * cap + c], st[S_RY * cap + c], st[S_RZ * cap + c]));
+  const uint32_t key = tiled_cell_key<kReorderTile>(
+      p, make_d3(st[S_RX * cap + c], st[S_RY * cap + c], st[S_RZ * cap + c]));
   ckey[c] = key;
   crank[c] = atomicAdd(&counts[key], 1u);
 }
