@@ -50,6 +50,7 @@ struct avf_ctx {
   double *d_stage = nullptr, *d_f = nullptr, *d_io = nullptr, *d_io2 = nullptr;
   double *d_stage2 = nullptr;  // second stage buffer of the fused list steps
   bool lean = false;           // avf_options.lean_state resolved at create
+  bool closed = false;         // avf_options.closed_forms resolved at create
   std::vector<hipEvent_t> prof_ev;  // timing events of the profiled / timed steps
   int32_t *d_orig = nullptr, *d_orig2 = nullptr, *d_inst = nullptr, *d_inst2 = nullptr;
   int64_t *d_inv = nullptr, *d_ncells = nullptr, *d_tmp64 = nullptr;
@@ -209,10 +210,10 @@ void launch_bin(avf_ctx *c, int mode) {
 
 ListTrack list_track(avf_ctx *c) { return ListTrack{c->d_dsp, c->d_wreach}; }
 
-template <int BS, bool VERLET, bool NEXT, bool LEAN = false>
+template <int BS, bool VERLET, bool NEXT, bool LEAN = false, bool CLOSED = false>
 void launch_force_bs(avf_ctx *c, int initial, bool ft, int time_ahead, const double *stage_in,
                      double *stage_out, int si) {
-  k_force<kMaxNeigh, BS, VERLET, NEXT, LEAN><<<grid_for(c->n, BS), BS, 0, c->stream>>>(
+  k_force<kMaxNeigh, BS, VERLET, NEXT, LEAN, CLOSED><<<grid_for(c->n, BS), BS, 0, c->stream>>>(
       c->d_st, c->d_sta, stage_in, c->d_inst, c->d_P, c->d_clist, c->d_ccnt, c->d_vl_m,
       c->d_vl_mask, c->d_vl_cnt, ft ? c->d_f : nullptr, c->d_diag, c->d_acc_cell, stage_out,
       c->d_orig, VERLET ? list_track(c) : ListTrack{nullptr, nullptr}, c->n, c->cap,
@@ -252,12 +253,18 @@ void launch_force(avf_ctx *c, int initial, bool ft = true, bool verlet = false,
     else
       launch_force_bs<128, false, false>(c, initial, ft, time_ahead, stage_in, nullptr, si);
   } else {
-    if (next && c->lean)
+    if (next && c->lean && c->closed)
+      launch_force_bs<kForceBlock, true, true, true, true>(c, initial, ft, time_ahead, stage_in,
+                                                           stage_out, si);
+    else if (next && c->lean)
       launch_force_bs<kForceBlock, true, true, true>(c, initial, ft, time_ahead, stage_in,
                                                      stage_out, si);
     else if (next)
       launch_force_bs<kForceBlock, true, true>(c, initial, ft, time_ahead, stage_in, stage_out,
                                                si);
+    else if (verlet && c->closed)
+      launch_force_bs<kForceBlock, true, false, false, true>(c, initial, ft, time_ahead,
+                                                             stage_in, nullptr, si);
     else if (verlet)
       launch_force_bs<kForceBlock, true, false>(c, initial, ft, time_ahead, stage_in, nullptr,
                                                 si);
@@ -643,6 +650,12 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
   // lean chunks unless asked otherwise: the stage record holds the whole carried state but
   // the angular acceleration, so a list step reads and writes each clump's state once
   c->lean = c->opt.lean_state >= 0;
+  // list steps recompute clump mass and inertia from radius and density (where they hold
+  // the closed forms) for working sets beyond the last-level cache, unless asked otherwise:
+  // there the bytes saved outweigh the arithmetic (+2.7 % at 2M clumps), while at 250k
+  // clumps the loads are cache hits and the arithmetic would cost 1 %
+  c->closed = c->opt.closed_forms > 0 ||
+              (c->opt.closed_forms == 0 && (double)total * 488.0 > 256.0 * 1024.0 * 1024.0);
   c->cell_cap = std::max<int64_t>(8 * 4 * total, 1 << 16);
   if (c->cell_cap > (int64_t)0xfffffff0) c->cell_cap = 0xfffffff0;
 
@@ -847,8 +860,13 @@ int avf_upload(avf_ctx *c, int32_t inst, const avf_particle *pp, const avf_doubl
                            c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_io2, extra.data(), extra.size() * sizeof(double),
                            hipMemcpyHostToDevice, c->stream));
+  // mass and inertia recomputed in the step when every clump holds the closed forms
+  const int32_t closed = c->closed ? 1 + c->desc[inst].geo.clump_mode : 0;
+  HIPCHK(c, hipMemcpyAsync(&c->d_P[inst].mass_closed, &closed, sizeof closed,
+                           hipMemcpyHostToDevice, c->stream));
   k_import_aos<<<grid_for(cnt, kBlock), kBlock, 0, c->stream>>>(
-      c->d_st, c->d_sta, c->d_inv, c->begin[inst], c->d_io, c->d_io2, cnt, c->cap);
+      c->d_st, c->d_sta, c->d_inv, c->begin[inst], c->d_io, c->d_io2, c->d_P + inst, cnt,
+      c->cap);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->sgrid_valid = false;

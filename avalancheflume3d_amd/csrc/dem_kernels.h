@@ -33,7 +33,10 @@ struct InstParams {
   int64_t begin, count;
   int64_t cc;
   int32_t regrid_interval, inter_flag, phase, modules_fill;
-  int32_t modules_aval, pad1;
+  // 1 + clump_mode when every clump's stored mass and inertia equal clump_mass_inertia() of
+  // its radius and density bit for bit (checked on upload) and the context asked for it:
+  // the step then recomputes them instead of loading them; 0 otherwise
+  int32_t modules_aval, mass_closed;
 };
 
 // Clump state fields (SoA, field-major: st[f * cap + c]).

@@ -55,6 +55,23 @@ __device__ __forceinline__ d3 sphere_of(d3 rr, const double *R, double pos, int 
                  rr.z + pos * (sx * R[6] + sy * R[7] + sz * R[8]));
 }
 
+// Mass and moment of inertia (isotropic) of a clump of four spheres of radius rad and density
+// rho, in the operation order of the library's setup (calculate_particle_parameters.cu:27-33;
+// clump_mode 1: I = 4 m_s (2/5 r^2 + 2/3 d^2) for the sphere offset d = sqrt(3) r / div).
+constexpr double kPiSetup = 3.1415926535897932;
+__device__ __forceinline__ void clump_mass_inertia(int mode, double rad, double rho, double div,
+                                                   double &mass, double &inertia) {
+  const double aux1 = 4.0 * kPiSetup / 3.0, aux2 = 112.0 * kPiSetup / 15.0;
+  const double rad3 = rad * rad * rad, rad5 = rad3 * rad * rad;
+  mass = 4.0 * aux1 * rad3 * rho;
+  if (mode == 0) {
+    inertia = aux2 * rho * rad5;
+  } else {
+    const double ms = aux1 * rad3 * rho, d = sqrt(3.0) * rad / div;
+    inertia = 4.0 * ms * (0.4 * rad * rad + (2.0 / 3.0) * d * d);
+  }
+}
+
 // Centre of sphere i of the clump whose stage record is g (its centre and rotation rows);
 // pos = the clump's radius / div (the record's radius: only records of clumps that are not
 // frozen, whose radius is the one they had before freezing, are used this way).
@@ -977,7 +994,7 @@ __device__ __forceinline__ double chunk_time(const InstParams &q, int si) {
 // * NEXT and LEAN: own from the stage record; only a, alpha go to st and the finished state
 //   is left in out for the next step's predict (predict_regs), which writes only R to st:
 //   144 B less HBM traffic per clump-step, for working sets beyond the last-level cache.
-template <int MAXN, int BS, bool VERLET, bool NEXT, bool LEAN>
+template <int MAXN, int BS, bool VERLET, bool NEXT, bool LEAN, bool CLOSED>
 __device__ __forceinline__ void force_clump(
     const InstParams &p, PPShared<MAXN, BS> &sh, int lane, int64_t c,
     double *__restrict__ st, const double *__restrict__ sta, const double *__restrict__ stage,
@@ -1275,9 +1292,17 @@ __device__ __forceinline__ void force_clump(
     return;
   }
 
-  // get_accel.cu:27-71
-  const double mass = sta[C_MASS * cap + c];
-  const d3 I = make_d3(sta[C_IX * cap + c], sta[C_IY * cap + c], sta[C_IZ * cap + c]);
+  // get_accel.cu:27-71 (rad is the radius before freezing here)
+  double mass;
+  d3 I;
+  if (CLOSED && p.mass_closed) {
+    double ic;
+    clump_mass_inertia(p.mass_closed - 1, rad, rho, p.sphere_div, mass, ic);
+    I = make_d3(ic, ic, ic);
+  } else {
+    mass = sta[C_MASS * cap + c];
+    I = make_d3(sta[C_IX * cap + c], sta[C_IY * cap + c], sta[C_IZ * cap + c]);
+  }
   const double Cxy = (I.x - I.y) / I.z, Cyz = (I.y - I.z) / I.x, Czx = (I.z - I.x) / I.y;
   double adrag = p.drag * rad * rad / mass;
   const double vmag = sqrt(vv.x * vv.x + vv.y * vv.y + vv.z * vv.z);
@@ -1360,8 +1385,11 @@ __device__ __forceinline__ void force_clump(
 // step (k_finalize applies the chunk's per-step advances once, at its end): the step's
 // time is chunk_time(P, si), its counter P.cc + si.
 // time_ahead: the force stage runs at the step's time + dt (candidate-list steps advance
-// the time at the end of the step instead of in k_scatter).  LEAN: see force_clump.
-template <int MAXN, int BS, bool VERLET, bool NEXT, bool LEAN>
+// the time at the end of the step instead of in k_scatter).  LEAN: see force_clump.  CLOSED:
+// clump mass and inertia from radius and density for instances whose stored values are the
+// closed forms (InstParams::mass_closed); a separate instantiation, since the code alone
+// costs the other kernels registers.
+template <int MAXN, int BS, bool VERLET, bool NEXT, bool LEAN, bool CLOSED>
 __global__ __launch_bounds__(BS) void k_force(
     double *__restrict__ st, double *__restrict__ sta, const double *__restrict__ stage,
     const int32_t *__restrict__ inst_of, const InstParams *__restrict__ P,
@@ -1409,10 +1437,9 @@ __global__ __launch_bounds__(BS) void k_force(
       if (inst == cur) {
         const InstParams &q = P[cur];
         const double t = chunk_time(q, si);
-        force_clump<MAXN, BS, VERLET, NEXT, LEAN>(q, sh, lane, c, st, sta, stage, clist, ccnt,
-                                                  vl_m, vl_mask, vl_cnt, lt.wreach, fout, diag,
-                                                  cap, initial,
-                                            time_ahead ? t + q.dt : t, own, cr);
+        force_clump<MAXN, BS, VERLET, NEXT, LEAN, CLOSED>(
+            q, sh, lane, c, st, sta, stage, clist, ccnt, vl_m, vl_mask, vl_cnt, lt.wreach, fout,
+            diag, cap, initial, time_ahead ? t + q.dt : t, own, cr);
         break;
       }
     }
@@ -1785,16 +1812,23 @@ __global__ void k_export_aos(const double *__restrict__ st, const double *__rest
 }
 
 // Inverse of k_export_aos for uploads (sphere centres and rrot are derived, not stored).
-// extra holds 8 doubles per clump: a_i[3], alpha_i[3], pre-freeze radius, unused.
+// extra holds 8 doubles per clump: a_i[3], alpha_i[3], pre-freeze radius, unused.  Clears
+// P->mass_closed (set beforehand to 1 + clump_mode, or 0) when a clump's mass or inertia
+// differs from the closed forms of its pre-freeze radius and density.
 __global__ void k_import_aos(double *__restrict__ st, double *__restrict__ sta,
                              const int64_t *__restrict__ inv, int64_t begin,
                              const double *__restrict__ in, const double *__restrict__ extra,
-                             int64_t count, int64_t cap) {
+                             InstParams *__restrict__ P, int64_t count, int64_t cap) {
   int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= count) return;
   const int64_t c = inv[begin + o];
   const double *r = in + o * 51;
   const double *x = extra + o * 8;
+  if (P->mass_closed) {
+    double m, ic;
+    clump_mass_inertia(P->mass_closed - 1, x[6], r[2], P->sphere_div, m, ic);
+    if (!(r[1] == m && r[3] == ic && r[4] == ic && r[5] == ic)) atomicAnd(&P->mass_closed, 0);
+  }
   sta[C_RAD * cap + c] = r[0];
   sta[C_MASS * cap + c] = r[1];
   sta[C_RHO * cap + c] = r[2];

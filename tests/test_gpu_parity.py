@@ -18,13 +18,14 @@ T_RTOL = 1e-9    # trajectories over 100 steps
 STATE = ["rr", "vv", "ww", "Rmatx", "Rmaty", "Rmatz", "rr_a", "rr_b", "rr_c", "rr_d", "radius"]
 
 
-def opts(graph=1, reorder=50, factor=1.0, skin=0.0, lean=0):
+def opts(graph=1, reorder=50, factor=1.0, skin=0.0, lean=0, closed=0):
     o = dem.default_options()
     o.use_graph = graph
     o.reorder_interval = reorder
     o.sphere_cell_factor = factor
     o.verlet_skin = skin
     o.lean_state = lean
+    o.closed_forms = closed
     return o
 
 
@@ -261,6 +262,34 @@ def test_graph_eager_and_reorder_invariance(gpu):
             assert np.array_equal(pp[fld].view(np.uint64), outs[0][0][fld].view(np.uint64)), fld
         assert np.array_equal(aa.view(np.uint64), outs[0][1].view(np.uint64))
         assert np.array_equal(f.view(np.uint64), outs[0][2].view(np.uint64))
+
+
+@pytest.mark.parametrize("skin", [0.0, 0.02])
+def test_closed_form_mass_and_inertia_change_no_bit(gpu, skin):
+    """Mass and inertia recomputed from radius and density (closed_forms 1) against loaded
+    (-1): reference clumps (clump_mode 0), config clumps (mode 1), polydisperse radii, and an
+    instance whose stored masses are not the closed form (checked on upload: loaded)."""
+    import copy
+
+    from avalancheflume3d_amd import workloads
+    poly = workloads.build(workloads.FlumeSpec(n=200, width=2.0, length=4.0, box=(2.0, 2.0, 3.21),
+                                               rmin=0.09, rmax=0.11, seed=5))
+    odd = copy.deepcopy(scenarios.hopper_state(200, 72))
+    odd.particles["mass"][7] *= 1.0 + 2.0 ** -40
+    for inst in (scenarios.hopper_state(300, 71), scenarios.avalanche_state(12, 73),
+                 workloads.small(), poly, odd):
+        outs = []
+        for closed in (1, -1):
+            ctx = gpu_ctx(inst, skin=skin, closed=closed)
+            ctx.initial_forces()
+            ctx.step(30)
+            pp, aa, anga = ctx.download()
+            outs.append((pp, aa, anga))
+            ctx.close()
+        for fld in STATE:
+            assert np.array_equal(outs[0][0][fld].view(np.uint64), outs[1][0][fld].view(np.uint64)), fld
+        assert np.array_equal(outs[0][1].view(np.uint64), outs[1][1].view(np.uint64))
+        assert np.array_equal(outs[0][2].view(np.uint64), outs[1][2].view(np.uint64))
 
 
 def test_batch_equals_separate_runs(gpu):
