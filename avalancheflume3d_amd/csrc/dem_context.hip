@@ -80,6 +80,7 @@ struct avf_ctx {
   bool binned = false;  // sorted sphere arrays describe the current positions
   int64_t steps_since_reorder = 0;
   int64_t steps_total = 0;
+  int64_t list_builds = 0, steps_undone = 0;  // avf_diag
   std::map<int64_t, hipGraphExec_t> graphs;
   std::vector<hipGraph_t> graph_defs;
   struct WinBuf *win = nullptr;  // buffers of the differentiable window (on first use)
@@ -476,7 +477,9 @@ int vsteps_profiled(avf_ctx *c, int64_t S, bool rebuild, StepProf *prof) {
 // chunk is undone from its checkpoint and redone after a rebuild; otherwise the next chunk
 // length is set from the bound's growth.  The results are those of per-step binning.
 int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
-  constexpr int64_t kMaxChunk = 128;
+  // chunk length cap: one chunk per reorder interval where the displacement bound allows
+  // (each chunk costs a checkpoint, its first predict, its last step and a host round trip)
+  constexpr int64_t kMaxChunk = 512, kGraphSteps = 128;
   const double skin = c->opt.verlet_skin;
   const size_t SF = (size_t)c->cap * S_NFIELDS * sizeof(double);
   int rc;
@@ -513,6 +516,7 @@ int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
       c->list_valid = true;
       c->binned = true;
       c->since_build = 0;
+      c->list_builds++;
       c->steps_since_reorder++;
       c->steps_total++;
       advance(1);
@@ -534,19 +538,28 @@ int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
     if (prof) {
       if ((rc = vsteps_profiled(c, S, false, prof))) return rc;
     } else if (c->opt.use_graph) {
-      const int64_t key = -S;  // list-step chunks (plain chunks use positive keys)
-      auto it = c->graphs.find(key);
-      if (it == c->graphs.end()) {
-        hipGraph_t g;
-        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (int64_t i = 0; i < S; i++) launch_vstep(c, false, nullptr, i + 1 == S, i, S);
-        HIPCHK(c, hipStreamEndCapture(c->stream, &g));
-        hipGraphExec_t ge;
-        HIPCHK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-        c->graph_defs.push_back(g);
-        it = c->graphs.emplace(key, ge).first;
+      // the chunk as a sequence of graphs of at most kGraphSteps steps (long graphs replay
+      // slower than a few short ones).  Only the piece holding the chunk's last step depends
+      // on the chunk length (its force variant, k_finalize over S steps): the others are
+      // cached by their first step alone.  Plain chunks use positive keys.
+      for (int64_t off = 0; off < S; off += kGraphSteps) {
+        const int64_t len = std::min(kGraphSteps, S - off);
+        const int64_t key = off + len < S ? -(int64_t{1} << 40) - off
+                                          : -(S * (kMaxChunk + 1) + off + 1);
+        auto it = c->graphs.find(key);
+        if (it == c->graphs.end()) {
+          hipGraph_t g;
+          HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+          for (int64_t i = off; i < off + len; i++)
+            launch_vstep(c, false, nullptr, i + 1 == S, i, S);
+          HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+          hipGraphExec_t ge;
+          HIPCHK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+          c->graph_defs.push_back(g);
+          it = c->graphs.emplace(key, ge).first;
+        }
+        HIPCHK(c, hipGraphLaunch(it->second, c->stream));
       }
-      HIPCHK(c, hipGraphLaunch(it->second, c->stream));
     } else {
       for (int64_t i = 0; i < S; i++) launch_vstep(c, false, nullptr, i + 1 == S, i, S);
     }
@@ -570,6 +583,7 @@ int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
                                c->stream));
       c->list_valid = false;
       c->vchunk = std::max<int64_t>(1, S / 2);
+      c->steps_undone += S;
       continue;
     }
     left -= S;
@@ -585,6 +599,7 @@ int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
       next = 1;
     }
     c->vchunk = std::min(next, kMaxChunk);
+    if (c->vchunk > 32) c->vchunk -= c->vchunk % 32;  // few distinct lengths: few graphs
   }
   return AVF_OK;
 }
@@ -1387,6 +1402,8 @@ int avf_get_diag(avf_ctx *c, avf_diag *d) {
   d->max_neighbours = (int64_t)h.max_neighbours;
   d->max_cell_spheres = (int64_t)h.max_cell;
   d->steps = c->steps_total;
+  d->list_builds = c->list_builds;
+  d->steps_undone = c->steps_undone;
   return AVF_OK;
 }
 

@@ -342,6 +342,8 @@ def main() -> int:
     ctx.sync()
     torch.cuda.synchronize()
     wall_graph = time.perf_counter() - t0
+    d2 = ctx.diag()
+    diag_graph = {k: d2[k] - diag[k] for k in ("steps", "list_builds", "steps_undone")}
     # per-kernel breakdown (events between all kernels: adds a few us per kernel)
     kern, _ = ctx.step_profiled(min(args.steps, 1000))
     ctx.close()
@@ -387,6 +389,7 @@ def main() -> int:
         "graph_path_clump_steps_per_s": n * args.steps / wall_graph,
         "per_step_rebuild_clump_steps_per_s": per_step,
         "diag": diag,
+        "diag_graph_path": diag_graph,
     }
     if args.batch_steps > 0:
         out["batched"] = batch_compare(args.batch_steps, skin=opt.verlet_skin,

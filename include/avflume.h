@@ -177,6 +177,9 @@ typedef struct avf_diag {
   int64_t max_neighbours;   /* largest neighbour count seen                             */
   int64_t max_cell_spheres; /* largest sphere-cell occupancy at the last binning        */
   int64_t steps;            /* steps taken by the context                               */
+  int64_t list_builds;      /* candidate-list builds                                    */
+  int64_t steps_undone;     /* candidate-list steps undone and redone after a rebuild    */
+                            /* (a sphere moved more than 0.49 skin within their chunk)   */
 } avf_diag;
 
 /* Directed sphere contact (l <- m) of force_part.cu:157-209. */
