@@ -37,7 +37,7 @@ DEFAULT_SKIN = 0.02
 # The timed region brackets k_force with HIP events on every TIMED_EVERY-th step (a regular
 # sample of its launches over the whole region; events on every step would add their own
 # pipeline stalls to the timed run).
-TIMED_EVERY = 4
+TIMED_EVERY = 8
 # Spatial reorder interval (steps) of the headline runs; a reorder also rebuilds the lists.
 DEFAULT_REORDER = 400
 
