@@ -987,13 +987,15 @@ __device__ __forceinline__ double chunk_time(const InstParams &q, int si) {
 // time: the force stage's time (sys.time after the step's advance).
 // State I/O of a clump (bit-identical results in every variant):
 // * plain (not VERLET): own centre and velocities from st; the finish writes v, w, a, alpha.
-// * VERLET, not NEXT (a rebuild step or the last step of a chunk): own centre and
-//   velocities from the stage record (own), which always holds them; the finish writes
-//   rr, v, w, a, alpha to st (in a LEAN chunk st holds rr, v, w only at the chunk start).
+// * VERLET, not NEXT (a rebuild step or the last step of a chunk): the own state (rr, v, w,
+//   R, a_i, radius, density, index) from the stage record (own), which always holds it;
+//   the finish writes rr, v, w, R, a, alpha to st (inside a LEAN chunk st keeps only alpha
+//   current).
 // * NEXT, not LEAN: as plain; the next step's predict then reads st and writes all of it.
-// * NEXT and LEAN: own from the stage record; only a, alpha go to st and the finished state
-//   is left in out for the next step's predict (predict_regs), which writes only R to st:
-//   144 B less HBM traffic per clump-step, for working sets beyond the last-level cache.
+// * NEXT and LEAN: own from the stage record; only alpha goes to st and the finished state
+//   is left in out for the next step's predict (predict_regs), which writes the next
+//   record: one read and one write of the state per clump-step.
+// wreach (VERLET): the wall modules each clump can reach while the lists are valid.
 template <int MAXN, int BS, bool VERLET, bool NEXT, bool LEAN, bool CLOSED>
 __device__ __forceinline__ void force_clump(
     const InstParams &p, PPShared<MAXN, BS> &sh, int lane, int64_t c,
