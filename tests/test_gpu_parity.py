@@ -243,6 +243,10 @@ def test_candidate_list_chunks_across_regrids(gpu, lean):
                 assert np.array_equal(pa[fld].view(np.uint64), pb[fld].view(np.uint64)), fld
             assert np.array_equal(aa.view(np.uint64), ab.view(np.uint64))
             assert a.time() == b.time()
+        d = b.diag()
+        # 144 steps with a reorder every 40: at least 4 list builds, every step counted
+        assert d["steps"] == 144 and d["list_builds"] >= 4 and d["steps_undone"] >= 0
+        assert a.diag()["list_builds"] == 0  # no candidate lists without a skin
         a.close()
         b.close()
 
