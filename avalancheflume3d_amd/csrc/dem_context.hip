@@ -318,9 +318,10 @@ void launch_step(avf_ctx *c, hipEvent_t *ev = nullptr, bool ft = true) {
 // avf_step_profiled are kept (the binning slots are empty on list steps, the predict slot
 // after the first step of a chunk, the finalize slot before its last).
 void launch_vstep(avf_ctx *c, bool rebuild, hipEvent_t *ev = nullptr, bool ft = true,
-                  int64_t i = 0, int64_t S = 1) {
-  auto mark = [&](int k) {
-    if (ev && (c->prof_kernel < 0 || k == c->prof_kernel || k == c->prof_kernel + 1))
+                  int64_t i = 0, int64_t S = 1, unsigned marks = ~0u) {
+  auto mark = [&](int k) {  // marks: which of the boundaries record their event
+    if (ev && ((marks >> k) & 1u) &&
+        (c->prof_kernel < 0 || k == c->prof_kernel || k == c->prof_kernel + 1))
       (void)hipEventRecord(ev[k], c->stream);
   };
   const unsigned nb = grid_for(c->cell_cap, kScanTile);
@@ -445,10 +446,44 @@ int vsteps_profiled(avf_ctx *c, int64_t S, bool rebuild, StepProf *prof) {
   constexpr int kEv = AVF_NKERNELS + 1;
   hipEvent_t *ev = nullptr;
   int rc = prof_events(c, (size_t)S * kEv, &ev);
+  const int64_t E = c->prof_every;
+  if (rc == AVF_OK && !rebuild && c->prof_kernel == AVF_K_FORCE && E > 1) {
+    // k_force of a chunk's list steps: runs of E consecutive launches (back to back, nothing
+    // else between them) inside one pair of events, one run starting every 2E steps; never
+    // the chunk's last step (another force variant).  The events' own cost is then shared
+    // by E launches.
+    std::vector<int64_t> runs;  // first step of each run
+    int64_t run_end = -1;
+    for (int64_t i = 0; i < S; i++) {
+      unsigned marks = 0u;
+      if (run_end < 0 && c->prof_seq % (2 * E) == 0 && i + E <= S - 1) {
+        runs.push_back(i);
+        run_end = i + E - 1;
+        marks |= 1u << AVF_K_FORCE;
+      }
+      if (i == run_end) marks |= 1u << (AVF_K_FORCE + 1);
+      c->prof_seq++;
+      launch_vstep(c, rebuild, marks ? &ev[(runs.size() - 1) * kEv] : nullptr, i + 1 == S, i,
+                   S, marks);
+      if (i == run_end) run_end = -1;
+    }
+    hipError_t he = hipSuccess;
+    if (!runs.empty()) he = hipEventSynchronize(ev[(runs.size() - 1) * kEv + AVF_K_FORCE + 1]);
+    for (size_t r = 0; r < runs.size() && he == hipSuccess; r++) {
+      float ms = 0.f;
+      he = hipEventElapsedTime(&ms, ev[r * kEv + AVF_K_FORCE], ev[r * kEv + AVF_K_FORCE + 1]);
+      prof->sum[AVF_K_FORCE] += ms;
+      prof->launched += E;
+    }
+    if (he != hipSuccess) rc = fail(c, AVF_ERR_HIP, "event timing: %s", hipGetErrorString(he));
+    return rc;
+  }
   if (rc == AVF_OK) {
     std::vector<int64_t> timed;  // steps of this chunk carrying events
     for (int64_t i = 0; i < S; i++) {
-      const bool rec = c->prof_every <= 1 || c->prof_seq % c->prof_every == 0;
+      // (runs of k_force list steps time only that variant: rebuild steps stay out)
+      const bool rec = (c->prof_every <= 1 || c->prof_seq % c->prof_every == 0) &&
+                       !(rebuild && c->prof_kernel == AVF_K_FORCE && c->prof_every > 1);
       c->prof_seq++;
       if (rec) timed.push_back(i);
       launch_vstep(c, rebuild, rec ? &ev[(size_t)i * kEv] : nullptr, i + 1 == S, i, S);

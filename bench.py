@@ -34,9 +34,9 @@ STEP_BYTES_PER_CLUMP = 480.0
 # Candidate-list skin (cm) of the headline runs: 10 % of the sphere radius (a wider skin lists
 # more candidates; a narrower one ends chunks and rebuilds the lists sooner).
 DEFAULT_SKIN = 0.02
-# The timed region brackets k_force with HIP events on every TIMED_EVERY-th step (a regular
-# sample of its launches over the whole region; events on every step would add their own
-# pipeline stalls to the timed run).
+# The timed region brackets runs of TIMED_EVERY consecutive k_force launches (candidate-list
+# steps, back to back) with one pair of HIP events, one run in every 2 * TIMED_EVERY steps: a
+# regular sample over the whole region, with the events' own cost shared by the run.
 TIMED_EVERY = 8
 # Spatial reorder interval (steps) of the headline runs; a reorder also rebuilds the lists.
 DEFAULT_REORDER = 400

@@ -343,9 +343,11 @@ enum {
  * *total_ms the elapsed time from the first to the last kernel of the run. */
 int avf_step_profiled(avf_ctx *ctx, int64_t nsteps, double *kernel_ms, double *total_ms);
 /* Like avf_step_profiled with events around one kernel only (AVF_K_*), on every
- * `every`-th step (1: all; on candidate-list steps, fewer events disturb the run less):
- * *kernel_ms receives its mean duration per timed launch, *total_ms the elapsed time of the
- * run. */
+ * `every`-th step (1: all).  For AVF_K_FORCE on candidate-list steps with every > 1, one
+ * pair of events brackets a run of `every` consecutive k_force launches (nothing else runs
+ * between them), one run in every 2 * `every` steps, so the events' own cost is shared by
+ * the run.  *kernel_ms receives the mean duration per timed launch, *total_ms the elapsed
+ * time of the run. */
 int avf_step_timed(avf_ctx *ctx, int64_t nsteps, int32_t kernel, int32_t every,
                    double *kernel_ms, double *total_ms);
 /* Waits for all work queued on the context's stream. */
