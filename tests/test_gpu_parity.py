@@ -1,6 +1,6 @@
 """GPU parity: the gfx950 path (through the C ABI) against the CPU oracle on seeded states.
 
-Bars (BASELINE north star): contact sets and clump-cell assignment bit-exact; per-step F/T
+Bars: contact sets and clump-cell assignment bit-exact; per-step F/T
 within 1e-12 relative; 100-step trajectories within 1e-9 relative.  The kernels keep the
 reference's operation order, so the tests also report whether results are bit-identical.
 """
