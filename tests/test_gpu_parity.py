@@ -294,6 +294,19 @@ def test_closed_form_mass_and_inertia_change_no_bit(gpu, skin):
             assert np.array_equal(outs[0][0][fld].view(np.uint64), outs[1][0][fld].view(np.uint64)), fld
         assert np.array_equal(outs[0][1].view(np.uint64), outs[1][1].view(np.uint64))
         assert np.array_equal(outs[0][2].view(np.uint64), outs[1][2].view(np.uint64))
+    # one batch: instances that pass the upload check next to one that does not
+    batch = [scenarios.hopper_state(150, 74), odd, poly]
+    outs = []
+    for closed in (1, -1):
+        ctx = gpu_ctx(batch, skin=skin, closed=closed)
+        ctx.initial_forces()
+        ctx.step(30)
+        outs.append([ctx.download(k) for k in range(len(batch))])
+        ctx.close()
+    for a, b in zip(*outs):
+        for fld in STATE:
+            assert np.array_equal(a[0][fld].view(np.uint64), b[0][fld].view(np.uint64)), fld
+        assert np.array_equal(a[1].view(np.uint64), b[1].view(np.uint64))
 
 
 def test_batch_equals_separate_runs(gpu):
