@@ -309,6 +309,28 @@ def test_closed_form_mass_and_inertia_change_no_bit(gpu, skin):
         assert np.array_equal(a[1].view(np.uint64), b[1].view(np.uint64))
 
 
+def test_large_configuration_default_path(gpu):
+    """2,000,000 polydisperse clumps (lean chunks, and by default for this working set the
+    mass and inertia recomputed from radius and density) against the same run with them
+    loaded: 20 candidate-list steps, bit for bit, with no clump outside the clump-cell grid
+    and no neighbour-list overflow."""
+    from avalancheflume3d_amd import workloads
+    inst = workloads.large()
+    outs = []
+    for closed in (0, -1):
+        ctx = gpu_ctx(inst, skin=0.02, reorder=400, closed=closed)
+        ctx.initial_forces()
+        ctx.step(20)
+        outs.append(ctx.download())
+        d = ctx.diag()
+        assert d["cell_below"] == 0 and d["cell_beyond"] == 0 and d["neigh_overflow"] == 0
+        ctx.close()
+    for fld in STATE:
+        assert np.array_equal(outs[0][0][fld].view(np.uint64), outs[1][0][fld].view(np.uint64)), fld
+    assert np.array_equal(outs[0][1].view(np.uint64), outs[1][1].view(np.uint64))
+    assert np.array_equal(outs[0][2].view(np.uint64), outs[1][2].view(np.uint64))
+
+
 def test_batch_equals_separate_runs(gpu):
     insts = [scenarios.hopper_state(120, 41), scenarios.avalanche_state(10, 42),
              scenarios.hopper_state(333, 43)]
