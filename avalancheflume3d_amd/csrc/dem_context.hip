@@ -51,6 +51,7 @@ struct avf_ctx {
   double *d_stage2 = nullptr;  // second stage buffer of the fused list steps
   bool lean = false;           // avf_options.lean_state resolved at create
   bool closed = false;         // avf_options.closed_forms resolved at create
+  bool deep = false;           // k_force gathers candidates ahead (small contexts)
   std::vector<hipEvent_t> prof_ev;  // timing events of the profiled / timed steps
   int32_t *d_orig = nullptr, *d_orig2 = nullptr, *d_inst = nullptr, *d_inst2 = nullptr;
   int64_t *d_inv = nullptr, *d_ncells = nullptr, *d_tmp64 = nullptr;
@@ -211,10 +212,12 @@ void launch_bin(avf_ctx *c, int mode) {
 
 ListTrack list_track(avf_ctx *c) { return ListTrack{c->d_dsp, c->d_wreach}; }
 
-template <int BS, bool VERLET, bool NEXT, bool LEAN = false, bool CLOSED = false>
+template <int BS, bool VERLET, bool NEXT, bool LEAN = false, bool CLOSED = false,
+          bool DEEP = false>
 void launch_force_bs(avf_ctx *c, int initial, bool ft, int time_ahead, const double *stage_in,
                      double *stage_out, int si) {
-  k_force<kMaxNeigh, BS, VERLET, NEXT, LEAN, CLOSED><<<grid_for(c->n, BS), BS, 0, c->stream>>>(
+  k_force<kMaxNeigh, BS, VERLET, NEXT, LEAN, CLOSED, DEEP>
+      <<<grid_for(c->n, BS), BS, 0, c->stream>>>(
       c->d_st, c->d_sta, stage_in, c->d_inst, c->d_P, c->d_clist, c->d_ccnt, c->d_vl_m,
       c->d_vl_mask, c->d_vl_cnt, ft ? c->d_f : nullptr, c->d_diag, c->d_acc_cell, stage_out,
       c->d_orig, VERLET ? list_track(c) : ListTrack{nullptr, nullptr}, c->n, c->cap,
@@ -257,6 +260,9 @@ void launch_force(avf_ctx *c, int initial, bool ft = true, bool verlet = false,
     if (next && c->lean && c->closed)
       launch_force_bs<kForceBlock, true, true, true, true>(c, initial, ft, time_ahead, stage_in,
                                                            stage_out, si);
+    else if (next && c->lean && c->deep)
+      launch_force_bs<kForceBlock, true, true, true, false, true>(c, initial, ft, time_ahead,
+                                                                  stage_in, stage_out, si);
     else if (next && c->lean)
       launch_force_bs<kForceBlock, true, true, true>(c, initial, ft, time_ahead, stage_in,
                                                      stage_out, si);
@@ -266,6 +272,9 @@ void launch_force(avf_ctx *c, int initial, bool ft = true, bool verlet = false,
     else if (verlet && c->closed)
       launch_force_bs<kForceBlock, true, false, false, true>(c, initial, ft, time_ahead,
                                                              stage_in, nullptr, si);
+    else if (verlet && c->deep)
+      launch_force_bs<kForceBlock, true, false, false, false, true>(c, initial, ft, time_ahead,
+                                                                    stage_in, nullptr, si);
     else if (verlet)
       launch_force_bs<kForceBlock, true, false>(c, initial, ft, time_ahead, stage_in, nullptr,
                                                 si);
@@ -704,6 +713,9 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
   // the closed forms) for working sets beyond the last-level cache, unless asked otherwise:
   // there the bytes saved outweigh the arithmetic (+2.7 % at 2M clumps), while at 250k
   // clumps the loads are cache hits and the arithmetic would cost 1 %
+  // contexts of at most one wavefront per SIMD (256 CUs x 4 SIMDs x 64 clumps) are bound by
+  // the latency of the candidate gathers, not by occupancy
+  c->deep = total <= 256 * 4 * kForceBlock;
   c->closed = c->opt.closed_forms > 0 ||
               (c->opt.closed_forms == 0 && (double)total * 488.0 > 256.0 * 1024.0 * 1024.0);
   c->cell_cap = std::max<int64_t>(8 * 4 * total, 1 << 16);

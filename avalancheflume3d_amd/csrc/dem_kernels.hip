@@ -996,7 +996,7 @@ __device__ __forceinline__ double chunk_time(const InstParams &q, int si) {
 //   is left in out for the next step's predict (predict_regs), which writes the next
 //   record: one read and one write of the state per clump-step.
 // wreach (VERLET): the wall modules each clump can reach while the lists are valid.
-template <int MAXN, int BS, bool VERLET, bool NEXT, bool LEAN, bool CLOSED>
+template <int MAXN, int BS, bool VERLET, bool NEXT, bool LEAN, bool CLOSED, bool DEEP>
 __device__ __forceinline__ void force_clump(
     const InstParams &p, PPShared<MAXN, BS> &sh, int lane, int64_t c,
     double *__restrict__ st, const double *__restrict__ sta, const double *__restrict__ stage,
@@ -1062,66 +1062,158 @@ __device__ __forceinline__ void force_clump(
     if (lk == p.ncell[2]) lk--;
     if constexpr (VERLET) {
       const int kc = vl_cnt[c];
-      // the next entry is fetched one iteration ahead and a candidate's centre, radius, id
-      // and sphere centres are loaded together, so an entry waits for one gather
-      int64_t cm_next = kc > 0 ? vl_m[c] : 0;
-      uint32_t bits_next = kc > 0 ? vl_mask[c] : 0u;
-      for (int e = 0; e < kc; e++) {
-        const int64_t cm = cm_next;
-        uint32_t bits = bits_next;
-        if (e + 1 < kc) {
-          cm_next = vl_m[(e + 1) * cap + c];
-          bits_next = vl_mask[(e + 1) * cap + c];
+      if constexpr (!DEEP) {
+        // the next entry is fetched one iteration ahead and a candidate's centre, radius, id
+        // and sphere centres are loaded together, so an entry waits for one gather
+        int64_t cm_next = kc > 0 ? vl_m[c] : 0;
+        uint32_t bits_next = kc > 0 ? vl_mask[c] : 0u;
+        for (int e = 0; e < kc; e++) {
+          const int64_t cm = cm_next;
+          uint32_t bits = bits_next;
+          if (e + 1 < kc) {
+            cm_next = vl_m[(e + 1) * cap + c];
+            bits_next = vl_mask[(e + 1) * cap + c];
+          }
+          const double *g = stage + cm * kStageDoubles;
+          const d3 rm = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
+          const double radm = g[G_RAD], idm = g[G_ID];
+          double Rm[9];
+  #pragma unroll
+          for (int k = 0; k < 9; k++) Rm[k] = g[G_R + k];
+          if (frozen(time, rm.x, p.side_x)) continue;  // not binned at this time (cells.cu:65)
+          const double posm = radm / p.sphere_div;
+          // the candidate pairings that touch now: k_detect's test on the same operands
+          const double sep = rad + radm, sep2 = sep * sep;
+          uint32_t mask = 0u;
+          while (bits) {
+            const int pr = __builtin_ctz(bits);
+            bits &= bits - 1u;
+            const int sdiv = pr >> 2, smod = pr & 3;
+            const d3 q = sphere_of(rr, R, pos0, smod);
+            const d3 o = sphere_of(rm, Rm, posm, sdiv);
+            const double dx = o.x - q.x, dy = o.y - q.y, dz = o.z - q.z;
+            if (dx * dx + dy * dy + dz * dz < sep2) mask |= 1u << pr;
+          }
+          if (!mask) continue;
+          const uint32_t morig = (uint32_t)__double_as_longlong(idm);
+          int64_t ai, aj, ak;
+          if (!ref_cell_ijk(p, rm, ai, aj, ak)) {
+            atomicAdd(&diag->cell_below, 1ull);
+            continue;
+          }
+          const int64_t di = ai - li, dj = aj - lj, dk = ak - lk;
+          if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) {
+            atomicAdd(&diag->cell_beyond, 1ull);
+            continue;
+          }
+          const uint32_t off = (uint32_t)((di + 1) * 9 + (dj + 1) * 3 + (dk + 1));
+          const uint32_t key = (off << 27) | morig;
+          if (cnt == MAXN) {
+            overflow = true;
+            continue;
+          }
+          int pos = cnt;
+          while (pos > 0 && sh.key[pos - 1][lane] > key) {
+            sh.key[pos][lane] = sh.key[pos - 1][lane];
+            sh.m[pos][lane] = sh.m[pos - 1][lane];
+            sh.mask[pos][lane] = sh.mask[pos - 1][lane];
+            pos--;
+          }
+          sh.key[pos][lane] = key;
+          sh.m[pos][lane] = (int32_t)cm;
+          sh.mask[pos][lane] = (uint16_t)mask;
+          cnt++;
         }
-        const double *g = stage + cm * kStageDoubles;
-        const d3 rm = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
-        const double radm = g[G_RAD], idm = g[G_ID];
-        double Rm[9];
+      } else {
+        // the same entry visit as above, written once as a lambda
+        // one candidate entry: the touching test of k_detect on the same operands, then the
+        // reference's visit rule and key
+        auto visit = [&](int64_t cm, uint32_t bits, d3 rm, double radm, double idm,
+                         const double *Rm) {
+          if (frozen(time, rm.x, p.side_x)) return;  // not binned at this time (cells.cu:65)
+          const double posm = radm / p.sphere_div;
+          // the candidate pairings that touch now: k_detect's test on the same operands
+          const double sep = rad + radm, sep2 = sep * sep;
+          uint32_t mask = 0u;
+          while (bits) {
+            const int pr = __builtin_ctz(bits);
+            bits &= bits - 1u;
+            const int sdiv = pr >> 2, smod = pr & 3;
+            const d3 q = sphere_of(rr, R, pos0, smod);
+            const d3 o = sphere_of(rm, Rm, posm, sdiv);
+            const double dx = o.x - q.x, dy = o.y - q.y, dz = o.z - q.z;
+            if (dx * dx + dy * dy + dz * dz < sep2) mask |= 1u << pr;
+          }
+          if (!mask) return;
+          const uint32_t morig = (uint32_t)__double_as_longlong(idm);
+          int64_t ai, aj, ak;
+          if (!ref_cell_ijk(p, rm, ai, aj, ak)) {
+            atomicAdd(&diag->cell_below, 1ull);
+            return;
+          }
+          const int64_t di = ai - li, dj = aj - lj, dk = ak - lk;
+          if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) {
+            atomicAdd(&diag->cell_beyond, 1ull);
+            return;
+          }
+          const uint32_t off = (uint32_t)((di + 1) * 9 + (dj + 1) * 3 + (dk + 1));
+          const uint32_t key = (off << 27) | morig;
+          if (cnt == MAXN) {
+            overflow = true;
+            return;
+          }
+          int pos = cnt;
+          while (pos > 0 && sh.key[pos - 1][lane] > key) {
+            sh.key[pos][lane] = sh.key[pos - 1][lane];
+            sh.m[pos][lane] = sh.m[pos - 1][lane];
+            sh.mask[pos][lane] = sh.mask[pos - 1][lane];
+            pos--;
+          }
+          sh.key[pos][lane] = key;
+          sh.m[pos][lane] = (int32_t)cm;
+          sh.mask[pos][lane] = (uint16_t)mask;
+          cnt++;
+        };
+        // a candidate's centre, radius, id and rotation rows, loaded together
+        auto gather = [&](int64_t cm, d3 &rm, double &radm, double &idm, double *Rm) {
+          const double *g = stage + cm * kStageDoubles;
+          rm = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
+          radm = g[G_RAD];
+          idm = g[G_ID];
+  #pragma unroll
+          for (int k = 0; k < 9; k++) Rm[k] = g[G_R + k];
+        };
+        int64_t cm_next = kc > 0 ? vl_m[c] : 0;
+        uint32_t bits_next = kc > 0 ? vl_mask[c] : 0u;
+        // small contexts (at most one wavefront per SIMD, registers to spare): the next
+        // entry's record is gathered, and the one after's index fetched, while an entry is
+        // tested
+        d3 rm_n = make_d3(0.0, 0.0, 0.0);
+        double radm_n = 0.0, idm_n = 0.0, Rm_n[9];
 #pragma unroll
-        for (int k = 0; k < 9; k++) Rm[k] = g[G_R + k];
-        if (frozen(time, rm.x, p.side_x)) continue;  // not binned at this time (cells.cu:65)
-        const double posm = radm / p.sphere_div;
-        // the candidate pairings that touch now: k_detect's test on the same operands
-        const double sep = rad + radm, sep2 = sep * sep;
-        uint32_t mask = 0u;
-        while (bits) {
-          const int pr = __builtin_ctz(bits);
-          bits &= bits - 1u;
-          const int sdiv = pr >> 2, smod = pr & 3;
-          const d3 q = sphere_of(rr, R, pos0, smod);
-          const d3 o = sphere_of(rm, Rm, posm, sdiv);
-          const double dx = o.x - q.x, dy = o.y - q.y, dz = o.z - q.z;
-          if (dx * dx + dy * dy + dz * dz < sep2) mask |= 1u << pr;
+        for (int k = 0; k < 9; k++) Rm_n[k] = 0.0;
+        if (kc > 0) gather(cm_next, rm_n, radm_n, idm_n, Rm_n);
+        int64_t cm_nn = kc > 1 ? vl_m[cap + c] : 0;
+        uint32_t bits_nn = kc > 1 ? vl_mask[cap + c] : 0u;
+        for (int e = 0; e < kc; e++) {
+          const int64_t cm = cm_next;
+          const uint32_t bits = bits_next;
+          const d3 rm = rm_n;
+          const double radm = radm_n, idm = idm_n;
+          double Rm[9];
+#pragma unroll
+          for (int k = 0; k < 9; k++) Rm[k] = Rm_n[k];
+          if (e + 1 < kc) {
+            cm_next = cm_nn;
+            bits_next = bits_nn;
+            gather(cm_next, rm_n, radm_n, idm_n, Rm_n);
+            if (e + 2 < kc) {
+              cm_nn = vl_m[(e + 2) * cap + c];
+              bits_nn = vl_mask[(e + 2) * cap + c];
+            }
+          }
+          visit(cm, bits, rm, radm, idm, Rm);
         }
-        if (!mask) continue;
-        const uint32_t morig = (uint32_t)__double_as_longlong(idm);
-        int64_t ai, aj, ak;
-        if (!ref_cell_ijk(p, rm, ai, aj, ak)) {
-          atomicAdd(&diag->cell_below, 1ull);
-          continue;
-        }
-        const int64_t di = ai - li, dj = aj - lj, dk = ak - lk;
-        if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) {
-          atomicAdd(&diag->cell_beyond, 1ull);
-          continue;
-        }
-        const uint32_t off = (uint32_t)((di + 1) * 9 + (dj + 1) * 3 + (dk + 1));
-        const uint32_t key = (off << 27) | morig;
-        if (cnt == MAXN) {
-          overflow = true;
-          continue;
-        }
-        int pos = cnt;
-        while (pos > 0 && sh.key[pos - 1][lane] > key) {
-          sh.key[pos][lane] = sh.key[pos - 1][lane];
-          sh.m[pos][lane] = sh.m[pos - 1][lane];
-          sh.mask[pos][lane] = sh.mask[pos - 1][lane];
-          pos--;
-        }
-        sh.key[pos][lane] = key;
-        sh.m[pos][lane] = (int32_t)cm;
-        sh.mask[pos][lane] = (uint16_t)mask;
-        cnt++;
       }
     } else {
     for (int s = 0; s < 4; s++) {
@@ -1390,9 +1482,11 @@ __device__ __forceinline__ void force_clump(
 // the time at the end of the step instead of in k_scatter).  LEAN: see force_clump.  CLOSED:
 // clump mass and inertia from radius and density for instances whose stored values are the
 // closed forms (InstParams::mass_closed); a separate instantiation, since the code alone
-// costs the other kernels registers.
-template <int MAXN, int BS, bool VERLET, bool NEXT, bool LEAN, bool CLOSED>
-__global__ __launch_bounds__(BS) void k_force(
+// costs the other kernels registers.  DEEP: candidate records gathered one entry ahead, for
+// contexts of at most one wavefront per SIMD (latency, not occupancy, bounds them).
+template <int MAXN, int BS, bool VERLET, bool NEXT, bool LEAN, bool CLOSED, bool DEEP>
+__global__ __launch_bounds__(BS)
+__attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
     double *__restrict__ st, double *__restrict__ sta, const double *__restrict__ stage,
     const int32_t *__restrict__ inst_of, const InstParams *__restrict__ P,
     const uint32_t *__restrict__ clist, const int32_t *__restrict__ ccnt,
@@ -1439,7 +1533,7 @@ __global__ __launch_bounds__(BS) void k_force(
       if (inst == cur) {
         const InstParams &q = P[cur];
         const double t = chunk_time(q, si);
-        force_clump<MAXN, BS, VERLET, NEXT, LEAN, CLOSED>(
+        force_clump<MAXN, BS, VERLET, NEXT, LEAN, CLOSED, DEEP>(
             q, sh, lane, c, st, sta, stage, clist, ccnt, vl_m, vl_mask, vl_cnt, lt.wreach, fout,
             diag, cap, initial, time_ahead ? t + q.dt : t, own, cr);
         break;
