@@ -68,14 +68,22 @@ class FlumeSpec:
 
 
 def build(spec: FlumeSpec) -> dem.Instance:
-    """Config-mode instance: avalanche phase from t = 0 (gate open), zero velocity."""
+    """Config-mode instance: avalanche phase from t = 0 (gate open), zero velocity.
+
+    The release box is the hopper of the geometry (x in [0, box x], y between yhop_l and
+    yhop_r, above z = 0), but its walls act only while filling: released clumps leave the box
+    underneath them, and force_hop.cu applies its walls to any sphere above the hopper
+    bottom, so a clump pushed back up past that edge from outside would meet the wall penalty
+    at full penetration.  The funnel planes, or without a funnel the flume's side walls
+    (then along the whole slope, up-slope of the box too), confine the released flow."""
     geo = dem.reference_geometry()
     geo.sphere_div = sphere_div(spec.rmax, 0.75 * spec.rmax)
     geo.clump_mode = 1
     bx, by, bz = spec.box
     fw = spec.funnel_width if spec.funnel_width is not None else spec.width
     geo.yfunnel = fw
-    geo.lfunnel = spec.funnel_length
+    # without a funnel the side walls start far up-slope (they act where x' > lfunnel)
+    geo.lfunnel = spec.funnel_length if spec.funnel_width is not None else -1.0e6
     geo.hop_width = bx
     geo.yhop_l = 0.5 * (fw - by)
     geo.yhop_r = geo.yhop_l + by
@@ -91,7 +99,7 @@ def build(spec: FlumeSpec) -> dem.Instance:
         geo.dep_x0, geo.dep_x1 = 0.0, 0.0
         geo.dep_z0 = -(spec.funnel_length + spec.length) * math.sin(tilt)
     geo.modules_fill = mods
-    geo.modules_avalanche = mods
+    geo.modules_avalanche = mods & ~abi.MOD_HOP
 
     ms = sphere_mass(spec.rmin)
     t_c = t_coll_for(spec.k_n, spec.eps, ms)
