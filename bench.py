@@ -321,7 +321,10 @@ def main() -> int:
     ctx.sync()
     torch.cuda.synchronize()
     # the state after the warmup: the graph-path run below replays the same segment
+    t0 = time.perf_counter()
     snap = ctx.download()
+    ctx.sync()
+    download_s = time.perf_counter() - t0
     t_snap, cc_snap = ctx.time()
     # timed region: K steps launched on the library's stream with HIP events around the
     # dominant kernel (k_force) only, for the roofline
@@ -333,7 +336,10 @@ def main() -> int:
     diag = ctx.diag()
     # the same steps from the same state, replayed as captured hipGraphs (the clump-cell grid
     # is recomputed for the restored state)
+    t0 = time.perf_counter()
     ctx.upload(0, *snap)
+    ctx.sync()
+    upload_s = time.perf_counter() - t0
     ctx.set_phase(0, inst.phase, t_snap, inst.sys.side.x, cc_snap)
     ctx.regrid()
     ctx.sync()
@@ -387,6 +393,13 @@ def main() -> int:
         "step_hbm_gbs_algorithmic": STEP_BYTES_PER_CLUMP * n / (wall / args.steps) / 1e9,
         "events_total_ms": total_ms,
         "graph_path_clump_steps_per_s": n * args.steps / wall_graph,
+        # the boundary's host hand-over (reference particle records, 408 B, plus the two
+        # acceleration arrays, 48 B, per clump, from and to pageable host memory) and the rate
+        # of a job that uploads the state, runs the timed steps and downloads it
+        "host_transfer": {"bytes": 456 * n, "upload_ms": 1e3 * upload_s,
+                          "download_ms": 1e3 * download_s,
+                          "pcie_inclusive_clump_steps_per_s":
+                              n * args.steps / (upload_s + wall + download_s)},
         "per_step_rebuild_clump_steps_per_s": per_step,
         "diag": diag,
         "diag_graph_path": diag_graph,
