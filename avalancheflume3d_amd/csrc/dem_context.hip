@@ -1440,9 +1440,14 @@ int avf_observe(avf_ctx *c, int32_t inst, avf_observables *obs, const avf_profil
 int avf_get_diag(avf_ctx *c, avf_diag *d) {
   if (!c || !d) return AVF_ERR_ARG;
   DevDiag h;
+  std::vector<InstParams> hp(c->ninst);
   HIPCHK(c, hipMemcpyAsync(&h, c->d_diag, sizeof h, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemsetAsync(c->d_diag, 0, sizeof h, c->stream));
+  HIPCHK(c, hipMemcpyAsync(hp.data(), c->d_P, hp.size() * sizeof(InstParams),
+                           hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  int64_t closed = 0;
+  for (const InstParams &p : hp) closed += p.mass_closed != 0;
   d->cell_below = (int64_t)h.cell_below;
   d->cell_beyond = (int64_t)h.cell_beyond;
   d->neigh_overflow = (int64_t)h.neigh_overflow;
@@ -1451,6 +1456,7 @@ int avf_get_diag(avf_ctx *c, avf_diag *d) {
   d->steps = c->steps_total;
   d->list_builds = c->list_builds;
   d->steps_undone = c->steps_undone;
+  d->closed_instances = closed;
   return AVF_OK;
 }
 

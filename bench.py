@@ -29,6 +29,9 @@ HBM_PEAK_GBS = 8000.0
 # bound (8 B).  The list entries (6 B each) and the neighbours' stage records (L2 hits) are
 # not counted.
 FORCE_BYTES_PER_CLUMP = 488.0
+# Mass and inertia are not read where the list steps recompute them from radius and density
+# (working sets beyond the last-level cache; avf_diag.closed_instances says so): 456 B there.
+MASS_INERTIA_BYTES = 32.0
 # Algorithmic bytes per clump-step of the whole step (480 B per clump-step).
 STEP_BYTES_PER_CLUMP = 480.0
 # Candidate-list skin (cm) of the headline runs: 10 % of the sphere radius (a wider skin lists
@@ -369,7 +372,8 @@ def main() -> int:
         c2.close()
 
     value = n * args.steps / wall
-    achieved = FORCE_BYTES_PER_CLUMP * n / (force_ms * 1e-3) / 1e9
+    bpc = FORCE_BYTES_PER_CLUMP - (MASS_INERTIA_BYTES if diag["closed_instances"] else 0.0)
+    achieved = bpc * n / (force_ms * 1e-3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
@@ -388,7 +392,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_force", "kernel_ms": force_ms,
-                     "bytes_per_clump": FORCE_BYTES_PER_CLUMP},
+                     "bytes_per_clump": bpc},
         "kernel_ms_breakdown": kern,
         "step_hbm_gbs_algorithmic": STEP_BYTES_PER_CLUMP * n / (wall / args.steps) / 1e9,
         "events_total_ms": total_ms,

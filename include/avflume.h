@@ -180,6 +180,9 @@ typedef struct avf_diag {
   int64_t list_builds;      /* candidate-list builds                                    */
   int64_t steps_undone;     /* candidate-list steps undone and redone after a rebuild    */
                             /* (a sphere moved more than 0.49 skin within their chunk)   */
+  int64_t closed_instances; /* instances whose list steps recompute clump mass and     */
+                            /* inertia from radius and density (avf_options.closed_forms */
+                            /* and the upload check); the others load them              */
 } avf_diag;
 
 /* Directed sphere contact (l <- m) of force_part.cu:157-209. */

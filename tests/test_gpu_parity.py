@@ -289,6 +289,8 @@ def test_closed_form_mass_and_inertia_change_no_bit(gpu, skin):
             ctx.step(30)
             pp, aa, anga = ctx.download()
             outs.append((pp, aa, anga))
+            # recomputed only where asked for and the upload check passed
+            assert ctx.diag()["closed_instances"] == int(closed == 1 and inst is not odd)
             ctx.close()
         for fld in STATE:
             assert np.array_equal(outs[0][0][fld].view(np.uint64), outs[1][0][fld].view(np.uint64)), fld
@@ -302,6 +304,7 @@ def test_closed_form_mass_and_inertia_change_no_bit(gpu, skin):
         ctx.initial_forces()
         ctx.step(30)
         outs.append([ctx.download(k) for k in range(len(batch))])
+        assert ctx.diag()["closed_instances"] == (2 if closed == 1 else 0)
         ctx.close()
     for a, b in zip(*outs):
         for fld in STATE:
@@ -324,6 +327,7 @@ def test_large_configuration_default_path(gpu):
         outs.append(ctx.download())
         d = ctx.diag()
         assert d["cell_below"] == 0 and d["cell_beyond"] == 0 and d["neigh_overflow"] == 0
+        assert d["closed_instances"] == int(closed == 0)
         ctx.close()
     for fld in STATE:
         assert np.array_equal(outs[0][0][fld].view(np.uint64), outs[1][0][fld].view(np.uint64)), fld
