@@ -1823,7 +1823,7 @@ __device__ __forceinline__ uint32_t tiled_cell_key(const InstParams &p, d3 q) {
   return (uint32_t)(p.sg_base + base + (ix - tx * TILE) +
                     (int64_t)wx * ((iy - ty * TILE) + (int64_t)hy * (iz - tz * TILE)));
 }
-constexpr int kReorderTile = 8;
+constexpr int kReorderTile = 16;
 
 __global__ void k_reorder_key(const double *__restrict__ st, const int32_t *__restrict__ inst_of,
                               const InstParams *__restrict__ P, uint32_t *__restrict__ counts,
