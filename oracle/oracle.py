@@ -1,9 +1,8 @@
 """ctypes bindings of the CPU oracle (TEST INFRASTRUCTURE ONLY).
 
-Loads oracle/liboracle.so (this repository's C restatement of the reference's host ops)
-and, where it was built, oracle/_ref/libref_hst.so (the reference's own *_hst ops compiled
-from their sources).  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
-use this module; the product never imports it.
+Loads oracle/liboracle.so (this repository's C restatement of the reference's host ops).
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this module; the
+product never imports it.
 """
 from __future__ import annotations
 
@@ -18,16 +17,13 @@ from avalancheflume3d_amd.abi import ptr
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, "liboracle.so")
-REF_LIB = os.path.join(HERE, "_ref", "libref_hst.so")
 
 _orc = None
-_ref = None
 
 
-def build(ref: bool = True) -> None:
-    """Compiles the oracle (and the reference library when its sources are present)."""
-    subprocess.run(["make", "-C", HERE, "oracle"] + (["ref"] if ref else []), check=True,
-                   capture_output=True)
+def build() -> None:
+    """Compiles the oracle."""
+    subprocess.run(["make", "-C", HERE, "oracle"], check=True, capture_output=True)
 
 
 def oracle_lib() -> C.CDLL:
@@ -72,30 +68,6 @@ def cell_table(cell: abi.Cell, sys: abi.System, pp: np.ndarray):
     rc = oracle_lib().orc_cell_locate(ptr(cell), ptr(s), ptr(np.ascontiguousarray(pp)),
                                       ptr(nocup), ptr(cell_vec), C.byref(bad))
     return cell_vec, nocup, rc, bad.value
-
-
-def ref_available() -> bool:
-    return os.path.exists(REF_LIB)
-
-
-def ref_lib() -> C.CDLL:
-    global _ref
-    if _ref is None:
-        lib = C.CDLL(REF_LIB)
-        vp = C.c_void_p
-        lib.ref_sim_create.restype = vp
-        lib.ref_sim_create.argtypes = [vp, vp, C.c_long, C.c_int, vp]
-        lib.ref_sim_destroy.argtypes = [vp]
-        lib.ref_sim_initial.argtypes = [vp]
-        lib.ref_sim_step.argtypes = [vp, C.c_long]
-        lib.ref_sim_open.argtypes = [vp, C.c_double]
-        lib.ref_sim_get.argtypes = [vp] * 8
-        lib.ref_rannew64.restype = C.c_double
-        lib.ref_rannew64.argtypes = [vp]
-        lib.ref_error_check.restype = C.c_int
-        lib.ref_error_check.argtypes = [vp, vp]
-        _ref = lib
-    return _ref
 
 
 class _Sim:
@@ -168,35 +140,6 @@ class OracleSim(_Sim):
     def __del__(self):
         if getattr(self, "h", None):
             self.lib.orc_sim_delete(self.h)
-            self.h = None
-
-
-class RefSim(_Sim):
-    """The reference's own _hst ops composed in the avalanche_hst.cu step order."""
-
-    def __init__(self, sys: abi.System, fun: abi.Funnel, ntags: int, phase: int, pp: np.ndarray):
-        super().__init__(sys, len(pp))
-        self.lib = ref_lib()
-        s = abi.System.from_buffer_copy(sys)
-        s.npart = len(pp)
-        self.h = self.lib.ref_sim_create(ptr(s), ptr(fun), ntags, phase,
-                                         ptr(np.ascontiguousarray(pp)))
-
-    def _get(self, *a):
-        self.lib.ref_sim_get(self.h, *a)
-
-    def initial(self):
-        self.lib.ref_sim_initial(self.h)
-
-    def step(self, k: int = 1):
-        self.lib.ref_sim_step(self.h, k)
-
-    def open(self, xfinal: float):
-        self.lib.ref_sim_open(self.h, C.c_double(xfinal))
-
-    def __del__(self):
-        if getattr(self, "h", None):
-            self.lib.ref_sim_destroy(self.h)
             self.h = None
 
 
