@@ -54,8 +54,8 @@ struct avf_ctx {
   bool deep = false;           // k_force gathers candidates ahead (small contexts)
   std::vector<hipEvent_t> prof_ev;  // timing events of the profiled / timed steps
   int32_t *d_orig = nullptr, *d_orig2 = nullptr, *d_inst = nullptr, *d_inst2 = nullptr;
-  int64_t *d_inv = nullptr, *d_ncells = nullptr, *d_tmp64 = nullptr;
-  uint32_t *d_counts = nullptr, *d_starts = nullptr, *d_bsum = nullptr;
+  int64_t *d_inv = nullptr, *d_ncells = nullptr, *d_nbricks = nullptr, *d_tmp64 = nullptr;
+  uint32_t *d_counts = nullptr, *d_starts = nullptr, *d_bsum = nullptr, *d_cursor = nullptr;
   uint32_t *d_skey = nullptr, *d_srank = nullptr, *d_sid = nullptr, *d_clist = nullptr;
   int32_t *d_ccnt = nullptr;
   double4 *d_spos = nullptr;
@@ -191,49 +191,65 @@ void reset_acc(std::vector<long long> &a, int ninst) {
   }
 }
 
-// Scan of the sphere-cell counts into starts (and counts back to zero).
-void launch_scan(avf_ctx *c) {
-  const unsigned nb = grid_for(c->cell_cap, kScanTile);
-  k_scan_a<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
-  k_scan_b<<<1, 1024, 0, c->stream>>>(c->d_bsum, c->d_ncells, c->d_starts);
-  k_scan_c<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_starts, c->d_ncells);
+// Workgroups of the two scan launches (each takes whole tiles of kScanTile cells).
+unsigned scan_grid(const avf_ctx *c) {
+  return std::max(1u, std::min((unsigned)kScanGrid, grid_for(c->cell_cap, kScanTile)));
 }
 
-// Binning of the staged spheres: predict (or stage), scan, scatter.
+// Scan of the sphere-cell counts into starts (counts back to zero).  finalize_prev /
+// advance: the parameter updates between two force stages of per-step binning
+// (step_params: the previous step's regrid and cc++, the next step's time advance).
+void launch_scan(avf_ctx *c, uint32_t *starts, int finalize_prev = 0, int advance = 0) {
+  const unsigned G = scan_grid(c);
+  k_count_sum<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
+  k_count_scan<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, starts, c->d_cursor,
+                                                c->d_ncells, c->d_P, c->d_acc_cell, c->ninst,
+                                                finalize_prev, advance);
+}
+
+// Spheres into cell order at the positions of c->d_cursor (left by the scan).  ccnt: the
+// per-sphere counts of the sweep that follows (those of clumps not binned are zeroed here);
+// NULL for a binning that no sweep reads.
+void launch_scatter(avf_ctx *c, const uint32_t *skey, const double *stage, double4 *spos,
+                    uint32_t *sid, int32_t *ccnt) {
+  k_scatter<<<std::max(1u, grid_for(4 * c->n, kScatterBlock)), kScatterBlock, 0, c->stream>>>(
+      skey, c->d_cursor, stage, spos, sid, c->d_inst, c->d_P, ccnt, 4 * c->n);
+}
+
+// Binning of the staged spheres: predict (mode 1, advancing the time) or stage (mode 0),
+// scan, scatter.
 void launch_bin(avf_ctx *c, int mode) {
   k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
       c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-      c->d_srank, c->n, c->cap, mode, ListTrack{nullptr, nullptr});
-  launch_scan(c);
-  k_scatter<<<std::max(1u, grid_for(4 * c->n, kBlock)), kBlock, 0, c->stream>>>(
-      c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid, c->d_inst, c->d_P, c->ninst,
-      4 * c->n, mode);
+      c->n, c->cap, mode, ListTrack{nullptr, nullptr});
+  launch_scan(c, c->d_starts, 0, mode == 1);
+  launch_scatter(c, c->d_skey, c->d_stage, c->d_spos, c->d_sid, c->d_ccnt);
 }
 
 ListTrack list_track(avf_ctx *c) { return ListTrack{c->d_dsp, c->d_wreach}; }
 
 template <int BS, bool VERLET, bool NEXT, bool LEAN = false, bool CLOSED = false,
-          bool DEEP = false>
+          bool DEEP = false, bool CELLS = false>
 void launch_force_bs(avf_ctx *c, int initial, bool ft, int time_ahead, const double *stage_in,
                      double *stage_out, int si) {
-  k_force<kMaxNeigh, BS, VERLET, NEXT, LEAN, CLOSED, DEEP>
+  k_force<kMaxNeigh, BS, VERLET, NEXT, LEAN, CLOSED, DEEP, CELLS>
       <<<grid_for(c->n, BS), BS, 0, c->stream>>>(
       c->d_st, c->d_sta, stage_in, c->d_inst, c->d_P, c->d_clist, c->d_ccnt, c->d_vl_m,
       c->d_vl_mask, c->d_vl_cnt, ft ? c->d_f : nullptr, c->d_diag, c->d_acc_cell, stage_out,
       c->d_orig, VERLET ? list_track(c) : ListTrack{nullptr, nullptr}, c->n, c->cap,
-      initial, time_ahead, si);
+      initial, time_ahead, si, c->d_counts, c->d_skey);
 }
 
-// Touching spheres of the staged spheres (skin 0) or candidate spheres within sep + skin.
+// Touching spheres of the binned spheres (skin 0) or candidate spheres within sep + skin.
 void launch_detect(avf_ctx *c, double skin = 0.0) {
   if (skin > 0.0)
-    k_detect<kSphListV><<<grid_for(4 * c->n, kBlock), kBlock, 0, c->stream>>>(
-        c->d_stage, c->d_P, c->d_inst, c->d_starts, c->d_spos, c->d_sid, c->d_clist, c->d_ccnt,
-        c->d_diag, 4 * c->n, skin);
+    k_sweep<kSphListV><<<kSwGrid, kSwBlock, 0, c->stream>>>(
+        c->d_P, c->ninst, c->d_nbricks, c->d_starts, c->d_spos, c->d_sid, c->d_clist,
+        c->d_ccnt, c->d_diag, skin);
   else
-    k_detect<kSphList><<<grid_for(4 * c->n, kBlock), kBlock, 0, c->stream>>>(
-        c->d_stage, c->d_P, c->d_inst, c->d_starts, c->d_spos, c->d_sid, c->d_clist, c->d_ccnt,
-        c->d_diag, 4 * c->n, 0.0);
+    k_sweep<kSphList><<<kSwGrid, kSwBlock, 0, c->stream>>>(
+        c->d_P, c->ninst, c->d_nbricks, c->d_starts, c->d_spos, c->d_sid, c->d_clist,
+        c->d_ccnt, c->d_diag, 0.0);
 }
 
 // k_force block shape: 64 (one wavefront) or 128 lanes (avf_options.block_clumps).  ft:
@@ -284,36 +300,64 @@ void launch_force(avf_ctx *c, int initial, bool ft = true, bool verlet = false,
   }
 }
 
-// One full step: avalanche_hst.cu:669-710 (or :439-467) plus the regrid and cc++.
-// ev (optional, AVF_NKERNELS + 1 events) brackets every kernel for avf_step_profiled.
-void launch_step(avf_ctx *c, hipEvent_t *ev = nullptr, bool ft = true) {
+// The force stage of per-step binning: own state from the stage record (stage_in), touching
+// spheres from the sweep; with stage_out, the next step's predict into that buffer with its
+// sphere-cell counts and ranks, otherwise the state back to st.
+void launch_cforce(avf_ctx *c, bool ft, const double *stage_in, double *stage_out) {
+  if (stage_out && c->closed)
+    launch_force_bs<kForceBlock, false, true, true, true, false, true>(c, 0, ft, 0, stage_in,
+                                                                       stage_out, 0);
+  else if (stage_out)
+    launch_force_bs<kForceBlock, false, true, true, false, false, true>(c, 0, ft, 0, stage_in,
+                                                                        stage_out, 0);
+  else if (c->closed)
+    launch_force_bs<kForceBlock, false, false, true, true, false, true>(c, 0, ft, 0, stage_in,
+                                                                        nullptr, 0);
+  else
+    launch_force_bs<kForceBlock, false, false, true, false, false, true>(c, 0, ft, 0, stage_in,
+                                                                         nullptr, 0);
+}
+
+// Step i of a chunk of S steps with the sphere cells rebuilt every step
+// (avalanche_hst.cu:669-710 or :439-467, plus the regrid and cc++).  The chunk's first step
+// predicts from st (k_predict, keys and LDS-staged counts); every step then scans the counts
+// (applying the previous step's regrid and cc++ and advancing the time), places the spheres
+// in cell order, sweeps the cells for touching spheres and runs the force stage, which also
+// predicts the next step into the other stage buffer with its keys and counts; the chunk's
+// last force stage writes the state back to st instead and one k_finalize applies its
+// regrid and cc++.  ev (optional, AVF_NKERNELS + 1 events) brackets the kernels in the slots
+// of avf_step_profiled (SCAN_A: count sums, SCAN_B: scan, SCAN_C: empty, DETECT: sweep).
+void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bool ft = true) {
   auto mark = [&](int k) {
     if (ev && (c->prof_kernel < 0 || k == c->prof_kernel || k == c->prof_kernel + 1))
       (void)hipEventRecord(ev[k], c->stream);
   };
-  const unsigned nb = grid_for(c->cell_cap, kScanTile);
+  double *buf[2] = {c->d_stage, c->d_stage2};
+  const double *cur = buf[i & 1];
+  const unsigned G = scan_grid(c);
   mark(AVF_K_PREDICT);
-  k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
-      c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-      c->d_srank, c->n, c->cap, 1, ListTrack{nullptr, nullptr});
+  if (i == 0)
+    k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
+        c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
+        c->n, c->cap, 1, ListTrack{nullptr, nullptr});
   mark(AVF_K_SCAN_A);
-  k_scan_a<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
+  k_count_sum<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
   mark(AVF_K_SCAN_B);
-  k_scan_b<<<1, 1024, 0, c->stream>>>(c->d_bsum, c->d_ncells, c->d_starts);
+  k_count_scan<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_starts,
+                                                c->d_cursor, c->d_ncells, c->d_P, c->d_acc_cell,
+                                                c->ninst, i > 0, 1);
   mark(AVF_K_SCAN_C);
-  k_scan_c<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_starts, c->d_ncells);
   mark(AVF_K_SCATTER);
-  k_scatter<<<std::max(1u, grid_for(4 * c->n, kBlock)), kBlock, 0, c->stream>>>(
-      c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid, c->d_inst, c->d_P, c->ninst,
-      4 * c->n, 1);
+  launch_scatter(c, c->d_skey, cur, c->d_spos, c->d_sid, c->d_ccnt);
   mark(AVF_K_DETECT);
   launch_detect(c);
   mark(AVF_K_FORCE);
-  launch_force(c, 0, ft);
+  launch_cforce(c, ft, cur, i + 1 < S ? buf[(i + 1) & 1] : nullptr);
   mark(AVF_K_CELLREDUCE);  // the regrid reduction runs in k_force
   mark(AVF_K_FINALIZE);
-  k_finalize<<<grid_for(c->ninst, 64), 64, 0, c->stream>>>(c->d_P, c->d_acc_cell, c->ninst, 0, 1,
-                                                           0);
+  if (i + 1 == S)
+    k_finalize<<<grid_for(c->ninst, 64), 64, 0, c->stream>>>(c->d_P, c->d_acc_cell, c->ninst, 0,
+                                                             1, 0, 1);
   mark(AVF_NKERNELS);
 }
 
@@ -333,24 +377,23 @@ void launch_vstep(avf_ctx *c, bool rebuild, hipEvent_t *ev = nullptr, bool ft = 
         (c->prof_kernel < 0 || k == c->prof_kernel || k == c->prof_kernel + 1))
       (void)hipEventRecord(ev[k], c->stream);
   };
-  const unsigned nb = grid_for(c->cell_cap, kScanTile);
+  const unsigned G = scan_grid(c);
   mark(AVF_K_PREDICT);
   if (rebuild || i == 0)
     k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
         c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-        c->d_srank, c->n, c->cap, rebuild ? 1 : 2, list_track(c));
+        c->n, c->cap, rebuild ? 1 : 2, list_track(c));
   mark(AVF_K_SCAN_A);
-  if (rebuild) k_scan_a<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
+  if (rebuild) k_count_sum<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
   mark(AVF_K_SCAN_B);
-  if (rebuild) k_scan_b<<<1, 1024, 0, c->stream>>>(c->d_bsum, c->d_ncells, c->d_starts);
+  if (rebuild)  // also advances the time (between the predict and the force stage)
+    k_count_scan<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_starts,
+                                                  c->d_cursor, c->d_ncells, c->d_P,
+                                                  c->d_acc_cell, c->ninst, 0, 1);
   mark(AVF_K_SCAN_C);
-  if (rebuild)
-    k_scan_c<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_starts, c->d_ncells);
   mark(AVF_K_SCATTER);
   if (rebuild)
-    k_scatter<<<std::max(1u, grid_for(4 * c->n, kBlock)), kBlock, 0, c->stream>>>(
-        c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid, c->d_inst, c->d_P, c->ninst,
-        4 * c->n, 1);
+    launch_scatter(c, c->d_skey, c->d_stage, c->d_spos, c->d_sid, c->d_ccnt);
   mark(AVF_K_DETECT);
   if (rebuild) {
     launch_detect(c, c->opt.verlet_skin);
@@ -384,11 +427,11 @@ int rebuild_grid(avf_ctx *c, bool reorder) {
   const double horizon = 2.0 * std::max(1, c->opt.reorder_interval);
   k_sgrid_finalize<<<1, 1, 0, c->stream>>>(c->d_P, c->d_acc_sg, c->ninst, c->cell_cap,
                                            c->opt.sphere_cell_factor, horizon, c->d_ncells,
-                                           c->opt.verlet_skin);
+                                           c->opt.verlet_skin, c->d_nbricks);
   if (reorder) {
     k_reorder_key<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(
         c->d_st, c->d_inst, c->d_P, c->d_counts, c->d_skey, c->d_srank, c->n, c->cap);
-    launch_scan(c);
+    launch_scan(c, c->d_starts);
     k_reorder_move<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(
         c->d_skey, c->d_srank, c->d_starts, c->d_st, c->d_st2, c->d_sta, c->d_sta2, c->d_orig,
         c->d_orig2, c->d_inst, c->d_inst2, c->d_P, c->d_inv, c->n, c->cap);
@@ -409,31 +452,10 @@ int rebuild_grid(avf_ctx *c, bool reorder) {
   return AVF_OK;
 }
 
-int run_steps(avf_ctx *c, int64_t k) {
-  if (k <= 0) return AVF_OK;
-  if (!c->opt.use_graph) {
-    for (int64_t i = 0; i < k; i++) launch_step(c, nullptr, i + 1 == k);
-    HIPCHK(c, hipGetLastError());
-    return AVF_OK;
-  }
-  auto it = c->graphs.find(k);
-  if (it == c->graphs.end()) {
-    hipGraph_t g;
-    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int64_t i = 0; i < k; i++) launch_step(c, nullptr, i + 1 == k);
-    HIPCHK(c, hipStreamEndCapture(c->stream, &g));
-    hipGraphExec_t ge;
-    HIPCHK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-    c->graph_defs.push_back(g);
-    it = c->graphs.emplace(k, ge).first;
-  }
-  HIPCHK(c, hipGraphLaunch(it->second, c->stream));
-  return AVF_OK;
-}
-
 // Per-kernel event sums of avf_step_profiled.
 struct StepProf {
   double sum[AVF_NKERNELS] = {0};
+  int64_t count[AVF_NKERNELS] = {0};  // timed launches per slot (per-step binning)
   int64_t launched = 0;
 };
 
@@ -648,6 +670,87 @@ int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
   return AVF_OK;
 }
 
+// S steps of per-step binning launched eagerly with events between the kernels of the
+// steps that are timed (every prof_every-th; for avf_step_timed only the brackets of
+// prof_kernel).
+int csteps_profiled(avf_ctx *c, int64_t S, StepProf *prof) {
+  constexpr int kEv = AVF_NKERNELS + 1;
+  hipEvent_t *ev = nullptr;
+  int rc = prof_events(c, (size_t)S * kEv, &ev);
+  if (rc) return rc;
+  std::vector<int64_t> timed;
+  for (int64_t i = 0; i < S; i++) {
+    const bool rec = c->prof_every <= 1 || c->prof_seq % c->prof_every == 0;
+    c->prof_seq++;
+    if (rec) timed.push_back(i);
+    launch_cstep(c, i, S, rec ? &ev[(size_t)i * kEv] : nullptr, i + 1 == S);
+  }
+  const int jlast = c->prof_kernel >= 0 ? c->prof_kernel + 1 : AVF_NKERNELS;
+  hipError_t he = hipSuccess;
+  if (!timed.empty()) he = hipEventSynchronize(ev[(size_t)timed.back() * kEv + jlast]);
+  for (size_t t = 0; t < timed.size() && he == hipSuccess; t++)
+    for (int j = 0; j < AVF_NKERNELS && he == hipSuccess; j++) {
+      if (c->prof_kernel >= 0 && j != c->prof_kernel) continue;
+      // slots of kernels this step does not launch stay out of the sums
+      if (j == AVF_K_SCAN_C || j == AVF_K_CELLREDUCE) continue;
+      if (j == AVF_K_PREDICT && timed[t] != 0) continue;
+      if (j == AVF_K_FINALIZE && timed[t] + 1 != S) continue;
+      const size_t i = (size_t)timed[t];
+      float ms = 0.f;
+      he = hipEventElapsedTime(&ms, ev[i * kEv + j], ev[i * kEv + j + 1]);
+      prof->sum[j] += ms;
+      prof->count[j] += 1;
+    }
+  if (he != hipSuccess) return fail(c, AVF_ERR_HIP, "event timing: %s", hipGetErrorString(he));
+  prof->launched += (int64_t)timed.size();
+  return AVF_OK;
+}
+
+// Steps with the sphere cells rebuilt every step (avf_options.verlet_skin == 0), in chunks
+// that end at the spatial reorders; a chunk runs as captured graphs of at most kGraphSteps
+// steps each (cached by chunk length and offset) unless profiled or use_graph is off.
+int run_cells(avf_ctx *c, int64_t nsteps, StepProf *prof) {
+  constexpr int64_t kGraphSteps = 64;
+  const int64_t chunk_max = c->opt.reorder_interval > 0 ? c->opt.reorder_interval : 1024;
+  int rc;
+  int64_t left = nsteps;
+  while (left > 0) {
+    if (!c->sgrid_valid ||
+        (c->opt.reorder_interval > 0 && c->steps_since_reorder >= c->opt.reorder_interval)) {
+      if ((rc = rebuild_grid(c, c->opt.reorder_interval > 0))) return rc;
+    }
+    const int64_t S = std::min(left, chunk_max - (c->steps_since_reorder % chunk_max));
+    if (prof) {
+      if ((rc = csteps_profiled(c, S, prof))) return rc;
+    } else if (c->opt.use_graph) {
+      for (int64_t off = 0; off < S; off += kGraphSteps) {
+        const int64_t len = std::min(kGraphSteps, S - off);
+        const int64_t key = (int64_t{1} << 50) + S * 65536 + off;
+        auto it = c->graphs.find(key);
+        if (it == c->graphs.end()) {
+          hipGraph_t g;
+          HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+          for (int64_t i = off; i < off + len; i++) launch_cstep(c, i, S, nullptr, i + 1 == S);
+          HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+          hipGraphExec_t ge;
+          HIPCHK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+          c->graph_defs.push_back(g);
+          it = c->graphs.emplace(key, ge).first;
+        }
+        HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+      }
+    } else {
+      for (int64_t i = 0; i < S; i++) launch_cstep(c, i, S, nullptr, i + 1 == S);
+    }
+    HIPCHK(c, hipGetLastError());
+    c->binned = true;
+    c->steps_since_reorder += S;
+    c->steps_total += S;
+    left -= S;
+  }
+  return AVF_OK;
+}
+
 int check_inst(avf_ctx *c, int32_t inst) {
   if (!c) return AVF_ERR_ARG;
   if (inst < 0 || inst >= c->ninst) return fail(c, AVF_ERR_ARG, "instance %d out of range", inst);
@@ -751,8 +854,10 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
       (rc = dalloc(c, &c->d_orig, N)) || (rc = dalloc(c, &c->d_orig2, N)) ||
       (rc = dalloc(c, &c->d_inst, N)) || (rc = dalloc(c, &c->d_inst2, N)) ||
       (rc = dalloc(c, &c->d_inv, N)) || (rc = dalloc(c, &c->d_ncells, 1)) ||
+      (rc = dalloc(c, &c->d_nbricks, 1)) || (rc = dalloc(c, &c->d_stage2, N * kStageDoubles)) ||
       (rc = dalloc(c, &c->d_tmp64, N)) || (rc = dalloc(c, &c->d_counts, c->cell_cap)) ||
       (rc = dalloc(c, &c->d_starts, c->cell_cap + 1)) ||
+      (rc = dalloc(c, &c->d_cursor, c->cell_cap)) ||
       (rc = dalloc(c, &c->d_bsum, grid_for(c->cell_cap, kScanTile) + 1)) ||
       (rc = dalloc(c, &c->d_skey, 4 * N)) || (rc = dalloc(c, &c->d_srank, 4 * N)) ||
       (rc = dalloc(c, &c->d_sid, 4 * N)) || (rc = dalloc(c, &c->d_spos, 4 * N)) ||
@@ -766,7 +871,7 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
        (rc = dalloc(c, &c->d_vl_cnt, N)) || (rc = dalloc(c, &c->d_dsp, N)) || (rc = dalloc(c, &c->d_wreach, N)) ||
        (rc = dalloc(c, &c->d_ck_st, N * S_NFIELDS)) || (rc = dalloc(c, &c->d_ck_rad, N)) ||
        (rc = dalloc(c, &c->d_ck_P, ninst)) || (rc = dalloc(c, &c->d_ck_diag, 1)) ||
-       (rc = dalloc(c, &c->d_umax, 1)) || (rc = dalloc(c, &c->d_stage2, N * kStageDoubles))))
+       (rc = dalloc(c, &c->d_umax, 1))))
     return bail(rc);
 
   // Parameters, identity storage order, zeroed tables.
@@ -804,7 +909,8 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
       (he = hipMemset(c->d_sta, 0, N * C_NFIELDS * sizeof(double))) != hipSuccess ||
       (he = hipMemset(c->d_f, 0, N * F_NFIELDS * sizeof(double))) != hipSuccess ||
       (he = hipMemset(c->d_diag, 0, sizeof(DevDiag))) != hipSuccess ||
-      (he = hipMemset(c->d_ncells, 0, sizeof(int64_t))) != hipSuccess) {
+      (he = hipMemset(c->d_ncells, 0, sizeof(int64_t))) != hipSuccess ||
+      (he = hipMemset(c->d_nbricks, 0, sizeof(int64_t))) != hipSuccess) {
     fail(c, AVF_ERR_HIP, "hipMemset: %s", hipGetErrorString(he));
     return bail(AVF_ERR_HIP);
   }
@@ -865,7 +971,8 @@ int avf_destroy(avf_ctx *c) {
   for (auto g : c->graph_defs) (void)hipGraphDestroy(g);
   void *bufs[] = {c->d_P,      c->d_st,    c->d_st2,   c->d_sta,   c->d_sta2,  c->d_stage,
                   c->d_f,      c->d_io,    c->d_io2,   c->d_orig,  c->d_orig2, c->d_inst,
-                  c->d_inst2,  c->d_inv,   c->d_ncells, c->d_tmp64, c->d_counts, c->d_starts,
+                  c->d_inst2,  c->d_inv,   c->d_ncells, c->d_nbricks, c->d_tmp64, c->d_counts,
+                  c->d_starts, c->d_cursor,
                   c->d_bsum,   c->d_skey,  c->d_srank, c->d_sid,   c->d_spos,  c->d_acc_cell,
                   c->d_acc_sg, c->d_diag,  c->d_count, c->d_clist, c->d_ccnt,
                   c->d_vl_m,   c->d_vl_cnt, c->d_vl_mask, c->d_dsp, c->d_wreach,
@@ -1059,106 +1166,30 @@ int avf_initial_forces(avf_ctx *c) {
 int avf_step(avf_ctx *c, int64_t nsteps) {
   if (!c || nsteps < 0) return AVF_ERR_ARG;
   if (c->opt.verlet_skin > 0.0) return run_verlet(c, nsteps, nullptr);
-  int rc;
-  const int64_t chunk_max = c->opt.reorder_interval > 0 ? c->opt.reorder_interval : 1000;
-  int64_t left = nsteps;
-  while (left > 0) {
-    if (!c->sgrid_valid ||
-        (c->opt.reorder_interval > 0 && c->steps_since_reorder >= c->opt.reorder_interval)) {
-      if ((rc = rebuild_grid(c, c->opt.reorder_interval > 0))) return rc;
-    }
-    int64_t k = std::min(left, chunk_max - (c->steps_since_reorder % chunk_max));
-    if ((rc = run_steps(c, k))) return rc;
-    c->binned = true;
-    c->steps_since_reorder += k;
-    c->steps_total += k;
-    left -= k;
-  }
-  return AVF_OK;
+  return run_cells(c, nsteps, nullptr);
 }
 
 int avf_step_profiled(avf_ctx *c, int64_t nsteps, double *kernel_ms, double *total_ms) {
   if (!c || nsteps < 0) return AVF_ERR_ARG;
-  if (c->opt.verlet_skin > 0.0) {
-    hipEvent_t first, last;
-    HIPCHK(c, hipEventCreate(&first));
-    HIPCHK(c, hipEventCreate(&last));
-    HIPCHK(c, hipEventRecord(first, c->stream));
-    StepProf prof;
-    int rc = run_verlet(c, nsteps, &prof);
-    HIPCHK(c, hipEventRecord(last, c->stream));
-    HIPCHK(c, hipEventSynchronize(last));
-    float tot = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&tot, first, last));
-    (void)hipEventDestroy(first);
-    (void)hipEventDestroy(last);
-    for (int j = 0; j < AVF_NKERNELS; j++)
-      if (kernel_ms) kernel_ms[j] = prof.launched ? prof.sum[j] / (double)prof.launched : 0.0;
-    if (total_ms) *total_ms = tot;
-    return rc;
-  }
-  // Two batches of events: batch b+1 is queued before batch b is read, so reading the
-  // timings never leaves the GPU idle.
-  constexpr int kBatch = 128, kEv = AVF_NKERNELS + 1;
-  std::vector<hipEvent_t> ev((size_t)2 * kBatch * kEv);
-  for (auto &e : ev) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+  const bool verlet = c->opt.verlet_skin > 0.0;
   hipEvent_t first, last;
   HIPCHK(c, hipEventCreate(&first));
   HIPCHK(c, hipEventCreate(&last));
-  double sum[AVF_NKERNELS] = {0};
-  int rc = AVF_OK;
-  int64_t queued = 0;
-  int64_t batch_len[2] = {0, 0};
-  int cur = 0;
-  bool started = false;
-  auto read_batch = [&](int b) -> int {
-    if (batch_len[b] == 0) return AVF_OK;
-    hipEvent_t *base = &ev[(size_t)b * kBatch * kEv];
-    const int jlast = c->prof_kernel >= 0 ? c->prof_kernel + 1 : AVF_NKERNELS;
-    HIPCHK(c, hipEventSynchronize(base[(batch_len[b] - 1) * kEv + jlast]));
-    for (int64_t i = 0; i < batch_len[b]; i++)
-      for (int j = 0; j < AVF_NKERNELS; j++) {
-        if (c->prof_kernel >= 0 && j != c->prof_kernel) continue;
-        float ms = 0.f;
-        HIPCHK(c, hipEventElapsedTime(&ms, base[i * kEv + j], base[i * kEv + j + 1]));
-        sum[j] += ms;
-      }
-    batch_len[b] = 0;
-    return AVF_OK;
-  };
-  while (queued < nsteps && rc == AVF_OK) {
-    const int64_t k = std::min<int64_t>(kBatch, nsteps - queued);
-    hipEvent_t *base = &ev[(size_t)cur * kBatch * kEv];
-    for (int64_t i = 0; i < k; i++) {
-      if (!c->sgrid_valid ||
-          (c->opt.reorder_interval > 0 && c->steps_since_reorder >= c->opt.reorder_interval)) {
-        if ((rc = rebuild_grid(c, c->opt.reorder_interval > 0))) break;
-      }
-      if (!started) {
-        HIPCHK(c, hipEventRecord(first, c->stream));
-        started = true;
-      }
-      launch_step(c, &base[i * kEv], queued + i + 1 == nsteps);
-      c->steps_since_reorder++;
-      c->steps_total++;
-      batch_len[cur]++;
-    }
-    HIPCHK(c, hipGetLastError());
-    queued += k;
-    cur ^= 1;
-    if ((rc = read_batch(cur))) break;  // the previous batch, while this one runs
-  }
-  if (rc == AVF_OK) rc = read_batch(cur ^ 1);
+  HIPCHK(c, hipEventRecord(first, c->stream));
+  StepProf prof;
+  int rc = verlet ? run_verlet(c, nsteps, &prof) : run_cells(c, nsteps, &prof);
   HIPCHK(c, hipEventRecord(last, c->stream));
   HIPCHK(c, hipEventSynchronize(last));
   float tot = 0.f;
-  if (started) HIPCHK(c, hipEventElapsedTime(&tot, first, last));
-  for (auto &e : ev) (void)hipEventDestroy(e);
+  HIPCHK(c, hipEventElapsedTime(&tot, first, last));
   (void)hipEventDestroy(first);
   (void)hipEventDestroy(last);
-  c->binned = true;
-  for (int j = 0; j < AVF_NKERNELS; j++)
-    if (kernel_ms) kernel_ms[j] = nsteps ? sum[j] / (double)nsteps : 0.0;
+  // mean duration per launch of each kernel (per-step binning: over the timed launches of
+  // that kernel; candidate lists: per timed step, slots of unlaunched kernels near 0)
+  for (int j = 0; j < AVF_NKERNELS; j++) {
+    const int64_t cnt = verlet ? prof.launched : prof.count[j];
+    if (kernel_ms) kernel_ms[j] = cnt ? prof.sum[j] / (double)cnt : 0.0;
+  }
   if (total_ms) *total_ms = tot;
   return rc;
 }
@@ -1576,15 +1607,11 @@ void win_step(avf_ctx *c, const InstParams *P, const double *cur, double *out, d
 // context's own binning stays as it is.
 void win_bin(avf_ctx *c, const double *st) {
   WinBuf *w = c->win;
-  const unsigned nb = grid_for(c->cell_cap, kScanTile);
   k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
       const_cast<double *>(st), c->d_sta, w->bstage, c->d_inst, c->d_orig, c->d_P, c->d_counts,
-      w->bkey, w->brank, c->n, c->cap, 0, ListTrack{nullptr, nullptr});
-  k_scan_a<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
-  k_scan_b<<<1, 1024, 0, c->stream>>>(c->d_bsum, c->d_ncells, w->bstarts);
-  k_scan_c<<<nb, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, w->bstarts, c->d_ncells);
-  k_scatter<<<std::max(1u, grid_for(4 * c->n, kBlock)), kBlock, 0, c->stream>>>(
-      w->bkey, w->brank, w->bstarts, w->bstage, w->bspos, w->bsid, c->d_inst, c->d_P, c->ninst, 4 * c->n, 0);
+      w->bkey, c->n, c->cap, 0, ListTrack{nullptr, nullptr});
+  launch_scan(c, w->bstarts);
+  launch_scatter(c, w->bkey, w->bstage, w->bspos, w->bsid, nullptr);
 }
 
 // Adds the touching sphere pairs of state st that the frozen list lacks to w->missed.

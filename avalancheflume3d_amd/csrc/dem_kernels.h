@@ -29,6 +29,7 @@ struct InstParams {
   double sg_lo[3], sg_inv_h, sg_h, sg_reach, sg_rsph;
   int32_t sg_n[3], pad0;
   int64_t sg_base;
+  int64_t sg_bbase;  // first brick of this instance in the contact sweep's brick range
   // storage range and bookkeeping
   int64_t begin, count;
   int64_t cc;

@@ -173,27 +173,54 @@ __device__ __forceinline__ void fill_stage(double *__restrict__ g, d3 rr, double
 // Candidate-list displacement bound of clump c since the list build, accumulated step by
 // step: sum of |x' - x| + sqrt(3) pos |dOmega| (a sphere sits sqrt(3) pos from the centre,
 // and a rotation by |dOmega| moves it at most that times as far), which is never below
-// the distance any sphere has actually moved.  Frozen clumps do not move.
+// the distance any sphere has actually moved.  Frozen clumps do not move.  No-op without
+// candidate lists (lt.dsp null).
 __device__ __forceinline__ void track_update(ListTrack lt, int64_t c, d3 r0, d3 r1, double pos,
                                              double rot_step) {
+  if (!lt.dsp) return;
   const double dx = r1.x - r0.x, dy = r1.y - r0.y, dz = r1.z - r0.z;
   const double d = sqrt(dx * dx + dy * dy + dz * dz) + 1.7320508075688772 * pos * rot_step;
   if (d > 0.0) lt.dsp[c] += d;
 }
 
-// mode 1: predict (init_Verlet) then stage and sphere-cell keys; mode 2: the same predict
-// and stage without keys (candidate-list step), updating the displacement bound; mode 0:
-// stage the current state only (initial force pass, avalanche_hst.cu:365-384, whose force
-// stage sees the unchanged time).  t_old: the time at the start of the step (P.time, or
+// Counting-sort counts of a clump's four sphere-cell keys (one atomic per binned sphere,
+// whose result is not used: no wait on its return; the positions within a cell are taken
+// later by k_scatter) and the keys themselves.  Measured against staging the counts of a
+// workgroup in an LDS hash table first (one atomic per distinct cell): the staging cost more
+// than the atomics it saved, in the force kernel (49.5 -> 63 us per launch at 250,000 clumps)
+// and in the scatter (23.5 -> 42.6 us).
+__device__ __forceinline__ void count_keys(const uint32_t *keys, uint32_t *__restrict__ counts,
+                                           uint32_t *__restrict__ skey, int64_t c) {
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+    if (keys[s] != kInvalidKey) atomicAdd(&counts[keys[s]], 1u);
+  reinterpret_cast<uint4 *>(skey)[c] = make_uint4(keys[0], keys[1], keys[2], keys[3]);
+}
+
+// Sphere-cell keys of a clump's four spheres at the force-stage time t_new.  Sphere centres
+// always use the radius the clump had before it froze (rad0): a frozen clump's centres are
+// the stale ones of the step it crossed side.x (rr, R unchanged since).  Clumps past side.x
+// at t_new are not binned (cells.cu:65): kInvalidKey.
+__device__ __forceinline__ void clump_keys(const InstParams &p, d3 rr, const double *R,
+                                           double rad0, double t_new, uint32_t *keys) {
+  d3 sph[4];
+  sphere_centres(rr, R, rad0 / p.sphere_div, sph);
+  const bool out = frozen(t_new, rr.x, p.side_x);
+#pragma unroll
+  for (int s = 0; s < 4; s++) keys[s] = out ? kInvalidKey : sphere_key(p, sph[s]);
+}
+
+// mode 1: predict (init_Verlet) then stage and sphere-cell keys (into keys[4]); mode 2: the
+// same predict and stage without keys (candidate-list step), updating the displacement
+// bound; mode 0: stage the current state only (initial force pass, avalanche_hst.cu:365-384,
+// whose force stage sees the unchanged time), with keys.  t_old: the time at the start of the step (P.time, or
 // within a chunk of list steps chunk_time(), which has the same sign: only comparisons of
 // the time with 0 are made here).
 __device__ __forceinline__ void predict_clump(double *__restrict__ st, double *__restrict__ sta,
                                               double *__restrict__ g, const InstParams &p,
-                                              const int32_t *__restrict__ orig,
-                                              uint32_t *__restrict__ counts,
-                                              uint32_t *__restrict__ skey,
-                                              uint32_t *__restrict__ srank, int64_t c, int64_t cap,
-                                              int mode, ListTrack lt, double t_old) {
+                                              const int32_t *__restrict__ orig, uint32_t *keys,
+                                              int64_t c, int64_t cap, int mode, ListTrack lt,
+                                              double t_old) {
   const double t_new = mode ? t_old + p.dt : t_old;
 
   d3 rr = make_d3(st[S_RX * cap + c], st[S_RY * cap + c], st[S_RZ * cap + c]);
@@ -236,22 +263,7 @@ __device__ __forceinline__ void predict_clump(double *__restrict__ st, double *_
     return;
   }
 
-  // Sphere centres always use the radius the clump had before it froze: a frozen clump's
-  // centres are the stale ones of the step it crossed side.x (rr, R unchanged since).
-  d3 sph[4];
-  sphere_centres(rr, R, rad0 / p.sphere_div, sph);
-  // Binning at the force-stage time: clumps past side.x are left out (cells.cu:65).
-  const bool out = frozen(t_new, rr.x, p.side_x);
-#pragma unroll
-  for (int s = 0; s < 4; s++) {
-    uint32_t key = kInvalidKey, rank = 0;
-    if (!out) {
-      key = sphere_key(p, sph[s]);
-      rank = atomicAdd(&counts[key], 1u);
-    }
-    skey[4 * c + s] = key;
-    srank[4 * c + s] = rank;
-  }
+  clump_keys(p, rr, R, rad0, t_new, keys);
 }
 
 // A clump's state in registers: its stage record as read at the start of a list step
@@ -295,21 +307,23 @@ __global__ __launch_bounds__(kPredictBlock) void k_predict(
     double *__restrict__ st, double *__restrict__ sta, double *__restrict__ stage,
     const int32_t *__restrict__ inst_of, const int32_t *__restrict__ orig,
     const InstParams *__restrict__ P, uint32_t *__restrict__ counts, uint32_t *__restrict__ skey,
-    uint32_t *__restrict__ srank, int64_t n, int64_t cap, int mode, ListTrack lt) {
+    int64_t n, int64_t cap, int mode, ListTrack lt) {
   __shared__ double rec[kPredictBlock * kStagePad];
   const int64_t c0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kPredictBlock;
   const int64_t c = c0 + threadIdx.x;
+  uint32_t keys[4] = {kInvalidKey, kInvalidKey, kInvalidKey, kInvalidKey};
   if (c < n) {
     const int inst = inst_of[c];
     while (true) {
       const int cur = __builtin_amdgcn_readfirstlane(inst);
       if (inst == cur) {
-        predict_clump(st, sta, rec + threadIdx.x * kStagePad, P[cur], orig, counts, skey, srank,
-                      c, cap, mode, lt, P[cur].time);
+        predict_clump(st, sta, rec + threadIdx.x * kStagePad, P[cur], orig, keys, c, cap, mode,
+                      lt, P[cur].time);
         break;
       }
     }
   }
+  if (mode != 2 && c < n) count_keys(keys, counts, skey, c);
   __syncthreads();
   const int64_t rows = n - c0 < kPredictBlock ? n - c0 : kPredictBlock;
   double *dst = stage + c0 * kStageDoubles;
@@ -318,6 +332,13 @@ __global__ __launch_bounds__(kPredictBlock) void k_predict(
 }
 
 // ------------------------------------------------------------------------- scan -----
+//
+// Exclusive scan of the sphere-cell counts into starts, in two launches of a fixed grid of
+// kScanGrid workgroups over the device-side cell count: k_count_sum sums each workgroup's
+// share of the cells, k_count_scan scans the kScanGrid sums in every workgroup, then its
+// share of the cells tile by tile (writing starts, zeroing the counts for the next binning).
+
+constexpr int kScanGrid = 1024;
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
   const int lane = threadIdx.x & 63;
@@ -347,89 +368,61 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total,
   return off + x - v;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_a(const uint32_t *__restrict__ counts,
-                                                       uint32_t *__restrict__ bsum,
-                                                       const int64_t *__restrict__ ncells_p) {
+// Cell range [lo, hi) of workgroup b: whole tiles of kScanTile cells, the same number of
+// tiles for every workgroup.
+__device__ __forceinline__ void scan_share(int64_t ncells, int64_t b, int64_t nb, int64_t &lo,
+                                           int64_t &hi) {
+  const int64_t tiles = (ncells + kScanTile - 1) / kScanTile;
+  const int64_t per = (tiles + nb - 1) / nb;
+  lo = b * per * kScanTile;
+  hi = lo + per * kScanTile;
+  if (lo > ncells) lo = ncells;
+  if (hi > ncells) hi = ncells;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_count_sum(const uint32_t *__restrict__ counts,
+                                                          uint32_t *__restrict__ bsum,
+                                                          const int64_t *__restrict__ ncells_p) {
   __shared__ uint32_t lds[kScanBlock / 64];
-  const int64_t ncells = *ncells_p;
-  const int64_t base = (int64_t)blockIdx.x * kScanTile;
-  if (base >= ncells) return;
-  const int64_t i0 = base + (int64_t)threadIdx.x * kScanItems;
-  uint32_t s = 0;
-#pragma unroll
-  for (int k = 0; k < kScanItems; k++)
-    if (i0 + k < ncells) s += counts[i0 + k];
+  int64_t lo, hi;
+  scan_share(*ncells_p, blockIdx.x, gridDim.x, lo, hi);
+  uint32_t s = 0u;
+  // lo is a multiple of kScanTile: 16-byte loads, coalesced across the workgroup
+  for (int64_t i = lo + 4 * (int64_t)threadIdx.x; i < hi; i += 4 * kScanBlock) {
+    if (i + 3 < hi) {
+      const uint4 v = *reinterpret_cast<const uint4 *>(counts + i);
+      s += v.x + v.y + v.z + v.w;
+    } else {
+      for (int64_t k = i; k < hi; k++) s += counts[k];
+    }
+  }
   uint32_t tot;
   block_excl_scan(s, &tot, lds);
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-// One block: exclusive scan of the tile sums; also advances time when requested (between
-// the predict pass that computed time + dt and the force pass that uses it).
-__global__ __launch_bounds__(1024) void k_scan_b(uint32_t *__restrict__ bsum,
-                                                 const int64_t *__restrict__ ncells_p,
-                                                 uint32_t *__restrict__ starts) {
-  __shared__ uint32_t lds[1024 / 64];
-  const int64_t ncells = *ncells_p;
-  const int64_t nb = (ncells + kScanTile - 1) / kScanTile;
-  const int64_t per = (nb + blockDim.x - 1) / blockDim.x;
-  const int64_t j0 = (int64_t)threadIdx.x * per;
-  uint32_t s = 0;
-  for (int64_t k = 0; k < per; k++)
-    if (j0 + k < nb) s += bsum[j0 + k];
-  uint32_t tot;
-  uint32_t off = block_excl_scan(s, &tot, lds);
-  for (int64_t k = 0; k < per; k++)
-    if (j0 + k < nb) {
-      uint32_t v = bsum[j0 + k];
-      bsum[j0 + k] = off;
-      off += v;
-    }
-  if (threadIdx.x == 0) starts[ncells] = tot;
-}
-
-__global__ __launch_bounds__(kScanBlock) void k_scan_c(uint32_t *__restrict__ counts,
-                                                       const uint32_t *__restrict__ boff,
-                                                       uint32_t *__restrict__ starts,
-                                                       const int64_t *__restrict__ ncells_p) {
-  __shared__ uint32_t lds[kScanBlock / 64];
-  const int64_t ncells = *ncells_p;
-  const int64_t base = (int64_t)blockIdx.x * kScanTile;
-  if (base >= ncells) return;
-  const int64_t i0 = base + (int64_t)threadIdx.x * kScanItems;
-  uint32_t v[kScanItems], s = 0;
-#pragma unroll
-  for (int k = 0; k < kScanItems; k++) {
-    v[k] = (i0 + k < ncells) ? counts[i0 + k] : 0u;
-    s += v[k];
-  }
-  uint32_t tot;
-  uint32_t off = block_excl_scan(s, &tot, lds) + boff[blockIdx.x];
-#pragma unroll
-  for (int k = 0; k < kScanItems; k++)
-    if (i0 + k < ncells) {
-      starts[i0 + k] = off;
-      off += v[k];
-      counts[i0 + k] = 0u;
-    }
-}
-
 // ----------------------------------------------------------------------- scatter ----
 
-__global__ void k_scatter(const uint32_t *__restrict__ skey, const uint32_t *__restrict__ srank,
-                          const uint32_t *__restrict__ starts, const double *__restrict__ stage,
-                          double4 *__restrict__ spos, uint32_t *__restrict__ sid,
-                          const int32_t *__restrict__ inst_of, InstParams *__restrict__ P,
-                          int32_t ninst, int64_t nsph, int advance) {
-  int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (advance && i < ninst) {  // ninst <= clumps < spheres, so every instance is covered
-    InstParams &p = P[i];
-    p.time = p.time + p.dt;  // avalanche_hst.cu:672 (sys_par.time += sys_par.dt)
-  }
+// One lane per sphere: the record (x, y, z, r) + id 4 m + s of each binned sphere at the next
+// free position of its cell, taken from cursor (the scanned starts, copied by k_count_scan)
+// with an atomic.  The centre is the predict's (sphere_of on the stage record's centre and
+// rotation is bit-identical to sphere_centres).  The sweep lists the touching spheres of
+// binned spheres only; ccnt (if given) of a sphere that is not binned (its clump is past
+// side.x) is set to 0 here, so every per-sphere count is written each binning.
+constexpr int kScatterBlock = 256;
+__global__ __launch_bounds__(kScatterBlock) void k_scatter(
+    const uint32_t *__restrict__ skey, uint32_t *__restrict__ cursor,
+    const double *__restrict__ stage, double4 *__restrict__ spos, uint32_t *__restrict__ sid,
+    const int32_t *__restrict__ inst_of, const InstParams *__restrict__ P,
+    int32_t *__restrict__ ccnt, int64_t nsph) {
+  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kScatterBlock + threadIdx.x;
   if (i >= nsph) return;
-  uint32_t key = skey[i];
-  if (key == kInvalidKey) return;
-  uint32_t dst = starts[key] + srank[i];
+  const uint32_t key = skey[i];
+  if (key == kInvalidKey) {
+    if (ccnt) ccnt[i] = 0;
+    return;
+  }
+  const uint32_t dst = atomicAdd(&cursor[key], 1u);
   const double *g = stage + (i >> 2) * kStageDoubles;
   const double rad = g[G_RAD];  // binned: not frozen, the pre-freeze radius
   const d3 q = rec_sphere(g, rad / P[inst_of[i >> 2]].sphere_div, (int)(i & 3));
@@ -733,76 +726,9 @@ __device__ __forceinline__ bool ref_cell_ijk(const InstParams &p, d3 r, int64_t 
   return true;
 }
 
-constexpr int kSphList = 16;  // touching spheres of other clumps per sphere
+constexpr int kSphList = 16;  // touching spheres of other clumps per sphere (k_sweep)
 
-// Contact detection, one lane per sphere (lane 4c + s is sphere s of clump c): the 27
-// sphere cells around the sphere, 9 contiguous rows of 3 cells whose 18 bounds are loaded
-// up front; every touching sphere of another clump is listed as (m << 2 | sm).  The test
-// is force_part.cu:140-157's ((dx^2 + dy^2) + dz^2) < sep^2 on the same operands, so the
-// listed set is exactly the set force_part evaluates.
-//
-// With skin > 0 (candidate lists) the test is d^2 < (sep + skin)^2 and CAP ids are kept per
-// sphere; the sphere cells are then at least 2 r_max + skin wide, so 27 cells still cover.
-template <int CAP>
-__global__ __launch_bounds__(256) void k_detect(
-    const double *__restrict__ stage, const InstParams *__restrict__ P,
-    const int32_t *__restrict__ inst_of, const uint32_t *__restrict__ starts,
-    const double4 *__restrict__ spos, const uint32_t *__restrict__ sid,
-    uint32_t *__restrict__ clist, int32_t *__restrict__ ccnt, DevDiag *__restrict__ diag,
-    int64_t nsph, double skin) {
-  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (i >= nsph) return;
-  const int64_t c = i >> 2;
-  const int s = (int)(i & 3);
-  const InstParams &p = P[inst_of[c]];
-  const double *g = stage + c * kStageDoubles;
-  if (frozen(p.time, g[G_RR], p.side_x)) {
-    ccnt[i] = 0;
-    return;
-  }
-  const double rad = g[G_RAD];
-  const d3 q = rec_sphere(g, rad / p.sphere_div, s);
-  const int nx = p.sg_n[0], ny = p.sg_n[1], nz = p.sg_n[2];
-  const int ix = grid_index(q.x, p.sg_lo[0], p.sg_inv_h, nx);
-  const int iy = grid_index(q.y, p.sg_lo[1], p.sg_inv_h, ny);
-  const int iz = grid_index(q.z, p.sg_lo[2], p.sg_inv_h, nz);
-  const int x0 = ix > 0 ? ix - 1 : 0, x1 = ix + 1 < nx ? ix + 1 : nx - 1;
-  uint32_t lo[9], hi[9];
-#pragma unroll
-  for (int r = 0; r < 9; r++) {
-    const int y = iy + (r % 3) - 1, z = iz + (r / 3) - 1;
-    lo[r] = hi[r] = 0;
-    if (y >= 0 && y < ny && z >= 0 && z < nz) {
-      const int64_t row = p.sg_base + (int64_t)nx * (y + (int64_t)ny * z);
-      lo[r] = starts[row + x0];
-      hi[r] = starts[row + x1 + 1];
-    }
-  }
-  int cnt = 0;
-  uint32_t *out = clist + i * CAP;
-#pragma unroll
-  for (int r = 0; r < 9; r++) {
-    for (uint32_t j = lo[r]; j < hi[r]; j++) {
-      const uint32_t id = sid[j];
-      const double4 o = spos[j];
-      if ((int64_t)(id >> 2) == c) continue;
-      const double sep = rad + o.w + skin, sep2 = sep * sep;
-      const double dx = o.x - q.x, dy = o.y - q.y, dz = o.z - q.z;
-      const double d2 = dx * dx + dy * dy + dz * dz;
-      if (d2 < sep2) {
-        if (cnt < CAP) out[cnt] = id;
-        cnt++;
-      }
-    }
-  }
-  if (cnt > CAP) {
-    atomicAdd(&diag->neigh_overflow, 1ull);
-    cnt = CAP;
-  }
-  ccnt[i] = cnt;
-}
-
-// Candidate list of one clump from the 4 sphere lists of k_detect (skin > 0): every clump
+// Candidate list of one clump from the 4 sphere lists of k_sweep (skin > 0): every clump
 // with a sphere pair within sep + skin, with the mask of those pairings, sorted by the
 // reference key at the build (clump-cell offset, reference index; clumps the visit rule
 // excludes at the build sort last, the force stage re-applies the rule every step).  Also
@@ -982,7 +908,7 @@ __device__ __forceinline__ double chunk_time(const InstParams &q, int si) {
 }
 
 // Main per-clump force pass.  One lane per clump (storage order = spatial order).  The
-// touching clumps come from k_detect's per-sphere lists or, with VERLET, from the candidate
+// touching clumps come from k_sweep's per-sphere lists or, with VERLET, from the candidate
 // list of k_vmerge, filtered every step by the same touching test and visit rule.
 // time: the force stage's time (sys.time after the step's advance).
 // State I/O of a clump (bit-identical results in every variant):
@@ -996,7 +922,8 @@ __device__ __forceinline__ double chunk_time(const InstParams &q, int si) {
 //   is left in out for the next step's predict (predict_regs), which writes the next
 //   record: one read and one write of the state per clump-step.
 // wreach (VERLET): the wall modules each clump can reach while the lists are valid.
-template <int MAXN, int BS, bool VERLET, bool NEXT, bool LEAN, bool CLOSED, bool DEEP>
+template <int MAXN, int BS, bool VERLET, bool NEXT, bool LEAN, bool CLOSED, bool DEEP,
+          bool CELLS>
 __device__ __forceinline__ void force_clump(
     const InstParams &p, PPShared<MAXN, BS> &sh, int lane, int64_t c,
     double *__restrict__ st, const double *__restrict__ sta, const double *__restrict__ stage,
@@ -1005,10 +932,12 @@ __device__ __forceinline__ void force_clump(
     const int32_t *__restrict__ vl_cnt, const uint8_t *__restrict__ wreach,
     double *__restrict__ fout, DevDiag *__restrict__ diag, int64_t cap, int initial,
     double time, const ClumpRegs &own, ClumpRegs &out) {
-  static_assert(!NEXT || VERLET, "the fused predict runs on candidate-list steps only");
-  constexpr bool kOwnStage = VERLET && (LEAN || !NEXT);
+  static_assert(!NEXT || VERLET || CELLS, "the fused predict needs the record-resident state");
+  static_assert(!CELLS || (!VERLET && LEAN), "per-step binning keeps the state in the records");
+  constexpr bool kRecState = VERLET || CELLS;  // the state lives in the stage records
+  constexpr bool kOwnStage = kRecState && (LEAN || !NEXT);
   constexpr bool kRegsOut = NEXT && LEAN;
-  constexpr bool kWriteRR = VERLET && !NEXT;
+  constexpr bool kWriteRR = kRecState && !NEXT;
   const d3 rr = kOwnStage ? own.rr
                           : make_d3(st[S_RX * cap + c], st[S_RY * cap + c], st[S_RZ * cap + c]);
   d3 vv = kOwnStage ? own.vv : make_d3(st[S_VX * cap + c], st[S_VY * cap + c], st[S_VZ * cap + c]);
@@ -1082,7 +1011,7 @@ __device__ __forceinline__ void force_clump(
           for (int k = 0; k < 9; k++) Rm[k] = g[G_R + k];
           if (frozen(time, rm.x, p.side_x)) continue;  // not binned at this time (cells.cu:65)
           const double posm = radm / p.sphere_div;
-          // the candidate pairings that touch now: k_detect's test on the same operands
+          // the candidate pairings that touch now: k_sweep's test on the same operands
           const double sep = rad + radm, sep2 = sep * sep;
           uint32_t mask = 0u;
           while (bits) {
@@ -1132,7 +1061,7 @@ __device__ __forceinline__ void force_clump(
                          const double *Rm) {
           if (frozen(time, rm.x, p.side_x)) return;  // not binned at this time (cells.cu:65)
           const double posm = radm / p.sphere_div;
-          // the candidate pairings that touch now: k_detect's test on the same operands
+          // the candidate pairings that touch now: k_sweep's test on the same operands
           const double sep = rad + radm, sep2 = sep * sep;
           uint32_t mask = 0u;
           while (bits) {
@@ -1484,7 +1413,8 @@ __device__ __forceinline__ void force_clump(
 // closed forms (InstParams::mass_closed); a separate instantiation, since the code alone
 // costs the other kernels registers.  DEEP: candidate records gathered one entry ahead, for
 // contexts of at most one wavefront per SIMD (latency, not occupancy, bounds them).
-template <int MAXN, int BS, bool VERLET, bool NEXT, bool LEAN, bool CLOSED, bool DEEP>
+template <int MAXN, int BS, bool VERLET, bool NEXT, bool LEAN, bool CLOSED, bool DEEP,
+          bool CELLS>
 __global__ __launch_bounds__(BS)
 __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
     double *__restrict__ st, double *__restrict__ sta, const double *__restrict__ stage,
@@ -1494,8 +1424,8 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
     const int32_t *__restrict__ vl_cnt, double *__restrict__ fout, DevDiag *__restrict__ diag,
     long long *__restrict__ acc, double *__restrict__ stage_out,
     const int32_t *__restrict__ orig, ListTrack lt, int64_t n, int64_t cap, int initial,
-    int time_ahead, int si) {
-  static_assert(!NEXT || VERLET, "the fused predict runs on candidate-list steps only");
+    int time_ahead, int si, uint32_t *__restrict__ counts, uint32_t *__restrict__ skey) {
+  static_assert(!NEXT || VERLET || CELLS, "the fused predict needs the record-resident state");
   static_assert(sizeof(PPShared<MAXN, BS>) >= sizeof(double) * BS * kStagePad,
                 "stage records of the workgroup are assembled in the pair scratch");
   __shared__ PPShared<MAXN, BS> sh;
@@ -1504,7 +1434,7 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
   const int64_t c = c0 + lane;
   const int inst = c < n ? inst_of[c] : -1;
   ClumpRegs cr, own;
-  constexpr bool kOwnStage = VERLET && (LEAN || !NEXT);
+  constexpr bool kOwnStage = (VERLET || CELLS) && (LEAN || !NEXT);
   constexpr bool kRegsOut = NEXT && LEAN;
   if (kOwnStage && c < n) {  // the own record, whole, in 16-byte loads
     const double2 *o = reinterpret_cast<const double2 *>(stage + c * kStageDoubles);
@@ -1533,7 +1463,7 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
       if (inst == cur) {
         const InstParams &q = P[cur];
         const double t = chunk_time(q, si);
-        force_clump<MAXN, BS, VERLET, NEXT, LEAN, CLOSED, DEEP>(
+        force_clump<MAXN, BS, VERLET, NEXT, LEAN, CLOSED, DEEP, CELLS>(
             q, sh, lane, c, st, sta, stage, clist, ccnt, vl_m, vl_mask, vl_cnt, lt.wreach, fout,
             diag, cap, initial, time_ahead ? t + q.dt : t, own, cr);
         break;
@@ -1565,22 +1495,29 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
   if constexpr (NEXT) {
     __syncthreads();  // the pair scratch is free: it now holds the workgroup's records
     double *rec = reinterpret_cast<double *>(&sh);
+    uint32_t keys[4] = {kInvalidKey, kInvalidKey, kInvalidKey, kInvalidKey};
     if (c < n) {
       while (true) {
         const int cur = __builtin_amdgcn_readfirstlane(inst);
         if (inst == cur) {
-          const InstParams &q = P[cur];  // the next step starts at this step's time + dt
+          const InstParams &q = P[cur];
+          // the next step starts at this step's force-stage time (P.time on per-step binning,
+          // the chunk's step time + dt on candidate-list steps)
+          const double t_old = CELLS ? q.time : chunk_time(q, si) + q.dt;
+          const double rad0 = own.rad != 0.0 ? own.rad : sta[C_RAD0 * cap + c];
           if constexpr (LEAN)
-            predict_regs(cr, sta, rec + lane * kStagePad, q, own.id, own.rad,
-                         own.rad != 0.0 ? own.rad : sta[C_RAD0 * cap + c], own.rho, c, cap, lt,
-                         chunk_time(q, si) + q.dt);
+            predict_regs(cr, sta, rec + lane * kStagePad, q, own.id, own.rad, rad0, own.rho, c,
+                         cap, lt, t_old);
           else
-            predict_clump(st, sta, rec + lane * kStagePad, q, orig, nullptr, nullptr, nullptr, c,
-                          cap, 2, lt, chunk_time(q, si) + q.dt);
+            predict_clump(st, sta, rec + lane * kStagePad, q, orig, nullptr, c, cap, 2, lt,
+                          t_old);
+          if constexpr (CELLS) clump_keys(q, cr.rr, cr.R, rad0, t_old + q.dt, keys);
           break;
         }
       }
     }
+    if constexpr (CELLS)  // sphere-cell counts of the next step
+      if (c < n) count_keys(keys, counts, skey, c);
     __syncthreads();
     const int64_t rows = n - c0 < BS ? n - c0 : BS;
     double *dst = stage_out + c0 * kStageDoubles;
@@ -1706,6 +1643,345 @@ __global__ void k_finalize(InstParams *__restrict__ P, long long *__restrict__ a
   p.time = t;
 }
 
+// The parameter updates between two force stages of per-step binning, applied by
+// k_count_scan: the previous step's clump-cell regrid when due and cc++ (k_finalize's work,
+// avalanche_hst.cu:713-748, 852), then the time advance of the next step
+// (avalanche_hst.cu:672, sys_par.time += sys_par.dt).
+__device__ __forceinline__ void step_params(InstParams &p, long long *acc, int finalize_prev,
+                                            int advance) {
+  if (finalize_prev) {
+    if (p.regrid_interval > 0 && p.cc % p.regrid_interval == 0) finalize_params(p, acc, 1, 0);
+    p.cc += 1;
+  }
+  if (advance) p.time = p.time + p.dt;
+}
+
+// Exclusive scan of the counts into starts and cursor (the next free position of each cell,
+// for k_scatter), the counts back to zero; starts[ncells] receives the total.  Workgroup 0's first lane also applies step_params to every instance
+// (no other kernel runs meanwhile; this kernel reads no parameter).
+__global__ __launch_bounds__(kScanBlock) void k_count_scan(
+    uint32_t *__restrict__ counts, const uint32_t *__restrict__ bsum,
+    uint32_t *__restrict__ starts, uint32_t *__restrict__ cursor,
+    const int64_t *__restrict__ ncells_p,
+    InstParams *__restrict__ P, long long *__restrict__ acc, int32_t ninst, int finalize_prev,
+    int advance) {
+  __shared__ uint32_t lds[kScanBlock / 64];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (finalize_prev || advance))
+    for (int32_t i = 0; i < ninst; i++) step_params(P[i], acc + 8 * i, finalize_prev, advance);
+  const int64_t ncells = *ncells_p;
+  uint32_t s = 0u;  // sum of the earlier workgroups' counts
+  for (int k = threadIdx.x; k < (int)blockIdx.x; k += kScanBlock) s += bsum[k];
+  uint32_t off;
+  block_excl_scan(s, &off, lds);
+  int64_t lo, hi;
+  scan_share(ncells, blockIdx.x, gridDim.x, lo, hi);
+  // tile by tile: wavefront w takes 1024 cells as 16 rows of 64 (row k of lane l is cell
+  // seg + 64 k + l: coalesced loads and stores), scans each row across the lanes, and the
+  // rows and the 4 wavefronts are chained in order
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  constexpr int kRows = kScanTile / kScanBlock;  // 16
+  for (int64_t base = lo; base < hi; base += kScanTile) {
+    const int64_t seg = base + (int64_t)wid * 64 * kRows;
+    uint32_t v[kRows], inc[kRows], tot = 0u;
+#pragma unroll
+    for (int k = 0; k < kRows; k++) {
+      const int64_t i = seg + 64 * k + lane;
+      v[k] = i < hi ? counts[i] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kRows; k++) {
+      inc[k] = wave_incl_scan(v[k]);
+      tot += __shfl(inc[k], 63, 64);
+    }
+    __syncthreads();  // lds is free (the previous tile's reads are done)
+    if (lane == 0) lds[wid] = tot;
+    __syncthreads();
+    uint32_t woff = off, tile = 0u;
+    for (int w = 0; w < kScanBlock / 64; w++) {
+      const uint32_t t = lds[w];
+      if (w < wid) woff += t;
+      tile += t;
+    }
+#pragma unroll
+    for (int k = 0; k < kRows; k++) {
+      const int64_t i = seg + 64 * k + lane;
+      if (i < hi) {
+        const uint32_t st0 = woff + inc[k] - v[k];
+        starts[i] = st0;
+        cursor[i] = st0;
+        counts[i] = 0u;
+      }
+      woff += __shfl(inc[k], 63, 64);
+    }
+    off += tile;
+  }
+  if (threadIdx.x == 0 && ((lo < hi && hi == ncells) || (ncells == 0 && blockIdx.x == 0)))
+    starts[ncells] = off;
+}
+
+// ------------------------------------------------------------------ contact sweep ---
+//
+// The touching spheres of every binned sphere, by bricks of sphere cells: a workgroup takes
+// a brick of kSwBX x kSwBY x kSwBZ cells of an instance's grid and the one-cell halo around
+// it, kSwRows rows of cells along x; a row is one contiguous range of the counting-sorted
+// sphere array, so two cell starts per row bound it.  The workgroup copies the spheres of
+// all rows to LDS with their cell along x, rebuilds each row's cell starts in LDS from
+// those cells (the spheres of a row are sorted by cell), then each lane takes one sphere of
+// the brick and tests it against the spheres of the 27 cells around its own, reading LDS
+// only.  A halo holding more than kSwCap spheres is swept from global memory instead (cell
+// starts loaded, same arithmetic).  The test is force_part.cu:140-157's
+// ((dx^2 + dy^2) + dz^2) < sep^2 on the same operands, so the listed set is exactly the set
+// force_part evaluates; each sphere lists the touching spheres of other clumps as
+// (m << 2 | sm), at most CAP of them (the rest are counted in diag.neigh_overflow).  With
+// skin > 0 (candidate lists) the test is d^2 < (sep + skin)^2 and the cells are at least
+// 2 r_max + skin wide.  Brick b of all instances' bricks goes to logical workgroup b mod G
+// (G = kSwGrid workgroups, more than stay resident, so that the hardware overlaps the
+// phases of different bricks); runs of 8 consecutive bricks share an XCD and its L2 (the
+// halos of neighbouring bricks overlap), successive runs go to successive XCDs.
+constexpr int kSwBX = 16, kSwBY = 8, kSwBZ = 4;
+constexpr int kSwHX = kSwBX + 2, kSwHY = kSwBY + 2, kSwHZ = kSwBZ + 2;
+constexpr int kSwRows = kSwHY * kSwHZ;          // halo rows of a brick
+constexpr int kSwOwnRows = kSwBY * kSwBZ;       // rows of the brick itself
+constexpr int kSwRowStarts = kSwHX + 1;         // cell starts of a halo row and its end
+constexpr int kSwBlock = 256;
+constexpr int kSwCap = 768;                     // halo spheres staged in LDS
+constexpr int kSwGrid = 8192;                   // workgroups (bricks in strides of kSwGrid)
+static_assert(kSwCap % kSwBlock == 0, "whole copy rounds");
+
+__device__ __forceinline__ int64_t bricks_of(const InstParams &p) {
+  return (int64_t)((p.sg_n[0] + kSwBX - 1) / kSwBX) * ((p.sg_n[1] + kSwBY - 1) / kSwBY) *
+         ((p.sg_n[2] + kSwBZ - 1) / kSwBZ);
+}
+
+// Index of the last entry of a[0..n) (ascending) that is <= v.
+__device__ __forceinline__ int upper_row(const uint32_t *a, int n, uint32_t v) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a[mid] <= v)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+struct SweepLds {
+  uint32_t hs[kSwRows][kSwRowStarts];  // first sphere of each halo cell (LDS or global index)
+  uint32_t lo[kSwRows];                // global index of each halo row's first sphere
+  uint32_t hoff[kSwRows + 1];          // LDS index of each halo row's first sphere
+  uint32_t own[kSwOwnRows + 1];        // prefix of the brick rows' sphere counts
+  double x[kSwCap], y[kSwCap], z[kSwCap], r[kSwCap];
+  uint32_t id[kSwCap];
+  uint8_t cx[kSwCap];                  // cell along x in the halo frame (0 .. kSwHX - 1)
+  uint8_t row[kSwCap];                 // halo row of the sphere
+};
+
+// Tests one sphere (q, rad, me) of the brick against the spheres of its 27 cells: halo rows
+// hr +- 1 (y) +- kSwHY (z), cells lxc - 1 .. lxc + 1 of each.  LDS: sh.hs holds LDS indices;
+// otherwise global indices into the sorted arrays.
+template <int CAP, bool LDS>
+__device__ __forceinline__ void sweep_one(const SweepLds &sh, const double4 *__restrict__ spos,
+                                          const uint32_t *__restrict__ sid, d3 q, double rad,
+                                          uint32_t me, int hr, int lxc, double skin,
+                                          uint32_t *__restrict__ clist,
+                                          int32_t *__restrict__ ccnt, DevDiag *__restrict__ diag) {
+  const uint32_t mc = me >> 2;
+  uint32_t *out = clist + (int64_t)me * CAP;
+  uint32_t j0[9], j1[9];
+#pragma unroll
+  for (int r9 = 0; r9 < 9; r9++) {
+    const int h2 = hr + (r9 % 3 - 1) + kSwHY * (r9 / 3 - 1);
+    j0[r9] = sh.hs[h2][lxc - 1];
+    j1[r9] = sh.hs[h2][lxc + 2];
+  }
+  int cnt = 0;
+#pragma unroll
+  for (int r9 = 0; r9 < 9; r9++) {
+    for (uint32_t j = j0[r9]; j < j1[r9]; j++) {
+      double ox, oy, oz, orad;
+      uint32_t id;
+      if constexpr (LDS) {
+        ox = sh.x[j];
+        oy = sh.y[j];
+        oz = sh.z[j];
+        orad = sh.r[j];
+        id = sh.id[j];
+      } else {
+        const double4 o = spos[j];
+        ox = o.x;
+        oy = o.y;
+        oz = o.z;
+        orad = o.w;
+        id = sid[j];
+      }
+      if ((id >> 2) == mc) continue;
+      const double sep = rad + orad + skin, sep2 = sep * sep;
+      const double dx = ox - q.x, dy = oy - q.y, dz = oz - q.z;
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 < sep2) {
+        if (cnt < CAP) out[cnt] = id;
+        cnt++;
+      }
+    }
+  }
+  if (cnt > CAP) {
+    atomicAdd(&diag->neigh_overflow, 1ull);
+    cnt = CAP;
+  }
+  ccnt[me] = cnt;
+}
+
+template <int CAP>
+__global__ __launch_bounds__(kSwBlock) void k_sweep(
+    const InstParams *__restrict__ P, int32_t ninst, const int64_t *__restrict__ nbricks_p,
+    const uint32_t *__restrict__ starts, const double4 *__restrict__ spos,
+    const uint32_t *__restrict__ sid, uint32_t *__restrict__ clist, int32_t *__restrict__ ccnt,
+    DevDiag *__restrict__ diag, double skin) {
+  __shared__ SweepLds sh;
+  const int tid = threadIdx.x;
+  const int64_t nb = *nbricks_p;
+  const int64_t G = gridDim.x, L = xcd_block_runs(blockIdx.x, gridDim.x, 8);
+#pragma unroll 1
+  for (int64_t b = L; b < nb; b += G) {
+    __syncthreads();  // the previous brick's LDS is no longer read
+    int inst = 0;
+    if (ninst > 1) {  // last instance whose first brick is <= b
+      int lo = 0, hi = ninst - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (P[mid].sg_bbase <= b)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      inst = lo;
+    }
+    const InstParams &p = P[inst];
+    const int nx = p.sg_n[0], ny = p.sg_n[1], nz = p.sg_n[2];
+    const int64_t lb = b - p.sg_bbase;
+    const int nbx = (nx + kSwBX - 1) / kSwBX, nby = (ny + kSwBY - 1) / kSwBY;
+    const int x0 = (int)(lb % nbx) * kSwBX, y0 = (int)((lb / nbx) % nby) * kSwBY;
+    const int z0 = (int)(lb / ((int64_t)nbx * nby)) * kSwBZ;
+    const double lo_x = p.sg_lo[0], inv_h = p.sg_inv_h;
+    // the rows' bounds: cells x0 - 1 .. x0 + kSwBX of the row, clamped into [0, nx] (cells
+    // outside the grid have no width; rows outside it are empty)
+    uint32_t rlen = 0u;
+    if (tid < kSwRows) {
+      const int y = y0 - 1 + tid % kSwHY, z = z0 - 1 + tid / kSwHY;
+      uint32_t a = 0u, e = 0u;
+      if (y >= 0 && y < ny && z >= 0 && z < nz) {
+        const int64_t row = p.sg_base + (int64_t)nx * (y + (int64_t)ny * z);
+        const int xa = x0 - 1 < 0 ? 0 : x0 - 1;
+        const int xe = x0 + kSwBX + 1 > nx ? nx : x0 + kSwBX + 1;
+        a = starts[row + xa];
+        e = starts[row + xe];
+      }
+      sh.lo[tid] = a;
+      rlen = e - a;
+    }
+    if (tid < 64) {  // LDS offsets of the rows
+      const uint32_t inc = wave_incl_scan(rlen);
+      if (tid < kSwRows) sh.hoff[tid + 1] = inc;
+      if (tid == 0) sh.hoff[0] = 0u;
+    }
+    __syncthreads();
+    const uint32_t H = sh.hoff[kSwRows];
+    if (H == 0u) continue;
+    const bool lds = H <= (uint32_t)kSwCap;
+    if (lds) {
+      // the halo spheres and their cells: every thread's rows found first, then all its
+      // loads issued together (their latencies overlap), then the LDS copies
+      constexpr int kIt = kSwCap / kSwBlock;
+      int hrs[kIt];
+      uint32_t js[kIt];
+#pragma unroll
+      for (int q = 0; q < kIt; q++) {
+        const uint32_t t = tid + q * kSwBlock;
+        hrs[q] = 0;
+        js[q] = 0u;
+        if (t < H) {
+          hrs[q] = upper_row(sh.hoff, kSwRows, t);
+          js[q] = sh.lo[hrs[q]] + (t - sh.hoff[hrs[q]]);
+        }
+      }
+      double4 ov[kIt];
+      uint32_t iv[kIt];
+#pragma unroll
+      for (int q = 0; q < kIt; q++)
+        if (tid + q * kSwBlock < H) {
+          ov[q] = spos[js[q]];
+          iv[q] = sid[js[q]];
+        }
+#pragma unroll
+      for (int q = 0; q < kIt; q++) {
+        const uint32_t t = tid + q * kSwBlock;
+        if (t < H) {
+          sh.x[t] = ov[q].x;
+          sh.y[t] = ov[q].y;
+          sh.z[t] = ov[q].z;
+          sh.r[t] = ov[q].w;
+          sh.id[t] = iv[q];
+          int lx = grid_index(ov[q].x, lo_x, inv_h, nx) - (x0 - 1);
+          sh.cx[t] = (uint8_t)(lx < 0 ? 0 : (lx > kSwHX - 1 ? kSwHX - 1 : lx));
+          sh.row[t] = (uint8_t)hrs[q];
+        }
+      }
+      for (int t = tid; t < kSwRows * kSwRowStarts; t += kSwBlock)  // empty cells: row end
+        sh.hs[t / kSwRowStarts][t % kSwRowStarts] = sh.hoff[t / kSwRowStarts + 1];
+      __syncthreads();
+      // cell starts from the sorted cells: sphere t starts every cell after its predecessor's
+      for (uint32_t t = tid; t < H; t += kSwBlock) {
+        const int hr = sh.row[t];
+        const int prev = t == sh.hoff[hr] ? -1 : (int)sh.cx[t - 1];
+        for (int k = prev + 1; k <= (int)sh.cx[t]; k++) sh.hs[hr][k] = t;
+      }
+    } else {
+      for (int t = tid; t < kSwRows * kSwRowStarts; t += kSwBlock) {
+        const int hr = t / kSwRowStarts, k = t % kSwRowStarts;
+        const int y = y0 - 1 + hr % kSwHY, z = z0 - 1 + hr / kSwHY;
+        uint32_t v = 0u;
+        if (y >= 0 && y < ny && z >= 0 && z < nz) {
+          int x = x0 - 1 + k;
+          x = x < 0 ? 0 : (x > nx ? nx : x);
+          v = starts[p.sg_base + (int64_t)nx * (y + (int64_t)ny * z) + x];
+        }
+        sh.hs[hr][k] = v;
+      }
+    }
+    __syncthreads();
+    if (tid < 64) {  // spheres of the brick rows (cells 1 .. kSwBX of the halo frame)
+      uint32_t len = 0u;
+      if (tid < kSwOwnRows) {
+        const int hr = (1 + tid % kSwBY) + kSwHY * (1 + tid / kSwBY);
+        len = sh.hs[hr][kSwBX + 1] - sh.hs[hr][1];
+      }
+      const uint32_t inc = wave_incl_scan(len);
+      if (tid < kSwOwnRows) sh.own[tid + 1] = inc;
+      if (tid == 0) sh.own[0] = 0u;
+    }
+    __syncthreads();
+    const uint32_t K = sh.own[kSwOwnRows];
+    for (uint32_t k = tid; k < K; k += kSwBlock) {
+      const int o = upper_row(sh.own, kSwOwnRows, k);
+      const int hr = (1 + o % kSwBY) + kSwHY * (1 + o / kSwBY);
+      const uint32_t j = sh.hs[hr][1] + (k - sh.own[o]);  // LDS or global index
+      if (lds) {
+        const d3 q = make_d3(sh.x[j], sh.y[j], sh.z[j]);
+        const int lxc = sh.cx[j] < 1 ? 1 : (sh.cx[j] > kSwBX ? kSwBX : (int)sh.cx[j]);
+        sweep_one<CAP, true>(sh, spos, sid, q, sh.r[j], sh.id[j], hr, lxc, skin, clist, ccnt,
+                             diag);
+      } else {
+        const double4 v = spos[j];
+        const d3 q = make_d3(v.x, v.y, v.z);
+        int lxc = grid_index(q.x, lo_x, inv_h, nx) - x0 + 1;
+        lxc = lxc < 1 ? 1 : (lxc > kSwBX ? kSwBX : lxc);
+        sweep_one<CAP, false>(sh, spos, sid, q, v.w, sid[j], hr, lxc, skin, clist, ccnt, diag);
+      }
+    }
+  }
+}
+
 // --------------------------------------------------------------- sphere-cell grid ---
 
 // AABB of clump centres and max |v| per instance (acc: 8 per instance as in
@@ -1741,7 +2017,8 @@ __global__ __launch_bounds__(256) void k_sgrid_reduce(const double *__restrict__
 // total fits the cell capacity.  Single thread.
 __global__ void k_sgrid_finalize(InstParams *__restrict__ P, long long *__restrict__ acc,
                                  int32_t ninst, int64_t cell_cap, double factor, double horizon,
-                                 int64_t *__restrict__ ncells_out, double skin) {
+                                 int64_t *__restrict__ ncells_out, double skin,
+                                 int64_t *__restrict__ nbricks_out) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   double scale = 1.0;
   bool fit = false;
@@ -1788,6 +2065,12 @@ __global__ void k_sgrid_finalize(InstParams *__restrict__ P, long long *__restri
     }
     *ncells_out = ninst;
   }
+  int64_t bricks = 0;  // the contact sweep's bricks, instance after instance
+  for (int i = 0; i < ninst; i++) {
+    P[i].sg_bbase = bricks;
+    bricks += bricks_of(P[i]);
+  }
+  *nbricks_out = bricks;
   for (int i = 0; i < ninst; i++) {
     long long *a = acc + 8 * i;
     for (int k = 0; k < 3; k++) {
