@@ -6,12 +6,40 @@
  * values reproduce the reference exactly).  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load it; the product never links it.
  *
- * Pinning (each function's check is listed in oracle/README.md):
- *   - every op is compared bit-for-bit with the reference's own _hst functions built from
- *     their sources (oracle/_ref, see oracle/Makefile) on seeded states;
- *   - closed forms (ballistic flight, Rodrigues rotation, two-sphere collision time and
- *     restitution, resting sphere on the incline) and an O(N^2) contact enumeration pin
- *     the arithmetic independently of the reference build.
+ * Pinning.  The reference's own sources cannot be built here (they need CUDA's runtime
+ * header), so every function is pinned by checks that the mathematics or the reference's
+ * definitions fix, each chosen so that a plausible slip fails it (tests/oracle_mutations.py
+ * plants 22 such slips; every one is caught):
+ *   orc_force_hop        hopper walls and gate at rest / approaching, gate only for t < 0,
+ *                        walls skipped below the bottom, second wall wins
+ *                        (tests/test_oracle_pins.py: test_hopper_*)
+ *   orc_force_funnel     plane force at rest (test_closed_forms.py: test_funnel_plane_*)
+ *   orc_force_fcorners   radial corner force and contact point (test_funnel_corner_*)
+ *   orc_force_walls      side walls, INTER_FLAG on / off, frozen skip (test_side_walls_*)
+ *   orc_force_flume      resting clump carries M g (test_closed_forms.py), tangential law
+ *                        on a wall: reset / stick / slide (test_tangential_law_on_a_wall)
+ *   orc_force_dep        deposit plane at three tilts (test_deposit_plane)
+ *   orc_force_part       contact set vs O(N^2) enumeration (test_closed_forms.py), normal
+ *                        law with unequal radii and m_eff, contact point and torque, host
+ *                        friction rule (test_normal_law_*, test_pp_contact_point_*,
+ *                        test_tangential_law_between_spheres_host_rule), kappa / gamma from
+ *                        t_coll and eps by contact time and restitution with unequal radii
+ *                        (test_pp_contact_time_*), summation order over pairings, cells and
+ *                        neighbours (test_pp_summation_order_*)
+ *   orc_sim_forces       module order of both phases (test_module_summation_order,
+ *                        test_fill_stage_order_pp_then_hop), wall kappa / gamma by contact
+ *                        time and restitution (test_wall_contact_time_*)
+ *   orc_cell_dim / orc_cell_setup   margins and grid (test_cell_dim_margins_and_grid)
+ *   orc_cell_locate      each clump in its geometric cell, ascending, frozen skipped
+ *                        (test_cell_locate_places_each_clump_in_its_geometric_cell)
+ *   orc_get_accel        free fall, drag (test_closed_forms.py), Euler's equations with an
+ *                        anisotropic inertia (test_euler_equations_anisotropic_inertia)
+ *   orc_init_Verlet / orc_finish_Verlet   ballistic flight, Rodrigues vs expm
+ *                        (test_closed_forms.py), frozen exit (test_frozen_exit), the
+ *                        integrator's restitution (test_pp_contact_time_*)
+ *   orc_rotate_vectors   slope frame of points on and above the floor
+ *                        (test_rotate_vectors_slope_frame)
+ *   orc_brute_contacts   itself a checker (plain O(N^2) enumeration)
  */
 #ifndef DEM_ORACLE_H
 #define DEM_ORACLE_H

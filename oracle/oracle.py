@@ -16,7 +16,9 @@ from avalancheflume3d_amd import abi
 from avalancheflume3d_amd.abi import ptr
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+# AVF_ORACLE_LIB: another build of the same restatement (e.g. a deliberately altered copy
+# built to check that the tests detect the alteration).
+ORACLE_LIB = os.environ.get("AVF_ORACLE_LIB", os.path.join(HERE, "liboracle.so"))
 
 _orc = None
 
