@@ -188,10 +188,12 @@ def headline(seed: int = 1, n: int = 250_000) -> dem.Instance:
                            spacing=0.4, jitter=0.02, seed=seed))
 
 
-def large(seed: int = 1, n: int = 2_000_000) -> dem.Instance:
+def large(seed: int = 1, n: int = 2_000_000, spacing: float = 0.44) -> dem.Instance:
     """L: 2,000,000 polydisperse clumps (r ~ U[0.09, 0.11]) in a 40 x 40 cm hopper above a
-    wide incline; mu 0.6, eps 0.4; dt from the smallest pair."""
-    a = 0.44
+    wide incline; mu 0.6, eps 0.4; dt from the smallest pair.  spacing: the lattice pitch
+    (0.44 cm keeps the clumps apart; below the clump diameter, 0.385 cm at most, neighbours
+    overlap from the start)."""
+    a = spacing
     per_layer = int(40.0 / a) ** 2
     layers = math.ceil(n / per_layer)
     return build(FlumeSpec(n=n, width=40.0, length=300.0, box=(40.0, 40.0, a * layers + 0.01),

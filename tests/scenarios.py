@@ -54,11 +54,13 @@ def _finish_records(pp, sys, geo):
 
 
 def hopper_state(n: int, seed: int, squeeze: float = 0.93, spin: float = 40.0,
-                 npart_sys: int | None = None) -> dem.Instance:
+                 npart_sys: int | None = None, rmin: float | None = None) -> dem.Instance:
     """Reference filling state: generate_particles + calculate_particle_parameters +
     place_particles + set_initial_velocity, then the lattice squeezed towards its centre so
-    neighbouring clumps overlap, random orientations, and random spins."""
-    sys, geo, fun, ntags = reference_system(n)
+    neighbouring clumps overlap, random orientations, and random spins.  rmin below the
+    input's rmax gives the reference's bidisperse radii (generate_particles.cu:26: every
+    20th clump rmax, the others rmin)."""
+    sys, geo, fun, ntags = reference_system(n) if rmin is None else reference_system(n, rmin=rmin)
     pp = dem.new_particles(n)
     dem.generate_particles(pp, sys)
     dem.calculate_particle_parameters(pp, sys, geo)
@@ -145,3 +147,15 @@ def avalanche_state(n_per_region: int, seed: int, time: float = 0.01,
     sys.time = time
     return dem.Instance(sys=sys, geo=geo, fun=fun, particles=pp, ntags=max(ntags, 40),
                         phase=abi.PHASE_AVALANCHE)
+
+
+def poly_flume(n: int, seed: int) -> dem.Instance:
+    """The large configuration's material and clumps at a small size: radii U[0.09, 0.11] cm
+    per clump (offset 0.75 r), mu 0.6, eps 0.4, a lattice tighter than the clump diameter in
+    a release box above a 35 deg flume, so neighbours of unequal radii overlap."""
+    from avalancheflume3d_amd import workloads
+    side = 4.0
+    per = int(side / 0.34) ** 2
+    return workloads.build(workloads.FlumeSpec(
+        n=n, width=side, length=30.0, box=(side, side, 0.34 * ((n + per - 1) // per) + 0.01),
+        mu=0.6, eps=0.4, rmin=0.09, rmax=0.11, spacing=0.34, jitter=0.02, seed=seed))
