@@ -79,3 +79,42 @@ def test_long_run_statistics_within_seed_spread(gpu):
     print(f"deposit profile: |mean difference| {diff:.3e}, oracle seed spread {spread:.3e}")
     assert spread > 0.0
     assert diff <= spread / math.sqrt(len(SEEDS))
+
+
+def test_batched_ensemble_configuration_equals_plain_runs(gpu):
+    """The batched-ensemble configuration: 64 flumes (clumps log-uniform in [5,000, 60,000],
+    per-instance seeds, friction and restitution U[0.3, 0.7], slope U[28, 40] deg) stepped
+    10,000 steps together in one context, against each instance run alone: per-instance
+    runout, front, mean speeds and 64-bin deposit profile bit for bit, and the whole state.
+    Both on candidate lists (skin 0.02 cm, reorder every 400 steps), whose results equal
+    per-step binning bit for bit (tests/test_gpu_parity.py)."""
+    from avalancheflume3d_amd import abi
+    insts = workloads.batch(64)
+    assert len(insts) == 64
+    o = dem.default_options()
+    o.verlet_skin, o.reorder_interval = 0.02, 400
+    steps = 10_000
+    ctx = dem.Context(insts, o)
+    ctx.regrid()
+    ctx.initial_forces()
+    ctx.step(steps)
+    assert ctx.diag()["neigh_overflow"] == 0
+    ps = abi.ProfileSpec()
+    ps.x0, ps.x1, ps.width, ps.sigma, ps.nbins = -2.0, 30.0, 4.0, 0.0, 64
+    batched = [(ctx.observe(k, ps), ctx.download(k)[0]) for k in range(len(insts))]
+    ctx.close()
+    moved = 0
+    for k, inst in enumerate(insts):
+        one = dem.Context([inst], o)
+        one.regrid()
+        one.initial_forces()
+        one.step(steps)
+        (obs, prof), pp = one.observe(0, ps), one.download(0)[0]
+        one.close()
+        (bobs, bprof), bpp = batched[k]
+        assert obs == bobs, k  # every field bit for bit (floats compared exactly)
+        assert np.array_equal(prof.view(np.uint64), bprof.view(np.uint64)), k
+        for fld in ("rr", "vv", "ww", "Rmatx", "Rmaty", "Rmatz", "radius"):
+            assert np.array_equal(pp[fld].view(np.uint64), bpp[fld].view(np.uint64)), (k, fld)
+        moved += obs["vmean"] > 0
+    assert moved == len(insts)
