@@ -2,11 +2,14 @@
 
     python bench.py [--steps K] [--warmup W] [--workload headline|small|large]
 
-Times K steps of the headline configuration (250,000 tetrahedral 4-sphere clumps, FP64,
-candidate lists with a 0.02 cm skin, spatial reorder every 400 steps) with inputs resident
-in HBM, and prints one JSON line: clump-steps/s, the roofline of the dominant kernel
-(k_force, from HIP events over the timed region) and the CPU baseline (the oracle on one
-host core, bounded sample of the same workload).
+Times K steps of the headline configuration as stated (250,000 tetrahedral 4-sphere clumps,
+FP64, the sphere cells rebuilt every step, spatial reorder every 50 steps) with inputs
+resident in HBM, and prints one JSON line: clump-steps/s, the roofline of the path's
+dominant kernel (from HIP events over the timed region), the step-level roofline (480 B per
+clump-step), and the CPU baseline (the oracle pinned to one host core, a 200-step window on
+a slab of the same workload).  The same workload on candidate lists (identical results,
+fewer rebuilds), the batched ensemble and the adjoint window are reported under their own
+keys.
 """
 from __future__ import annotations
 
@@ -41,27 +44,20 @@ DEFAULT_SKIN = 0.02
 # steps, back to back) with one pair of HIP events, one run in every 2 * TIMED_EVERY steps: a
 # regular sample over the whole region, with the events' own cost shared by the run.
 TIMED_EVERY = 8
-# Spatial reorder interval (steps) of the headline runs; a reorder also rebuilds the lists.
+# Spatial reorder interval (steps) of the candidate-list comparison runs.
 DEFAULT_REORDER = 400
-
-
-def cpu_baseline(inst, budget_s: float) -> dict:
-    """The oracle (single-threaded C restatement of the reference host path) on the same
-    workload: initial force pass, then as many steps as fit in about budget_s seconds."""
-    from oracle import oracle
-    o = oracle.OracleSim(inst.sys, inst.geo, inst.fun, inst.ntags, inst.phase, inst.particles)
-    o.initial()
-    t0 = time.perf_counter()
-    o.step(1)
-    one = time.perf_counter() - t0
-    k = max(1, int(budget_s / max(one, 1e-6)) - 1)
-    t0 = time.perf_counter()
-    o.step(k)
-    dt = time.perf_counter() - t0
-    n = len(inst.particles)
-    return {"value": n * k / dt, "unit": "clump-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle, {n} clumps, steps 2..{k + 1} of the same workload "
-                      f"({dt:.1f} s on one host core)"}
+# Algorithmic bytes per clump and launch of k_force on a per-step-binning step (the chunk's
+# steps but its last): the clump's stage record (192 B), the angular acceleration (24 B),
+# mass and inertia (32 B), the instance index (4 B), the four per-sphere contact counts
+# (16 B) read; the next stage record (192 B), the angular acceleration (24 B) and the four
+# sphere-cell keys (16 B) written.  The contact entries (4 B each), the neighbours' records
+# (gathered, cache hits) and the count increments are not counted.
+CELLS_FORCE_BYTES = 500.0
+# Algorithmic bytes per clump of the contact sweep: each binned sphere's sorted record
+# (x, y, z, r: 32 B) and id (4 B) read once, two cell starts per sphere (the grid holds about
+# two cells per sphere: 8 B), its contact count (4 B) written; contact entries not counted.
+SWEEP_BYTES_PER_CLUMP = 4 * (32.0 + 4.0 + 8.0 + 4.0)
+KERNEL_NAMES = {"force": "k_force", "detect": "k_sweep"}
 
 
 def cpu_batch_baseline(budget_s: float, cores: int) -> dict:
@@ -198,51 +194,98 @@ def adjoint_bench(steps: int, checkpoint: int = 100, count: int = 16, n: int = 2
             "tangent": tangent, "fd": adjoint_fd_check()}
 
 
-def adjoint_fd_check(n: int = 2000, pre: int = 8000, steps: int = 200) -> dict:
-    """Central differences of the GPU forward window against the adjoint on one slope
-    release of n clumps (a window short enough that no contact switches on or off within
-    the step sizes used), step sizes 1e-6 .. 1e-4 relative to each parameter."""
+def adjoint_fd_check(steps: int = 300) -> dict:
+    """Central differences of the GPU forward window against the adjoint, per parameter, on
+    two windows whose loss resolves every parameter that acts: two packets of clumps
+    colliding head-on in free space (eps_pp, kappa_pp) and a packet hitting the floor at an
+    angle (mu_pw, eps_pw, kappa_pw and its own pp contacts).  Step sizes 1e-6, 1e-5, 1e-4
+    relative; mu_pp is inactive under the host friction rule (gradient and differences 0)."""
     import numpy as np
 
     from avalancheflume3d_amd import abi, dem, workloads
-    inst = workloads.slope_release(n=n, seed=11, width=4.0)
-    ctx = dem.Context([inst])
+    insts = [workloads.colliding_packets(1), workloads.impacting_packet(2)]
+    ctx = dem.Context(insts)
     ctx.regrid()
     ctx.initial_forces()
-    ctx.step(pre)
-    ctx.sync()
     ps = abi.ProfileSpec()
-    ps.x0, ps.x1, ps.width, ps.sigma, ps.nbins = 0.0, 40.0, 4.0, 0.0, 32
-    s0 = inst.sys
-    base = np.array([[s0.mu_pp, s0.eps_pp, s0.kappa_pp, s0.mu_pw, s0.eps_pw, s0.kappa_pw]])
-    kw = dict(checkpoint=50, skin=0.05, w_runout=1.0, w_profile=10.0, profile=ps)
-    target = np.linspace(0.05, 0.01, ps.nbins)[None, :]
-    dv = np.random.default_rng(6).normal(scale=1.0, size=(n, 3))
-    out = ctx.window_run(steps, theta=base, dv=dv, target=target, **kw)
-    g_all = out["grad_theta"][0]
-    scale = max(abs(g_all[k] * base[0, k]) for k in range(6))
-    res = {"clumps": n, "window": steps}
-    for k in (1, 2, 3, 4, 5):
-        g = float(g_all[k])
-        row = {"adjoint": g}
-        best = None
-        for hr in (1e-6, 1e-5, 1e-4):
-            tp, tm = base.copy(), base.copy()
-            h = hr * base[0, k]
-            tp[:, k] += h
-            tm[:, k] -= h
-            lp = ctx.window_run(steps, theta=tp, dv=dv, target=target, grad=False, **kw)["loss"][0]
-            lm = ctx.window_run(steps, theta=tm, dv=dv, target=target, grad=False, **kw)["loss"][0]
-            f = (lp - lm) / (2 * h)
-            row[f"fd_h{hr:.0e}"] = f
-            err = abs(f - g)
-            best = err if best is None else min(best, err)
-        row["best_rel"] = best / max(abs(g), 1e-300)
-        # components far below the largest parameter sensitivity are compared on that scale
-        row["best_rel_to_largest"] = best * base[0, k] / scale
-        res[abi.THETA[k]] = row
+    ps.x0, ps.x1, ps.width, ps.sigma, ps.nbins = -4.0, 4.0, 2.0, 0.0, 32
+    th = np.array([[i.sys.mu_pp, i.sys.eps_pp, i.sys.kappa_pp, i.sys.mu_pw, i.sys.eps_pw,
+                    i.sys.kappa_pw] for i in insts])
+    kw = dict(checkpoint=50, skin=0.1, w_runout=1.0, w_profile=10.0, profile=ps,
+              target=np.zeros((2, ps.nbins)))
+    out = ctx.window_run(steps, theta=th, **kw)
+    res = {"window": steps, "clumps": [len(i.particles) for i in insts],
+           "list_valid": out["diag"]["list_valid"]}
+    worst = 0.0
+    for i, name in enumerate(("collision", "impact")):
+        for k in range(6):
+            g = float(out["grad_theta"][i, k])
+            row = {"adjoint": g}
+            best = None
+            for hr in (1e-6, 1e-5, 1e-4):
+                tp, tm = th.copy(), th.copy()
+                h = hr * th[i, k]
+                tp[i, k] += h
+                tm[i, k] -= h
+                lp = ctx.window_run(steps, theta=tp, grad=False, **kw)["loss"][i]
+                lm = ctx.window_run(steps, theta=tm, grad=False, **kw)["loss"][i]
+                f = (lp - lm) / (2 * h)
+                row[f"fd_h{hr:.0e}"] = f
+                err = abs(f - g)
+                best = err if best is None else min(best, err)
+            if g != 0.0:
+                row["best_rel"] = best / abs(g)
+                worst = max(worst, row["best_rel"])
+            res[f"{name}:{abi.THETA[k]}"] = row
+    res["worst_best_rel"] = worst
     ctx.close()
     return res
+
+
+def cpu_model() -> str:
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_window(inst, n_sample: int, steps: int) -> dict:
+    """The oracle on one host core (the calling thread pinned to the first core it may run
+    on): the first n_sample clumps of the workload (its lowest layers), the initial force
+    pass, then a timed window of `steps` steps."""
+    from oracle import oracle
+    pp = inst.particles[:n_sample].copy()
+    sys_ = type(inst.sys).from_buffer_copy(inst.sys)
+    sys_.npart = len(pp)
+    old = os.sched_getaffinity(0)
+    core = min(old)
+    os.sched_setaffinity(0, {core})
+    try:
+        o = oracle.OracleSim(sys_, inst.geo, inst.fun, inst.ntags, inst.phase, pp)
+        o.initial()
+        t0 = time.perf_counter()
+        o.step(steps)
+        dt = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, old)
+    return {"value": len(pp) * steps / dt, "unit": "clump-steps/s", "cores": 1,
+            "kind": "port", "cpu": cpu_model(),
+            "sample": f"oracle (gcc -O2 C restatement of the reference host path) pinned to 1 "
+                      f"core: the first {len(pp)} of the workload's {len(inst.particles)} "
+                      f"clumps, a {steps}-step window after the initial force pass "
+                      f"({dt:.1f} s)"}
+
+
+def kernel_bytes(name: str, n: int, diag: dict) -> float:
+    """Algorithmic bytes per launch of the kernels a roofline is quoted for."""
+    if name == "force":
+        return (CELLS_FORCE_BYTES - (MASS_INERTIA_BYTES if diag["closed_instances"] else 0.0)) * n
+    if name == "detect":
+        return SWEEP_BYTES_PER_CLUMP * n
+    raise ValueError(name)
 
 
 def main() -> int:
@@ -251,16 +294,13 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="headline", choices=["headline", "small", "large"])
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--skin", type=float, default=None,
-                    help=f"candidate-list skin in cm (default {DEFAULT_SKIN}; 0: sphere cells "
-                         "rebuilt every step)")
-    ap.add_argument("--reorder", type=int, default=None,
-                    help=f"spatial reorder interval in steps (default {DEFAULT_REORDER})")
-    ap.add_argument("--lean", type=int, default=0, choices=[-1, 0, 1],
-                    help="state layout of candidate-list chunks (avf_options.lean_state)")
-    ap.add_argument("--compare-steps", type=int, default=2000,
-                    help="steps of the per-step-rebuild comparison run (0: skip)")
+    ap.add_argument("--cpu-clumps", type=int, default=25000,
+                    help="clumps of the workload in the CPU baseline's sample")
+    ap.add_argument("--cpu-steps", type=int, default=200, help="steps of the CPU window")
+    ap.add_argument("--reorder", type=int, default=50,
+                    help="spatial reorder interval in steps (the headline configuration: 50)")
+    ap.add_argument("--list-steps", type=int, default=20000,
+                    help="steps of the candidate-list comparison run (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--adjoint-steps", type=int, default=2000)
@@ -299,10 +339,7 @@ def main() -> int:
     from avalancheflume3d_amd import dem, workloads
 
     if args.batch_only:
-        print(json.dumps({"batched": batch_compare(
-            args.batch_steps, sequential=False,
-            skin=DEFAULT_SKIN if args.skin is None else args.skin,
-            reorder=DEFAULT_REORDER if args.reorder is None else args.reorder)}),
+        print(json.dumps({"batched": batch_compare(args.batch_steps, sequential=False)}),
               flush=True)
         return 0
 
@@ -313,14 +350,18 @@ def main() -> int:
     make = {"headline": workloads.headline, "small": workloads.small, "large": workloads.large}
     inst = make[args.workload]()
     n = len(inst.particles)
+    # the stated configuration: sphere cells rebuilt every step, reorder every 50 steps
     opt = dem.default_options()
-    opt.verlet_skin = DEFAULT_SKIN if args.skin is None else args.skin
-    opt.reorder_interval = DEFAULT_REORDER if args.reorder is None else args.reorder
-    opt.lean_state = args.lean
+    opt.verlet_skin = 0.0
+    opt.reorder_interval = args.reorder
     ctx = dem.Context([inst], opt)
     ctx.regrid()
     ctx.initial_forces()
     ctx.step(args.warmup)
+    # per-kernel breakdown (events between all kernels, a few us each): names the dominant
+    # kernel of this path, whose launches the timed region brackets
+    kern, _ = ctx.step_profiled(min(args.steps, 400))
+    dominant = max(("force", "detect"), key=lambda k: kern[k])
     ctx.sync()
     torch.cuda.synchronize()
     # the state after the warmup: the graph-path run below replays the same segment
@@ -329,16 +370,17 @@ def main() -> int:
     ctx.sync()
     download_s = time.perf_counter() - t0
     t_snap, cc_snap = ctx.time()
-    # timed region: K steps launched on the library's stream with HIP events around the
-    # dominant kernel (k_force) only, for the roofline
+    # timed region: K steps launched on the library's stream, HIP events around the dominant
+    # kernel on every TIMED_EVERY-th step
     t0 = time.perf_counter()
-    force_ms, total_ms = ctx.step_timed(args.steps, "force", TIMED_EVERY)
+    kernel_ms, total_ms = ctx.step_timed(args.steps, dominant, TIMED_EVERY)
     ctx.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     diag = ctx.diag()
-    # the same steps from the same state, replayed as captured hipGraphs (the clump-cell grid
-    # is recomputed for the restored state)
+    if diag["neigh_overflow"] or diag["cell_below"]:
+        raise RuntimeError(f"contact list overflow or clump below the grid: {diag}")
+    # the same steps from the same state, replayed as captured hipGraphs (the production path)
     t0 = time.perf_counter()
     ctx.upload(0, *snap)
     ctx.sync()
@@ -351,50 +393,33 @@ def main() -> int:
     ctx.sync()
     torch.cuda.synchronize()
     wall_graph = time.perf_counter() - t0
-    d2 = ctx.diag()
-    diag_graph = {k: d2[k] - diag[k] for k in ("steps", "list_builds", "steps_undone")}
-    # per-kernel breakdown (events between all kernels: adds a few us per kernel)
-    kern, _ = ctx.step_profiled(min(args.steps, 1000))
     ctx.close()
-    # the same workload with the sphere cells rebuilt every step (no candidate lists)
-    per_step = None
-    if opt.verlet_skin > 0.0 and args.compare_steps > 0:
-        o2 = dem.default_options()
-        c2 = dem.Context([inst], o2)
-        c2.regrid()
-        c2.initial_forces()
-        c2.step(args.warmup)
-        c2.sync()
-        t0 = time.perf_counter()
-        c2.step(args.compare_steps)
-        c2.sync()
-        per_step = n * args.compare_steps / (time.perf_counter() - t0)
-        c2.close()
-
     value = n * args.steps / wall
-    bpc = FORCE_BYTES_PER_CLUMP - (MASS_INERTIA_BYTES if diag["closed_instances"] else 0.0)
-    achieved = bpc * n / (force_ms * 1e-3) / 1e9
+    kb = kernel_bytes(dominant, n, diag)
+    achieved = kb / (kernel_ms * 1e-3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
-        traffic = json.load(open(tf)).get(args.workload, {}).get("force_bytes_per_launch")
-    lists = (f"candidate lists within sep + {opt.verlet_skin} cm rebuilt when a sphere has "
-             "moved skin/2 (same results as rebuilding every step)" if opt.verlet_skin > 0
-             else "sphere cells rebuilt every step")
+        traffic = json.load(open(tf)).get(args.workload, {}).get(f"{dominant}_bytes_per_launch")
+    step_gbs = STEP_BYTES_PER_CLUMP * value / 1e9
     out = {
         "metric": METRIC, "value": value, "unit": "clump-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {n} clumps ({4 * n} spheres), FP64, {lists}, "
-                               f"spatial reorder every {opt.reorder_interval} steps",
+        "config": {"workload": f"{args.workload}: {n} clumps ({4 * n} spheres), FP64, sphere "
+                               f"cells rebuilt every step, spatial reorder every "
+                               f"{opt.reorder_interval} steps",
                    "clumps": n, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_force", "kernel_ms": force_ms,
-                     "bytes_per_clump": bpc},
-        "kernel_ms_breakdown": kern,
-        "step_hbm_gbs_algorithmic": STEP_BYTES_PER_CLUMP * n / (wall / args.steps) / 1e9,
+                     "kernel": KERNEL_NAMES[dominant], "kernel_ms": kernel_ms,
+                     "bytes_per_launch": kb},
+        # the whole step at 480 B per clump-step (algorithmic), against the HBM peak
+        "roofline_step": {"achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": step_gbs / HBM_PEAK_GBS,
+                          "bytes_per_clump_step": STEP_BYTES_PER_CLUMP},
+        "kernel_ms_breakdown": {k: v for k, v in kern.items() if v > 0},
         "events_total_ms": total_ms,
         "graph_path_clump_steps_per_s": n * args.steps / wall_graph,
         # the boundary's host hand-over (reference particle records, 408 B, plus the two
@@ -404,25 +429,55 @@ def main() -> int:
                           "download_ms": 1e3 * download_s,
                           "pcie_inclusive_clump_steps_per_s":
                               n * args.steps / (upload_s + wall + download_s)},
-        "per_step_rebuild_clump_steps_per_s": per_step,
         "diag": diag,
-        "diag_graph_path": diag_graph,
     }
+    if args.list_steps > 0:
+        out["candidate_lists"] = list_run(inst, args.list_steps, args.warmup)
     if args.batch_steps > 0:
-        out["batched"] = batch_compare(args.batch_steps, skin=opt.verlet_skin,
-                                       reorder=opt.reorder_interval)
+        out["batched"] = batch_compare(args.batch_steps)
     if args.adjoint_steps > 0:
         out["adjoint"] = adjoint_bench(args.adjoint_steps)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(inst, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_window(inst, min(args.cpu_clumps, n), args.cpu_steps)
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
         if "batched" in out:  # the batch against the oracle on all cores (16: one GPU's share)
-            cb = cpu_batch_baseline(0.5 * args.cpu_seconds, min(16, os.cpu_count() or 1))
+            cb = cpu_batch_baseline(6.0, min(16, os.cpu_count() or 1))
             out["batched"]["cpu_all_cores"] = cb
             out["batched"]["speedup_vs_cpu_all_cores"] = (
                 out["batched"]["batched_clump_steps_per_s"] / cb["value"])
     print(json.dumps(out), flush=True)
     return 0
+
+
+def list_run(inst, steps: int, warmup: int) -> dict:
+    """The same workload on candidate lists (skin 0.02 cm, reorder every 400 steps; the
+    library's per-step-binning results to the bit), with its k_force roofline (events around
+    runs of consecutive list-step launches)."""
+    from avalancheflume3d_amd import dem
+    n = len(inst.particles)
+    opt = dem.default_options()
+    opt.verlet_skin = DEFAULT_SKIN
+    opt.reorder_interval = DEFAULT_REORDER
+    ctx = dem.Context([inst], opt)
+    ctx.regrid()
+    ctx.initial_forces()
+    ctx.step(warmup)
+    ctx.sync()
+    t0 = time.perf_counter()
+    force_ms, _ = ctx.step_timed(steps, "force", TIMED_EVERY)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    diag = ctx.diag()
+    ctx.close()
+    bpc = FORCE_BYTES_PER_CLUMP - (MASS_INERTIA_BYTES if diag["closed_instances"] else 0.0)
+    ach = bpc * n / (force_ms * 1e-3) / 1e9
+    return {"workload": f"the same {n} clumps on candidate lists within sep + {DEFAULT_SKIN} cm, "
+                        f"spatial reorder every {DEFAULT_REORDER} steps",
+            "clump_steps_per_s": n * steps / wall, "steps": steps,
+            "k_force_roofline": {"achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": ach / HBM_PEAK_GBS, "kernel_ms": force_ms,
+                                 "bytes_per_clump": bpc},
+            "list_builds": diag["list_builds"], "steps_undone": diag["steps_undone"]}
 
 
 if __name__ == "__main__":

@@ -159,67 +159,3 @@ def poly_flume(n: int, seed: int) -> dem.Instance:
     return workloads.build(workloads.FlumeSpec(
         n=n, width=side, length=30.0, box=(side, side, 0.34 * ((n + per - 1) // per) + 0.01),
         mu=0.6, eps=0.4, rmin=0.09, rmax=0.11, spacing=0.34, jitter=0.02, seed=seed))
-
-
-def _slope_frame_lattice_near_floor(pp, geo, sys, lo_xy, per, a, seed, gap):
-    """A one-clump-deep jittered lattice in the slope frame whose lowest sphere sits gap
-    above the flume floor."""
-    import math as _m
-    dem.place_lattice(pp, geo, (lo_xy[0], lo_xy[1], 1.0), (lo_xy[0] + per[0] * a,
-                                                         lo_xy[1] + per[1] * a, 1.0 + per[2] * a),
-                      a, 0.01, seed)
-    c, s = sys.cost, sys.sint
-    q = pp["rr"].copy()
-    pp["rr"][:, 0] = q[:, 0] * c + q[:, 2] * s
-    pp["rr"][:, 2] = q[:, 2] * c - q[:, 0] * s
-    dem.sphere_centres(pp, geo)
-    sph = np.concatenate([pp[f] for f in ("rr_a", "rr_b", "rr_c", "rr_d")])
-    zmin = (sph[:, 0] * s + sph[:, 2] * c).min()
-    shift = zmin - (sys.rmax + gap)
-    pp["rr"][:, 0] -= shift * s
-    pp["rr"][:, 2] -= shift * c
-
-
-def colliding_packets(seed: int, per=(4, 4, 4), a: float = 0.34, v: float = 50.0) -> dem.Instance:
-    """Two packets of clumps (lattices tighter than the clump diameter) meeting head-on at
-    relative speed 2 v in free space (no gravity, far from every wall): a window dominated
-    by particle-particle contacts."""
-    from avalancheflume3d_amd import workloads
-    n = 2 * per[0] * per[1] * per[2]
-    inst = workloads.build(workloads.FlumeSpec(n=n, width=20.0, length=100.0, box=(8, 8, 8),
-                                               seed=seed))
-    inst.geo.grav = 0.0
-    pp = inst.particles
-    half = n // 2
-    L = per[0] * a
-    lo = np.array([0.3, 8.0, 5.0])
-    for k, x0 in enumerate((0.0, L + 0.01)):
-        part = pp[k * half:(k + 1) * half]
-        dem.place_lattice(part, inst.geo, tuple(lo + [x0, 0, 0]),
-                          tuple(lo + [x0 + L, per[1] * a, per[2] * a]), a, 0.01, seed * 7 + k)
-        pp[k * half:(k + 1) * half] = part
-    pp["vv"] = 0.0
-    pp["vv"][:half, 0] = v
-    pp["vv"][half:, 0] = -v
-    dem.sphere_centres(pp, inst.geo)
-    inst.sys.time = 0.01
-    inst.sys.side.x = 1.0e9
-    return inst
-
-
-def impacting_packet(seed: int, per=(6, 6, 2), a: float = 0.34, v: float = 40.0) -> dem.Instance:
-    """A packet of clumps moving down the 35 deg slope at v and into the floor at 0.4 v, its
-    lowest sphere 0.004 cm above it: impacts, sliding friction and the neighbours' contacts."""
-    from avalancheflume3d_amd import workloads
-    n = per[0] * per[1] * per[2]
-    inst = workloads.build(workloads.FlumeSpec(n=n, width=20.0, length=100.0, box=(8, 8, 8),
-                                               deposit_length=50.0, seed=seed))
-    pp = inst.particles
-    _slope_frame_lattice_near_floor(pp, inst.geo, inst.sys, (0.3, 5.0), per, a, seed, 0.004)
-    c, s = inst.sys.cost, inst.sys.sint
-    vs = np.array([v, 0.0, -0.4 * v])  # slope frame
-    pp["vv"] = [vs[0] * c + vs[2] * s, 0.0, vs[2] * c - vs[0] * s]
-    dem.sphere_centres(pp, inst.geo)
-    inst.sys.time = 0.01
-    inst.sys.side.x = 1.0e9
-    return inst

@@ -1,6 +1,6 @@
 """The host side of bench.py: a request for more than one GPU is refused before anything touches
-a device, and the CPU-baseline legs (the oracle on a bounded sample of the workload) return
-the fields the bench line carries."""
+a device, and the CPU-baseline legs (the oracle on a bounded sample of the workload, pinned
+to one core) return the fields the bench line carries."""
 import os
 import subprocess
 import sys
@@ -21,11 +21,12 @@ def test_cpu_baseline_fields():
     import bench
     from avalancheflume3d_amd import workloads
     inst = workloads.small()
-    cb = bench.cpu_baseline(inst, 0.3)
-    assert set(cb) == {"value", "unit", "cores", "kind", "sample"}
+    cb = bench.cpu_window(inst, 300, 5)
+    assert set(cb) == {"value", "unit", "cores", "kind", "sample", "cpu"}
     assert cb["value"] > 0 and cb["unit"] == "clump-steps/s"
-    assert cb["cores"] == 1 and cb["kind"] == "port"
-    assert f"{len(inst.particles)} clumps" in cb["sample"]
+    assert cb["cores"] == 1 and cb["kind"] == "port" and cb["cpu"]
+    assert "the first 300 of the workload's 2000 clumps, a 5-step window" in cb["sample"]
+    assert "pinned to 1 core" in cb["sample"]
 
 
 def test_cpu_batch_baseline_fields():

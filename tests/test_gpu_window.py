@@ -193,10 +193,9 @@ def test_gradients_vs_central_fd_of_host_path(gpu):
     eps_pp, kappa_pp) and a packet hitting the floor at an angle (impacts and sliding
     friction: mu_pw, eps_pw, kappa_pw, plus the packet's own pp contacts).  Loss: mean
     runout + 10 x the squared 32-bin deposit profile; dv ~ N(0, 1 cm/s)."""
-    from tests import scenarios
     steps = 300
     items = []
-    for inst in (scenarios.colliding_packets(1), scenarios.impacting_packet(2)):
+    for inst in (workloads.colliding_packets(1), workloads.impacting_packet(2)):
         o = oracle.OracleSim(inst.sys, inst.geo, inst.fun, inst.ntags, inst.phase, inst.particles)
         o.initial()
         g = o.get()
