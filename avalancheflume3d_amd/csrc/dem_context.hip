@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -69,6 +70,14 @@ struct avf_ctx {
   InstParams *d_ck_P = nullptr;
   DevDiag *d_ck_diag = nullptr;
   unsigned long long *d_umax = nullptr;
+  // pinned host words for the small read-backs of the step loops (the chunk's displacement
+  // bound, the diagnostic counters, the instances' parameters at a run's start)
+  struct HostMail {
+    unsigned long long umax;
+    DevDiag diag;
+  } *h_mail = nullptr;
+  InstParams *h_P = nullptr;
+  unsigned long long seen_below = 0, seen_overflow = 0;  // counters at the last check
   bool list_valid = false;
   int64_t vchunk = 8;             // next chunk of candidate-list steps
   int64_t since_build = 0;        // steps since the last list build
@@ -114,6 +123,25 @@ int fail(avf_ctx *ctx, int code, const char *fmt, ...) {
   } while (0)
 
 inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// One k_copy_set launch: {source, destination, 4-byte words} per range (unused ranges empty).
+struct CopyRange {
+  const void *src;
+  void *dst;
+  int64_t words;
+};
+void copy_set(avf_ctx *c, const std::array<CopyRange, kCopySets> &r) {
+  CopySet cs{};
+  int64_t most = 0;
+  for (int k = 0; k < kCopySets; k++) {
+    cs.src[k] = static_cast<const uint32_t *>(r[k].src);
+    cs.dst[k] = static_cast<uint32_t *>(r[k].dst);
+    cs.words[k] = r[k].words;
+    most = std::max(most, r[k].words / 4 + 1);
+  }
+  k_copy_set<<<std::min<unsigned>(kCopyGrid, grid_for(most, kCopyBlock)), kCopyBlock, 0,
+               c->stream>>>(cs);
+}
 
 template <typename T>
 int dalloc(avf_ctx *ctx, T **p, int64_t count) {
@@ -435,14 +463,10 @@ int rebuild_grid(avf_ctx *c, bool reorder) {
     k_reorder_move<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(
         c->d_skey, c->d_srank, c->d_starts, c->d_st, c->d_st2, c->d_sta, c->d_sta2, c->d_orig,
         c->d_orig2, c->d_inst, c->d_inst2, c->d_P, c->d_inv, c->n, c->cap);
-    const size_t nb = (size_t)c->cap * sizeof(double);
-    HIPCHK(c, hipMemcpyAsync(c->d_st, c->d_st2, nb * S_NFIELDS, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_sta, c->d_sta2, nb * C_NFIELDS, hipMemcpyDeviceToDevice,
-                             c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_orig, c->d_orig2, c->cap * sizeof(int32_t),
-                             hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_inst, c->d_inst2, c->cap * sizeof(int32_t),
-                             hipMemcpyDeviceToDevice, c->stream));
+    copy_set(c, {{{c->d_st2, c->d_st, (int64_t)c->cap * S_NFIELDS * 2},
+                  {c->d_sta2, c->d_sta, (int64_t)c->cap * C_NFIELDS * 2},
+                  {c->d_orig2, c->d_orig, c->cap},
+                  {c->d_inst2, c->d_inst, c->cap}}});
   }
   HIPCHK(c, hipGetLastError());
   c->sgrid_valid = true;
@@ -542,6 +566,28 @@ int vsteps_profiled(avf_ctx *c, int64_t S, bool rebuild, StepProf *prof) {
 // the build decides: beyond 0.49 skin a pair outside the list could have touched, so the
 // chunk is undone from its checkpoint and redone after a rebuild; otherwise the next chunk
 // length is set from the bound's growth.  The results are those of per-step binning.
+// Status of the steps just taken, from the device's counters (compared with their values
+// at the previous check; avf_get_diag restarts them): a clump outside its instance's
+// clump-cell grid, where the reference's cell_locate exits (cells.cu:79-88) or writes past
+// its table, and a neighbour list that overflowed (contacts dropped).
+int step_status(avf_ctx *c) {
+  HIPCHK(c, hipMemcpyAsync(&c->h_mail->diag, c->d_diag, sizeof(DevDiag), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const unsigned long long below = c->h_mail->diag.cell_below;
+  const unsigned long long over = c->h_mail->diag.neigh_overflow;
+  const unsigned long long nb = below - c->seen_below, no = over - c->seen_overflow;
+  c->seen_below = below;
+  c->seen_overflow = over;
+  if (nb)
+    return fail(c, AVF_ERR_CELL_NEG,
+                "%llu clump visit(s) outside the clump-cell grid (the reference exits)", nb);
+  if (no)
+    return fail(c, AVF_ERR_NEIGH_OVERFLOW, "%llu neighbour list overflow(s): contacts dropped",
+                no);
+  return AVF_OK;
+}
+
 int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
   // chunk length cap: one chunk per reorder interval where the displacement bound allows
   // (each chunk costs a checkpoint, its first predict, its last step and a host round trip)
@@ -552,10 +598,10 @@ int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
   int64_t left = nsteps;
   // clump-cell regrid schedule: a chunk's k_finalize applies its steps at its end, so every
   // chunk ends on a step whose regrid is due (cc % regrid_interval == 0), if it has one
-  std::vector<InstParams> hp((size_t)c->ninst);
-  HIPCHK(c, hipMemcpyAsync(hp.data(), c->d_P, c->ninst * sizeof(InstParams),
+  HIPCHK(c, hipMemcpyAsync(c->h_P, c->d_P, c->ninst * sizeof(InstParams),
                            hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<InstParams> hp(c->h_P, c->h_P + c->ninst);
   auto to_regrid = [&hp]() {  // steps up to and including the next due one
     int64_t lim = std::numeric_limits<int64_t>::max();
     for (const InstParams &q : hp)
@@ -594,13 +640,10 @@ int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
       S = std::min<int64_t>(S, c->opt.reorder_interval - c->steps_since_reorder);
     S = std::min(S, to_regrid());
     // checkpoint: state, radii (clumps may freeze), parameters (time, cc, grids), counters
-    HIPCHK(c, hipMemcpyAsync(c->d_ck_st, c->d_st, SF, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_ck_rad, c->d_sta + (size_t)C_RAD * c->cap,
-                             c->cap * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_ck_P, c->d_P, c->ninst * sizeof(InstParams),
-                             hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_ck_diag, c->d_diag, sizeof(DevDiag), hipMemcpyDeviceToDevice,
-                             c->stream));
+    copy_set(c, {{{c->d_st, c->d_ck_st, (int64_t)(SF / 4)},
+                  {c->d_sta + (size_t)C_RAD * c->cap, c->d_ck_rad, c->cap * 2},
+                  {c->d_P, c->d_ck_P, c->ninst * (int64_t)(sizeof(InstParams) / 4)},
+                  {c->d_diag, c->d_ck_diag, (int64_t)(sizeof(DevDiag) / 4)}}});
     if (prof) {
       if ((rc = vsteps_profiled(c, S, false, prof))) return rc;
     } else if (c->opt.use_graph) {
@@ -630,23 +673,20 @@ int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
       for (int64_t i = 0; i < S; i++) launch_vstep(c, false, nullptr, i + 1 == S, i, S);
     }
     // largest displacement bound since the build
-    unsigned long long um = 0;
-    HIPCHK(c, hipMemsetAsync(c->d_umax, 0, sizeof um, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_umax, 0, sizeof(unsigned long long), c->stream));
     k_win_max<<<std::min(kWinMaxGrid, grid_for(c->n, kWinMaxBlock)), kWinMaxBlock, 0,
                 c->stream>>>(c->d_dsp, c->n, c->d_umax);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(&um, c->d_umax, sizeof um, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&c->h_mail->umax, c->d_umax, sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     double maxd;
-    std::memcpy(&maxd, &um, sizeof maxd);
+    std::memcpy(&maxd, &c->h_mail->umax, sizeof maxd);
     if (maxd > 0.49 * skin) {
-      HIPCHK(c, hipMemcpyAsync(c->d_st, c->d_ck_st, SF, hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(c, hipMemcpyAsync(c->d_sta + (size_t)C_RAD * c->cap, c->d_ck_rad,
-                               c->cap * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(c, hipMemcpyAsync(c->d_P, c->d_ck_P, c->ninst * sizeof(InstParams),
-                               hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(c, hipMemcpyAsync(c->d_diag, c->d_ck_diag, sizeof(DevDiag), hipMemcpyDeviceToDevice,
-                               c->stream));
+      copy_set(c, {{{c->d_ck_st, c->d_st, (int64_t)(SF / 4)},
+                    {c->d_ck_rad, c->d_sta + (size_t)C_RAD * c->cap, c->cap * 2},
+                    {c->d_ck_P, c->d_P, c->ninst * (int64_t)(sizeof(InstParams) / 4)},
+                    {c->d_ck_diag, c->d_diag, (int64_t)(sizeof(DevDiag) / 4)}}});
       c->list_valid = false;
       c->vchunk = std::max<int64_t>(1, S / 2);
       c->steps_undone += S;
@@ -866,6 +906,11 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
       (rc = dalloc(c, &c->d_acc_cell, 8 * ninst)) || (rc = dalloc(c, &c->d_acc_sg, 8 * ninst)) ||
       (rc = dalloc(c, &c->d_diag, 1)) || (rc = dalloc(c, &c->d_count, 1)))
     return bail(rc);
+  if ((he = hipHostMalloc((void **)&c->h_mail, sizeof(avf_ctx::HostMail))) != hipSuccess ||
+      (he = hipHostMalloc((void **)&c->h_P, ninst * sizeof(InstParams))) != hipSuccess) {
+    fail(c, AVF_ERR_NOMEM, "hipHostMalloc: %s", hipGetErrorString(he));
+    return bail(AVF_ERR_NOMEM);
+  }
   if (c->opt.verlet_skin > 0.0 &&
       ((rc = dalloc(c, &c->d_vl_m, kVMax * N)) || (rc = dalloc(c, &c->d_vl_mask, kVMax * N)) ||
        (rc = dalloc(c, &c->d_vl_cnt, N)) || (rc = dalloc(c, &c->d_dsp, N)) || (rc = dalloc(c, &c->d_wreach, N)) ||
@@ -981,6 +1026,8 @@ int avf_destroy(avf_ctx *c) {
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+  if (c->h_mail) (void)hipHostFree(c->h_mail);
+  if (c->h_P) (void)hipHostFree(c->h_P);
   free_window(c);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1165,8 +1212,9 @@ int avf_initial_forces(avf_ctx *c) {
 
 int avf_step(avf_ctx *c, int64_t nsteps) {
   if (!c || nsteps < 0) return AVF_ERR_ARG;
-  if (c->opt.verlet_skin > 0.0) return run_verlet(c, nsteps, nullptr);
-  return run_cells(c, nsteps, nullptr);
+  const int rc = c->opt.verlet_skin > 0.0 ? run_verlet(c, nsteps, nullptr)
+                                          : run_cells(c, nsteps, nullptr);
+  return rc ? rc : step_status(c);
 }
 
 int avf_step_profiled(avf_ctx *c, int64_t nsteps, double *kernel_ms, double *total_ms) {
@@ -1178,6 +1226,7 @@ int avf_step_profiled(avf_ctx *c, int64_t nsteps, double *kernel_ms, double *tot
   HIPCHK(c, hipEventRecord(first, c->stream));
   StepProf prof;
   int rc = verlet ? run_verlet(c, nsteps, &prof) : run_cells(c, nsteps, &prof);
+  if (!rc) rc = step_status(c);
   HIPCHK(c, hipEventRecord(last, c->stream));
   HIPCHK(c, hipEventSynchronize(last));
   float tot = 0.f;
@@ -1470,15 +1519,16 @@ int avf_observe(avf_ctx *c, int32_t inst, avf_observables *obs, const avf_profil
 
 int avf_get_diag(avf_ctx *c, avf_diag *d) {
   if (!c || !d) return AVF_ERR_ARG;
-  DevDiag h;
-  std::vector<InstParams> hp(c->ninst);
-  HIPCHK(c, hipMemcpyAsync(&h, c->d_diag, sizeof h, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->d_diag, 0, sizeof h, c->stream));
-  HIPCHK(c, hipMemcpyAsync(hp.data(), c->d_P, hp.size() * sizeof(InstParams),
-                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&c->h_mail->diag, c->d_diag, sizeof(DevDiag), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_diag, 0, sizeof(DevDiag), c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->h_P, c->d_P, c->ninst * sizeof(InstParams), hipMemcpyDeviceToHost,
+                           c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  const DevDiag h = c->h_mail->diag;
+  c->seen_below = c->seen_overflow = 0;  // the device counters restart from zero
   int64_t closed = 0;
-  for (const InstParams &p : hp) closed += p.mass_closed != 0;
+  for (int32_t i = 0; i < c->ninst; i++) closed += c->h_P[i].mass_closed != 0;
   d->cell_below = (int64_t)h.cell_below;
   d->cell_beyond = (int64_t)h.cell_beyond;
   d->neigh_overflow = (int64_t)h.neigh_overflow;

@@ -430,6 +430,35 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter(
   sid[dst] = (uint32_t)i;
 }
 
+// Up to kCopySets device-to-device copies of 4-byte words in one launch (the spatial
+// reorder's buffer copies, the checkpoints of candidate-list chunks): 16-byte moves where
+// both ends are 16-byte aligned, word moves for the rest.
+constexpr int kCopySets = 4, kCopyBlock = 256, kCopyGrid = 2048;
+struct CopySet {
+  const uint32_t *src[kCopySets];
+  uint32_t *dst[kCopySets];
+  int64_t words[kCopySets];
+};
+
+__global__ __launch_bounds__(kCopyBlock) void k_copy_set(CopySet cs) {
+  const int64_t stride = (int64_t)gridDim.x * kCopyBlock;
+  const int64_t i0 = (int64_t)blockIdx.x * kCopyBlock + threadIdx.x;
+#pragma unroll 1
+  for (int k = 0; k < kCopySets; k++) {
+    const uint32_t *src = cs.src[k];
+    uint32_t *dst = cs.dst[k];
+    const int64_t n = cs.words[k];
+    int64_t done = 0;
+    if (((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15u) == 0) {
+      const int64_t n4 = n >> 2;
+      for (int64_t i = i0; i < n4; i += stride)
+        reinterpret_cast<uint4 *>(dst)[i] = reinterpret_cast<const uint4 *>(src)[i];
+      done = n4 << 2;
+    }
+    for (int64_t i = done + i0; i < n; i += stride) dst[i] = src[i];
+  }
+}
+
 // ------------------------------------------------------------------ boundaries -----
 
 // force_hop.cu:9-244 for one nw (0: x walls, 1: y walls, 2: bottom gate while time < 0).
@@ -989,131 +1018,97 @@ __device__ __forceinline__ void force_clump(
     if (lj == p.ncell[1]) lj--;
     int64_t lk = (int64_t)((rr.z - p.rr_min[2]) / p.cell_side[2]);
     if (lk == p.ncell[2]) lk--;
+    {  // cell_locate's check of the clump itself: the reference exits (cells.cu:79-88)
+      int64_t ii, jj, kk;
+      ref_cell_raw(p, rr, ii, jj, kk);
+      if (ii < 0 || jj < 0 || kk < 0) atomicAdd(&diag->cell_below, 1ull);
+    }
+    // one touching clump m (centre rm, reference index morig, touching pairings mask):
+    // its entry under the reference's key, (clump-cell offset of m from l, m) ascending, by
+    // insertion; a clump the reference's cell visit would not reach is counted and skipped
+    auto place = [&](int64_t cm, d3 rm, uint32_t morig, uint32_t mask) {
+      int64_t ai, aj, ak;
+      if (!ref_cell_ijk(p, rm, ai, aj, ak)) {
+        int64_t ii, jj, kk;
+        ref_cell_raw(p, rm, ii, jj, kk);
+        if (ii < 0 || jj < 0 || kk < 0)
+          atomicAdd(&diag->cell_below, 1ull);  // cell_locate exits (cells.cu:79-88)
+        else
+          atomicAdd(&diag->cell_beyond, 1ull);  // past the reference's table
+        return;
+      }
+      const int64_t di = ai - li, dj = aj - lj, dk = ak - lk;
+      if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) {
+        // outside the 27 clump cells the reference visits (force_part.cu:88-96)
+        atomicAdd(&diag->cell_beyond, 1ull);
+        return;
+      }
+      const uint32_t off = (uint32_t)((di + 1) * 9 + (dj + 1) * 3 + (dk + 1));
+      const uint32_t key = (off << 27) | morig;
+      if (cnt == MAXN) {
+        overflow = true;
+        return;
+      }
+      int pos = cnt;
+      while (pos > 0 && sh.key[pos - 1][lane] > key) {
+        sh.key[pos][lane] = sh.key[pos - 1][lane];
+        sh.m[pos][lane] = sh.m[pos - 1][lane];
+        sh.mask[pos][lane] = sh.mask[pos - 1][lane];
+        pos--;
+      }
+      sh.key[pos][lane] = key;
+      sh.m[pos][lane] = (int32_t)cm;
+      sh.mask[pos][lane] = (uint16_t)mask;
+      cnt++;
+    };
     if constexpr (VERLET) {
       const int kc = vl_cnt[c];
+      // one candidate-list entry: the touching test of k_sweep on the same operands for each
+      // candidate pairing, then its place
+      auto visit = [&](int64_t cm, uint32_t bits, d3 rm, double radm, double idm,
+                       const double *Rm) {
+        if (frozen(time, rm.x, p.side_x)) return;  // not binned at this time (cells.cu:65)
+        const double posm = radm / p.sphere_div;
+        const double sep = rad + radm, sep2 = sep * sep;
+        uint32_t mask = 0u;
+        while (bits) {
+          const int pr = __builtin_ctz(bits);
+          bits &= bits - 1u;
+          const int sdiv = pr >> 2, smod = pr & 3;
+          const d3 q = sphere_of(rr, R, pos0, smod);
+          const d3 o = sphere_of(rm, Rm, posm, sdiv);
+          const double dx = o.x - q.x, dy = o.y - q.y, dz = o.z - q.z;
+          if (dx * dx + dy * dy + dz * dz < sep2) mask |= 1u << pr;
+        }
+        if (mask) place(cm, rm, (uint32_t)__double_as_longlong(idm), mask);
+      };
+      // a candidate's centre, radius, id and rotation rows, loaded together
+      auto gather = [&](int64_t cm, d3 &rm, double &radm, double &idm, double *Rm) {
+        const double *g = stage + cm * kStageDoubles;
+        rm = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
+        radm = g[G_RAD];
+        idm = g[G_ID];
+#pragma unroll
+        for (int k = 0; k < 9; k++) Rm[k] = g[G_R + k];
+      };
+      int64_t cm_next = kc > 0 ? vl_m[c] : 0;
+      uint32_t bits_next = kc > 0 ? vl_mask[c] : 0u;
       if constexpr (!DEEP) {
         // the next entry is fetched one iteration ahead and a candidate's centre, radius, id
         // and sphere centres are loaded together, so an entry waits for one gather
-        int64_t cm_next = kc > 0 ? vl_m[c] : 0;
-        uint32_t bits_next = kc > 0 ? vl_mask[c] : 0u;
         for (int e = 0; e < kc; e++) {
           const int64_t cm = cm_next;
-          uint32_t bits = bits_next;
+          const uint32_t bits = bits_next;
           if (e + 1 < kc) {
             cm_next = vl_m[(e + 1) * cap + c];
             bits_next = vl_mask[(e + 1) * cap + c];
           }
-          const double *g = stage + cm * kStageDoubles;
-          const d3 rm = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
-          const double radm = g[G_RAD], idm = g[G_ID];
-          double Rm[9];
-  #pragma unroll
-          for (int k = 0; k < 9; k++) Rm[k] = g[G_R + k];
-          if (frozen(time, rm.x, p.side_x)) continue;  // not binned at this time (cells.cu:65)
-          const double posm = radm / p.sphere_div;
-          // the candidate pairings that touch now: k_sweep's test on the same operands
-          const double sep = rad + radm, sep2 = sep * sep;
-          uint32_t mask = 0u;
-          while (bits) {
-            const int pr = __builtin_ctz(bits);
-            bits &= bits - 1u;
-            const int sdiv = pr >> 2, smod = pr & 3;
-            const d3 q = sphere_of(rr, R, pos0, smod);
-            const d3 o = sphere_of(rm, Rm, posm, sdiv);
-            const double dx = o.x - q.x, dy = o.y - q.y, dz = o.z - q.z;
-            if (dx * dx + dy * dy + dz * dz < sep2) mask |= 1u << pr;
-          }
-          if (!mask) continue;
-          const uint32_t morig = (uint32_t)__double_as_longlong(idm);
-          int64_t ai, aj, ak;
-          if (!ref_cell_ijk(p, rm, ai, aj, ak)) {
-            atomicAdd(&diag->cell_below, 1ull);
-            continue;
-          }
-          const int64_t di = ai - li, dj = aj - lj, dk = ak - lk;
-          if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) {
-            atomicAdd(&diag->cell_beyond, 1ull);
-            continue;
-          }
-          const uint32_t off = (uint32_t)((di + 1) * 9 + (dj + 1) * 3 + (dk + 1));
-          const uint32_t key = (off << 27) | morig;
-          if (cnt == MAXN) {
-            overflow = true;
-            continue;
-          }
-          int pos = cnt;
-          while (pos > 0 && sh.key[pos - 1][lane] > key) {
-            sh.key[pos][lane] = sh.key[pos - 1][lane];
-            sh.m[pos][lane] = sh.m[pos - 1][lane];
-            sh.mask[pos][lane] = sh.mask[pos - 1][lane];
-            pos--;
-          }
-          sh.key[pos][lane] = key;
-          sh.m[pos][lane] = (int32_t)cm;
-          sh.mask[pos][lane] = (uint16_t)mask;
-          cnt++;
+          d3 rm;
+          double radm, idm, Rm[9];
+          gather(cm, rm, radm, idm, Rm);
+          visit(cm, bits, rm, radm, idm, Rm);
         }
       } else {
-        // the same entry visit as above, written once as a lambda
-        // one candidate entry: the touching test of k_detect on the same operands, then the
-        // reference's visit rule and key
-        auto visit = [&](int64_t cm, uint32_t bits, d3 rm, double radm, double idm,
-                         const double *Rm) {
-          if (frozen(time, rm.x, p.side_x)) return;  // not binned at this time (cells.cu:65)
-          const double posm = radm / p.sphere_div;
-          // the candidate pairings that touch now: k_sweep's test on the same operands
-          const double sep = rad + radm, sep2 = sep * sep;
-          uint32_t mask = 0u;
-          while (bits) {
-            const int pr = __builtin_ctz(bits);
-            bits &= bits - 1u;
-            const int sdiv = pr >> 2, smod = pr & 3;
-            const d3 q = sphere_of(rr, R, pos0, smod);
-            const d3 o = sphere_of(rm, Rm, posm, sdiv);
-            const double dx = o.x - q.x, dy = o.y - q.y, dz = o.z - q.z;
-            if (dx * dx + dy * dy + dz * dz < sep2) mask |= 1u << pr;
-          }
-          if (!mask) return;
-          const uint32_t morig = (uint32_t)__double_as_longlong(idm);
-          int64_t ai, aj, ak;
-          if (!ref_cell_ijk(p, rm, ai, aj, ak)) {
-            atomicAdd(&diag->cell_below, 1ull);
-            return;
-          }
-          const int64_t di = ai - li, dj = aj - lj, dk = ak - lk;
-          if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) {
-            atomicAdd(&diag->cell_beyond, 1ull);
-            return;
-          }
-          const uint32_t off = (uint32_t)((di + 1) * 9 + (dj + 1) * 3 + (dk + 1));
-          const uint32_t key = (off << 27) | morig;
-          if (cnt == MAXN) {
-            overflow = true;
-            return;
-          }
-          int pos = cnt;
-          while (pos > 0 && sh.key[pos - 1][lane] > key) {
-            sh.key[pos][lane] = sh.key[pos - 1][lane];
-            sh.m[pos][lane] = sh.m[pos - 1][lane];
-            sh.mask[pos][lane] = sh.mask[pos - 1][lane];
-            pos--;
-          }
-          sh.key[pos][lane] = key;
-          sh.m[pos][lane] = (int32_t)cm;
-          sh.mask[pos][lane] = (uint16_t)mask;
-          cnt++;
-        };
-        // a candidate's centre, radius, id and rotation rows, loaded together
-        auto gather = [&](int64_t cm, d3 &rm, double &radm, double &idm, double *Rm) {
-          const double *g = stage + cm * kStageDoubles;
-          rm = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
-          radm = g[G_RAD];
-          idm = g[G_ID];
-  #pragma unroll
-          for (int k = 0; k < 9; k++) Rm[k] = g[G_R + k];
-        };
-        int64_t cm_next = kc > 0 ? vl_m[c] : 0;
-        uint32_t bits_next = kc > 0 ? vl_mask[c] : 0u;
         // small contexts (at most one wavefront per SIMD, registers to spare): the next
         // entry's record is gathered, and the one after's index fetched, while an entry is
         // tested
@@ -1145,55 +1140,26 @@ __device__ __forceinline__ void force_clump(
         }
       }
     } else {
-    for (int s = 0; s < 4; s++) {
-      const int k = ccnt[4 * c + s];
-      for (int e = 0; e < k; e++) {
-        const uint32_t id = clist[(4 * c + s) * kSphList + e];
-        const int64_t cm = (int64_t)(id >> 2);
-        {
-            // contact (l sphere s, m sphere id&3): add to m's entry or insert m by key
-            const uint32_t bit = 1u << (((id & 3u) << 2) | (uint32_t)s);
-            int e2 = 0;
-            for (; e2 < cnt; e2++)
-              if (sh.m[e2][lane] == (int32_t)cm) break;
-            if (e2 < cnt) {
-              sh.mask[e2][lane] = (uint16_t)(sh.mask[e2][lane] | bit);
-              continue;
-            }
-            const double *g = stage + cm * kStageDoubles;
-            const d3 rm = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
-            const uint32_t morig = (uint32_t)__double_as_longlong(g[G_ID]);
-            int64_t ai, aj, ak;
-            if (!ref_cell_ijk(p, rm, ai, aj, ak)) {
-              atomicAdd(&diag->cell_below, 1ull);
-              continue;
-            }
-            const int64_t di = ai - li, dj = aj - lj, dk = ak - lk;
-            if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) {
-              // outside the 27 clump cells the reference visits (force_part.cu:88-96)
-              atomicAdd(&diag->cell_beyond, 1ull);
-              continue;
-            }
-            const uint32_t off = (uint32_t)((di + 1) * 9 + (dj + 1) * 3 + (dk + 1));
-            const uint32_t key = (off << 27) | morig;
-            if (cnt == MAXN) {
-              overflow = true;
-              continue;
-            }
-            int pos = cnt;
-            while (pos > 0 && sh.key[pos - 1][lane] > key) {
-              sh.key[pos][lane] = sh.key[pos - 1][lane];
-              sh.m[pos][lane] = sh.m[pos - 1][lane];
-              sh.mask[pos][lane] = sh.mask[pos - 1][lane];
-              pos--;
-            }
-            sh.key[pos][lane] = key;
-            sh.m[pos][lane] = (int32_t)cm;
-            sh.mask[pos][lane] = (uint16_t)bit;
-            cnt++;
+      // the sphere lists of k_sweep: every entry is a touching pairing (l sphere s, m sphere
+      // id & 3), added to m's entry or placed as a new one
+      for (int s = 0; s < 4; s++) {
+        const int k = ccnt[4 * c + s];
+        for (int e = 0; e < k; e++) {
+          const uint32_t id = clist[(4 * c + s) * kSphList + e];
+          const int64_t cm = (int64_t)(id >> 2);
+          const uint32_t bit = 1u << (((id & 3u) << 2) | (uint32_t)s);
+          int e2 = 0;
+          for (; e2 < cnt; e2++)
+            if (sh.m[e2][lane] == (int32_t)cm) break;
+          if (e2 < cnt) {
+            sh.mask[e2][lane] = (uint16_t)(sh.mask[e2][lane] | bit);
+            continue;
+          }
+          const double *g = stage + cm * kStageDoubles;
+          place(cm, make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]),
+                (uint32_t)__double_as_longlong(g[G_ID]), bit);
         }
       }
-    }
     }
     if (overflow) atomicAdd(&diag->neigh_overflow, 1ull);
     diag_max(&diag->max_neighbours, (unsigned long long)cnt);
@@ -1725,13 +1691,19 @@ __global__ __launch_bounds__(kScanBlock) void k_count_scan(
 // a brick of kSwBX x kSwBY x kSwBZ cells of an instance's grid and the one-cell halo around
 // it, kSwRows rows of cells along x; a row is one contiguous range of the counting-sorted
 // sphere array, so two cell starts per row bound it.  The workgroup copies the spheres of
-// all rows to LDS with their cell along x, rebuilds each row's cell starts in LDS from
-// those cells (the spheres of a row are sorted by cell), then each lane takes one sphere of
-// the brick and tests it against the spheres of the 27 cells around its own, reading LDS
-// only.  A halo holding more than kSwCap spheres is swept from global memory instead (cell
-// starts loaded, same arithmetic).  The test is force_part.cu:140-157's
-// ((dx^2 + dy^2) + dz^2) < sep^2 on the same operands, so the listed set is exactly the set
-// force_part evaluates; each sphere lists the touching spheres of other clumps as
+// all rows to LDS as float records (centre relative to the halo origin, radius), lists the
+// brick's own spheres, rebuilds each row's cell starts in LDS from the spheres' cells (the
+// spheres of a row are sorted by cell), then each lane takes one own sphere and tests it
+// against the spheres of the 27 cells around its own, reading LDS only.
+//
+// The listed set is exactly the set force_part.cu:140-157 evaluates: the pairs with
+// ((dx^2 + dy^2) + dz^2) < sep^2 in FP64 on the sphere centres.  The float test decides the
+// pairs whose distance is further than kSwMargin cell sides from sep (its rounding error is
+// below a third of that margin while every coordinate lies within kSwFar cell sides of the
+// halo origin); the pairs within the margin are decided by the FP64 test on the sorted
+// centres themselves.  A halo holding more than kSwCap spheres, or a sphere further than
+// kSwFar cell sides from the origin, is swept from global memory in FP64 (cell starts
+// loaded, same test).  Each sphere lists the touching spheres of other clumps as
 // (m << 2 | sm), at most CAP of them (the rest are counted in diag.neigh_overflow).  With
 // skin > 0 (candidate lists) the test is d^2 < (sep + skin)^2 and the cells are at least
 // 2 r_max + skin wide.  Brick b of all instances' bricks goes to logical workgroup b mod G
@@ -1744,9 +1716,13 @@ constexpr int kSwRows = kSwHY * kSwHZ;          // halo rows of a brick
 constexpr int kSwOwnRows = kSwBY * kSwBZ;       // rows of the brick itself
 constexpr int kSwRowStarts = kSwHX + 1;         // cell starts of a halo row and its end
 constexpr int kSwBlock = 256;
-constexpr int kSwCap = 768;                     // halo spheres staged in LDS
+constexpr int kSwCap = 704;                     // halo spheres staged in LDS
+constexpr int kSwIt = 1;                        // halo loads in flight per thread
 constexpr int kSwGrid = 8192;                   // workgroups (bricks in strides of kSwGrid)
-static_assert(kSwCap % kSwBlock == 0, "whole copy rounds");
+constexpr float kSwMargin = 1.0f / 65536.0f;    // undecided band of the float test (cell sides)
+constexpr double kSwFar = 20.0;                 // coordinate range of the float test (cell sides)
+static_assert(kSwCap < 65536 && kSwRows < 256, "16-bit sphere and 8-bit row indices");
+static_assert(kSwRows * kSwRowStarts * 4 <= kSwCap * 16, "global cell starts fit the records");
 
 __device__ __forceinline__ int64_t bricks_of(const InstParams &p) {
   return (int64_t)((p.sg_n[0] + kSwBX - 1) / kSwBX) * ((p.sg_n[1] + kSwBY - 1) / kSwBY) *
@@ -1767,59 +1743,122 @@ __device__ __forceinline__ int upper_row(const uint32_t *a, int n, uint32_t v) {
 }
 
 struct SweepLds {
-  uint32_t hs[kSwRows][kSwRowStarts];  // first sphere of each halo cell (LDS or global index)
+  uint16_t hs[kSwRows][kSwRowStarts];  // first LDS sphere of each halo cell
   uint32_t lo[kSwRows];                // global index of each halo row's first sphere
   uint32_t hoff[kSwRows + 1];          // LDS index of each halo row's first sphere
-  uint32_t own[kSwOwnRows + 1];        // prefix of the brick rows' sphere counts
-  double x[kSwCap], y[kSwCap], z[kSwCap], r[kSwCap];
+  uint32_t ownp[kSwOwnRows + 1];       // global sweep: prefix of the brick rows' counts
+  uint32_t nown;                       // own spheres listed in own[]
+  uint32_t far;                        // a coordinate lies beyond the float test's range
+  union {
+    float4 pr[kSwCap];                 // centre - halo origin, radius + margin (float)
+    uint32_t gs[kSwRows][kSwRowStarts];  // global sweep: first sorted sphere of each cell
+  };
   uint32_t id[kSwCap];
+  uint16_t own[kSwCap];                // LDS indices of the brick's own spheres
   uint8_t cx[kSwCap];                  // cell along x in the halo frame (0 .. kSwHX - 1)
   uint8_t row[kSwCap];                 // halo row of the sphere
 };
 
-// Tests one sphere (q, rad, me) of the brick against the spheres of its 27 cells: halo rows
-// hr +- 1 (y) +- kSwHY (z), cells lxc - 1 .. lxc + 1 of each.  LDS: sh.hs holds LDS indices;
-// otherwise global indices into the sorted arrays.
-template <int CAP, bool LDS>
-__device__ __forceinline__ void sweep_one(const SweepLds &sh, const double4 *__restrict__ spos,
-                                          const uint32_t *__restrict__ sid, d3 q, double rad,
-                                          uint32_t me, int hr, int lxc, double skin,
-                                          uint32_t *__restrict__ clist,
+// force_part.cu:140-157's test on the sorted centres (q: the listing sphere).
+__device__ __forceinline__ bool touch_fp64(double4 q, double4 o, double skin) {
+  const double sep = q.w + o.w + skin, sep2 = sep * sep;
+  const double dx = o.x - q.x, dy = o.y - q.y, dz = o.z - q.z;
+  return dx * dx + dy * dy + dz * dz < sep2;
+}
+
+// Halo row of the r9-th of the 27 cells' rows around halo row hr: hr + dy + kSwHY dz.
+__device__ __forceinline__ int nb_row(int hr, int r9) {
+  return hr + (r9 % 3 - 1) + kSwHY * (r9 / 3 - 1);
+}
+
+// Lists the touching spheres of own sphere t (LDS index) from the LDS records: cells
+// lxc - 1 .. lxc + 1 of its 9 neighbour rows.  mh: the float test's margin (float, length).
+// A pair within the margin is listed as kSwUndecided | its sorted index and decided by the
+// FP64 test after the sweep (rare: the band is 2 mh wide), which keeps FP64 registers out
+// of the candidate loop.
+constexpr uint32_t kSwUndecided = 0x80000000u;  // sphere ids are below 2^31
+template <int CAP>
+__device__ __forceinline__ void sweep_lds(const SweepLds &sh, const double4 *__restrict__ spos,
+                                          const uint32_t *__restrict__ sid, int t, double skin,
+                                          float mh, uint32_t *__restrict__ clist,
                                           int32_t *__restrict__ ccnt, DevDiag *__restrict__ diag) {
-  const uint32_t mc = me >> 2;
+  const float4 q = sh.pr[t];
+  const uint32_t me = sh.id[t];
+  const int hr = sh.row[t];
+  const int lxc = sh.cx[t] < 1 ? 1 : (sh.cx[t] > kSwBX ? kSwBX : (int)sh.cx[t]);
+  const float rqs = (q.w - mh) + (float)skin;  // own radius + skin; candidates hold r + mh
   uint32_t *out = clist + (int64_t)me * CAP;
-  uint32_t j0[9], j1[9];
-#pragma unroll
-  for (int r9 = 0; r9 < 9; r9++) {
-    const int h2 = hr + (r9 % 3 - 1) + kSwHY * (r9 / 3 - 1);
-    j0[r9] = sh.hs[h2][lxc - 1];
-    j1[r9] = sh.hs[h2][lxc + 2];
-  }
   int cnt = 0;
+  bool und = false;
+  uint32_t na = sh.hs[nb_row(hr, 0)][lxc - 1], ne = sh.hs[nb_row(hr, 0)][lxc + 2];
 #pragma unroll
   for (int r9 = 0; r9 < 9; r9++) {
-    for (uint32_t j = j0[r9]; j < j1[r9]; j++) {
-      double ox, oy, oz, orad;
-      uint32_t id;
-      if constexpr (LDS) {
-        ox = sh.x[j];
-        oy = sh.y[j];
-        oz = sh.z[j];
-        orad = sh.r[j];
-        id = sh.id[j];
-      } else {
-        const double4 o = spos[j];
-        ox = o.x;
-        oy = o.y;
-        oz = o.z;
-        orad = o.w;
-        id = sid[j];
+    uint32_t j = na;
+    const uint32_t e = ne;
+    if (r9 < 8) {  // the next row's bounds load while this row is swept
+      na = sh.hs[nb_row(hr, r9 + 1)][lxc - 1];
+      ne = sh.hs[nb_row(hr, r9 + 1)][lxc + 2];
+    }
+    for (; j < e; j++) {
+      const float4 o = sh.pr[j];
+      const uint32_t id = sh.id[j];
+      const float dx = o.x - q.x, dy = o.y - q.y, dz = o.z - q.z;
+      const float d2 = __fmaf_rn(dx, dx, __fmaf_rn(dy, dy, dz * dz));
+      const float hi = rqs + o.w;  // sep + margin
+      if (d2 < hi * hi && (id ^ me) > 3u) {  // not surely apart, another clump
+        const float lo = hi - 2.0f * mh;     // sep - margin
+        uint32_t v = id;
+        if (!(d2 < lo * lo)) {  // within the margin of sep
+          const int h2 = nb_row(hr, r9);
+          v = kSwUndecided | (sh.lo[h2] + j - sh.hoff[h2]);
+          und = true;
+        }
+        if (cnt < CAP) out[cnt] = v;
+        cnt++;
       }
-      if ((id >> 2) == mc) continue;
-      const double sep = rad + orad + skin, sep2 = sep * sep;
-      const double dx = ox - q.x, dy = oy - q.y, dz = oz - q.z;
-      const double d2 = dx * dx + dy * dy + dz * dz;
-      if (d2 < sep2) {
+    }
+  }
+  if (und) {  // the FP64 test for the undecided entries; the list is compacted in place
+    const double4 q64 = spos[sh.lo[hr] + (uint32_t)t - sh.hoff[hr]];
+    const int n = cnt < CAP ? cnt : CAP;
+    int w = 0;
+    for (int e = 0; e < n; e++) {
+      uint32_t v = out[e];
+      if (v & kSwUndecided) {
+        const uint32_t jg = v & ~kSwUndecided;
+        if (!touch_fp64(q64, spos[jg], skin)) continue;
+        v = sid[jg];
+      }
+      out[w++] = v;
+    }
+    cnt -= n - w;
+  }
+  if (cnt > CAP) {
+    atomicAdd(&diag->neigh_overflow, 1ull);
+    cnt = CAP;
+  }
+  ccnt[me] = cnt;
+}
+
+// The same listing from global memory in FP64: sh.gs holds global indices into the sorted
+// arrays; q, me: the listing sphere.
+template <int CAP>
+__device__ __forceinline__ void sweep_global(const SweepLds &sh, const double4 *__restrict__ spos,
+                                             const uint32_t *__restrict__ sid, double4 q,
+                                             uint32_t me, int hr, int lxc, double skin,
+                                             uint32_t *__restrict__ clist,
+                                             int32_t *__restrict__ ccnt,
+                                             DevDiag *__restrict__ diag) {
+  uint32_t *out = clist + (int64_t)me * CAP;
+  int cnt = 0;
+#pragma unroll 1
+  for (int r9 = 0; r9 < 9; r9++) {
+    const int h2 = nb_row(hr, r9);
+    const uint32_t e = sh.gs[h2][lxc + 2];
+    for (uint32_t j = sh.gs[h2][lxc - 1]; j < e; j++) {
+      const double4 o = spos[j];
+      const uint32_t id = sid[j];
+      if (touch_fp64(q, o, skin) && (id ^ me) > 3u) {
         if (cnt < CAP) out[cnt] = id;
         cnt++;
       }
@@ -1839,7 +1878,7 @@ __global__ __launch_bounds__(kSwBlock) void k_sweep(
     const uint32_t *__restrict__ sid, uint32_t *__restrict__ clist, int32_t *__restrict__ ccnt,
     DevDiag *__restrict__ diag, double skin) {
   __shared__ SweepLds sh;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
   const int64_t nb = *nbricks_p;
   const int64_t G = gridDim.x, L = xcd_block_runs(blockIdx.x, gridDim.x, 8);
 #pragma unroll 1
@@ -1863,7 +1902,7 @@ __global__ __launch_bounds__(kSwBlock) void k_sweep(
     const int nbx = (nx + kSwBX - 1) / kSwBX, nby = (ny + kSwBY - 1) / kSwBY;
     const int x0 = (int)(lb % nbx) * kSwBX, y0 = (int)((lb / nbx) % nby) * kSwBY;
     const int z0 = (int)(lb / ((int64_t)nbx * nby)) * kSwBZ;
-    const double lo_x = p.sg_lo[0], inv_h = p.sg_inv_h;
+    const double lo_x = p.sg_lo[0], inv_h = p.sg_inv_h, h = p.sg_h;
     // the rows' bounds: cells x0 - 1 .. x0 + kSwBX of the row, clamped into [0, nx] (cells
     // outside the grid have no width; rows outside it are empty)
     uint32_t rlen = 0u;
@@ -1885,58 +1924,81 @@ __global__ __launch_bounds__(kSwBlock) void k_sweep(
       if (tid < kSwRows) sh.hoff[tid + 1] = inc;
       if (tid == 0) sh.hoff[0] = 0u;
     }
+    if (tid == 0) {
+      sh.nown = 0u;
+      sh.far = 0u;
+    }
     __syncthreads();
     const uint32_t H = sh.hoff[kSwRows];
     if (H == 0u) continue;
-    const bool lds = H <= (uint32_t)kSwCap;
+    const float mh = kSwMargin * (float)h;
+    bool lds = H <= (uint32_t)kSwCap;
     if (lds) {
-      // the halo spheres and their cells: every thread's rows found first, then all its
-      // loads issued together (their latencies overlap), then the LDS copies
-      constexpr int kIt = kSwCap / kSwBlock;
-      int hrs[kIt];
-      uint32_t js[kIt];
-#pragma unroll
-      for (int q = 0; q < kIt; q++) {
-        const uint32_t t = tid + q * kSwBlock;
-        hrs[q] = 0;
-        js[q] = 0u;
-        if (t < H) {
-          hrs[q] = upper_row(sh.hoff, kSwRows, t);
-          js[q] = sh.lo[hrs[q]] + (t - sh.hoff[hrs[q]]);
-        }
-      }
-      double4 ov[kIt];
-      uint32_t iv[kIt];
-#pragma unroll
-      for (int q = 0; q < kIt; q++)
-        if (tid + q * kSwBlock < H) {
-          ov[q] = spos[js[q]];
-          iv[q] = sid[js[q]];
-        }
-#pragma unroll
-      for (int q = 0; q < kIt; q++) {
-        const uint32_t t = tid + q * kSwBlock;
-        if (t < H) {
-          sh.x[t] = ov[q].x;
-          sh.y[t] = ov[q].y;
-          sh.z[t] = ov[q].z;
-          sh.r[t] = ov[q].w;
-          sh.id[t] = iv[q];
-          int lx = grid_index(ov[q].x, lo_x, inv_h, nx) - (x0 - 1);
-          sh.cx[t] = (uint8_t)(lx < 0 ? 0 : (lx > kSwHX - 1 ? kSwHX - 1 : lx));
-          sh.row[t] = (uint8_t)hrs[q];
-        }
-      }
-      for (int t = tid; t < kSwRows * kSwRowStarts; t += kSwBlock)  // empty cells: row end
-        sh.hs[t / kSwRowStarts][t % kSwRowStarts] = sh.hoff[t / kSwRowStarts + 1];
+      // each row's thread marks its spheres with the row; empty cells start at the row end
+      if (tid < kSwRows)
+        for (uint32_t t = sh.hoff[tid]; t < sh.hoff[tid + 1]; t++) sh.row[t] = (uint8_t)tid;
+      for (int t = tid; t < kSwRows * kSwRowStarts; t += kSwBlock)
+        sh.hs[t / kSwRowStarts][t % kSwRowStarts] = (uint16_t)sh.hoff[t / kSwRowStarts + 1];
       __syncthreads();
-      // cell starts from the sorted cells: sphere t starts every cell after its predecessor's
-      for (uint32_t t = tid; t < H; t += kSwBlock) {
-        const int hr = sh.row[t];
-        const int prev = t == sh.hoff[hr] ? -1 : (int)sh.cx[t - 1];
-        for (int k = prev + 1; k <= (int)sh.cx[t]; k++) sh.hs[hr][k] = t;
+      // the halo spheres: all of a thread's loads issued together (their latencies overlap),
+      // then the float records, the cells along x and the list of the brick's own spheres
+      const double ox = lo_x + (double)(x0 - 1) * h, oy = p.sg_lo[1] + (double)(y0 - 1) * h,
+                   oz = p.sg_lo[2] + (double)(z0 - 1) * h, lim = kSwFar * h;
+#pragma unroll 1
+      for (uint32_t t0 = 0; t0 < H; t0 += kSwIt * kSwBlock) {
+        int hrs[kSwIt];
+        double4 ov[kSwIt];
+        uint32_t iv[kSwIt];
+#pragma unroll
+        for (int q = 0; q < kSwIt; q++) {
+          const uint32_t t = t0 + tid + q * kSwBlock;
+          hrs[q] = 0;
+          if (t < H) {
+            hrs[q] = sh.row[t];
+            const uint32_t j = sh.lo[hrs[q]] + (t - sh.hoff[hrs[q]]);
+            ov[q] = spos[j];
+            iv[q] = sid[j];
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < kSwIt; q++) {
+          const uint32_t t = t0 + tid + q * kSwBlock;
+          bool own = false;
+          if (t < H) {
+            const double ux = ov[q].x - ox, uy = ov[q].y - oy, uz = ov[q].z - oz;
+            if (!(fabs(ux) <= lim && fabs(uy) <= lim && fabs(uz) <= lim)) sh.far = 1u;
+            sh.pr[t] = make_float4((float)ux, (float)uy, (float)uz, (float)ov[q].w + mh);
+            sh.id[t] = iv[q];
+            int lx = grid_index(ov[q].x, lo_x, inv_h, nx) - (x0 - 1);
+            lx = lx < 0 ? 0 : (lx > kSwHX - 1 ? kSwHX - 1 : lx);
+            sh.cx[t] = (uint8_t)lx;
+            const int hy = hrs[q] % kSwHY, hz = hrs[q] / kSwHY;
+            own = hy >= 1 && hy <= kSwBY && hz >= 1 && hz <= kSwBZ && lx >= 1 && lx <= kSwBX;
+          }
+          const unsigned long long m = __ballot(own);
+          if (m) {  // the wave's own spheres appended together
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            uint32_t base = 0u;
+            if (lane == 0) base = atomicAdd(&sh.nown, (uint32_t)__popcll(m));
+            base = __shfl(base, 0, 64);
+            if (own) sh.own[base + rank] = (uint16_t)t;
+          }
+        }
       }
-    } else {
+      __syncthreads();
+      lds = sh.far == 0u;
+      if (lds) {
+        // cell starts from the sorted cells: sphere t starts every cell after its
+        // predecessor's
+        for (uint32_t t = tid; t < H; t += kSwBlock) {
+          const int hr = sh.row[t];
+          const int prev = t == sh.hoff[hr] ? -1 : (int)sh.cx[t - 1];
+          for (int k = prev + 1; k <= (int)sh.cx[t]; k++) sh.hs[hr][k] = (uint16_t)t;
+        }
+      }
+    }
+    if (!lds) {
       for (int t = tid; t < kSwRows * kSwRowStarts; t += kSwBlock) {
         const int hr = t / kSwRowStarts, k = t % kSwRowStarts;
         const int y = y0 - 1 + hr % kSwHY, z = z0 - 1 + hr / kSwHY;
@@ -1946,37 +2008,35 @@ __global__ __launch_bounds__(kSwBlock) void k_sweep(
           x = x < 0 ? 0 : (x > nx ? nx : x);
           v = starts[p.sg_base + (int64_t)nx * (y + (int64_t)ny * z) + x];
         }
-        sh.hs[hr][k] = v;
+        sh.gs[hr][k] = v;
+      }
+      __syncthreads();
+      if (tid < 64) {  // spheres of the brick rows (cells 1 .. kSwBX of the halo frame)
+        uint32_t len = 0u;
+        if (tid < kSwOwnRows) {
+          const int hr = (1 + tid % kSwBY) + kSwHY * (1 + tid / kSwBY);
+          len = sh.gs[hr][kSwBX + 1] - sh.gs[hr][1];
+        }
+        const uint32_t inc = wave_incl_scan(len);
+        if (tid < kSwOwnRows) sh.ownp[tid + 1] = inc;
+        if (tid == 0) sh.ownp[0] = 0u;
       }
     }
     __syncthreads();
-    if (tid < 64) {  // spheres of the brick rows (cells 1 .. kSwBX of the halo frame)
-      uint32_t len = 0u;
-      if (tid < kSwOwnRows) {
-        const int hr = (1 + tid % kSwBY) + kSwHY * (1 + tid / kSwBY);
-        len = sh.hs[hr][kSwBX + 1] - sh.hs[hr][1];
-      }
-      const uint32_t inc = wave_incl_scan(len);
-      if (tid < kSwOwnRows) sh.own[tid + 1] = inc;
-      if (tid == 0) sh.own[0] = 0u;
-    }
-    __syncthreads();
-    const uint32_t K = sh.own[kSwOwnRows];
-    for (uint32_t k = tid; k < K; k += kSwBlock) {
-      const int o = upper_row(sh.own, kSwOwnRows, k);
-      const int hr = (1 + o % kSwBY) + kSwHY * (1 + o / kSwBY);
-      const uint32_t j = sh.hs[hr][1] + (k - sh.own[o]);  // LDS or global index
-      if (lds) {
-        const d3 q = make_d3(sh.x[j], sh.y[j], sh.z[j]);
-        const int lxc = sh.cx[j] < 1 ? 1 : (sh.cx[j] > kSwBX ? kSwBX : (int)sh.cx[j]);
-        sweep_one<CAP, true>(sh, spos, sid, q, sh.r[j], sh.id[j], hr, lxc, skin, clist, ccnt,
-                             diag);
-      } else {
+    if (lds) {
+      const uint32_t K = sh.nown;
+      for (uint32_t k = tid; k < K; k += kSwBlock)
+        sweep_lds<CAP>(sh, spos, sid, sh.own[k], skin, mh, clist, ccnt, diag);
+    } else {
+      const uint32_t K = sh.ownp[kSwOwnRows];
+      for (uint32_t k = tid; k < K; k += kSwBlock) {
+        const int o = upper_row(sh.ownp, kSwOwnRows, k);
+        const int hr = (1 + o % kSwBY) + kSwHY * (1 + o / kSwBY);
+        const uint32_t j = sh.gs[hr][1] + (k - sh.ownp[o]);
         const double4 v = spos[j];
-        const d3 q = make_d3(v.x, v.y, v.z);
-        int lxc = grid_index(q.x, lo_x, inv_h, nx) - x0 + 1;
+        int lxc = grid_index(v.x, lo_x, inv_h, nx) - x0 + 1;
         lxc = lxc < 1 ? 1 : (lxc > kSwBX ? kSwBX : lxc);
-        sweep_one<CAP, false>(sh, spos, sid, q, v.w, sid[j], hr, lxc, skin, clist, ccnt, diag);
+        sweep_global<CAP>(sh, spos, sid, v, sid[j], hr, lxc, skin, clist, ccnt, diag);
       }
     }
   }

@@ -274,6 +274,7 @@ def test_window_rejects_unsupported_states(gpu, state):
     ctx.close()
     funnel = workloads.build(workloads.FlumeSpec(n=100, funnel_width=6.0, funnel_length=4.0))
     c2 = dem.Context([funnel])
+    c2.regrid()
     c2.step(2)
     with pytest.raises(dem.AvfError):
         c2.window_run(10)
