@@ -56,7 +56,7 @@ struct avf_ctx {
   std::vector<hipEvent_t> prof_ev;  // timing events of the profiled / timed steps
   int32_t *d_orig = nullptr, *d_orig2 = nullptr, *d_inst = nullptr, *d_inst2 = nullptr;
   int64_t *d_inv = nullptr, *d_ncells = nullptr, *d_nbricks = nullptr, *d_tmp64 = nullptr;
-  uint32_t *d_counts = nullptr, *d_starts = nullptr, *d_bsum = nullptr, *d_cursor = nullptr;
+  uint32_t *d_counts = nullptr, *d_starts = nullptr, *d_bsum = nullptr;
   uint32_t *d_skey = nullptr, *d_srank = nullptr, *d_sid = nullptr, *d_clist = nullptr;
   int32_t *d_ccnt = nullptr;
   double4 *d_spos = nullptr;
@@ -230,18 +230,19 @@ unsigned scan_grid(const avf_ctx *c) {
 void launch_scan(avf_ctx *c, uint32_t *starts, int finalize_prev = 0, int advance = 0) {
   const unsigned G = scan_grid(c);
   k_count_sum<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
-  k_count_scan<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, starts, c->d_cursor,
+  k_count_scan<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, starts,
                                                 c->d_ncells, c->d_P, c->d_acc_cell, c->ninst,
                                                 finalize_prev, advance);
 }
 
-// Spheres into cell order at the positions of c->d_cursor (left by the scan).  ccnt: the
-// per-sphere counts of the sweep that follows (those of clumps not binned are zeroed here);
-// NULL for a binning that no sweep reads.
-void launch_scatter(avf_ctx *c, const uint32_t *skey, const double *stage, double4 *spos,
-                    uint32_t *sid, int32_t *ccnt) {
+// Spheres into cell order at their cell's start (from the scan) + their rank (from the
+// count).  ccnt: the per-sphere counts of the sweep that follows (those of clumps not binned
+// are zeroed here); NULL for a binning that no sweep reads.
+void launch_scatter(avf_ctx *c, const uint32_t *skey, const uint32_t *srank,
+                    const uint32_t *starts, const double *stage, double4 *spos, uint32_t *sid,
+                    int32_t *ccnt) {
   k_scatter<<<std::max(1u, grid_for(4 * c->n, kScatterBlock)), kScatterBlock, 0, c->stream>>>(
-      skey, c->d_cursor, stage, spos, sid, c->d_inst, c->d_P, ccnt, 4 * c->n);
+      skey, srank, starts, stage, spos, sid, c->d_inst, c->d_P, ccnt, 4 * c->n);
 }
 
 // Binning of the staged spheres: predict (mode 1, advancing the time) or stage (mode 0),
@@ -249,9 +250,10 @@ void launch_scatter(avf_ctx *c, const uint32_t *skey, const double *stage, doubl
 void launch_bin(avf_ctx *c, int mode) {
   k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
       c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-      c->n, c->cap, mode, ListTrack{nullptr, nullptr});
+      c->d_srank, c->n, c->cap, mode, ListTrack{nullptr, nullptr});
   launch_scan(c, c->d_starts, 0, mode == 1);
-  launch_scatter(c, c->d_skey, c->d_stage, c->d_spos, c->d_sid, c->d_ccnt);
+  launch_scatter(c, c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid,
+                 c->d_ccnt);
 }
 
 ListTrack list_track(avf_ctx *c) { return ListTrack{c->d_dsp, c->d_wreach}; }
@@ -265,7 +267,7 @@ void launch_force_bs(avf_ctx *c, int initial, bool ft, int time_ahead, const dou
       c->d_st, c->d_sta, stage_in, c->d_inst, c->d_P, c->d_clist, c->d_ccnt, c->d_vl_m,
       c->d_vl_mask, c->d_vl_cnt, ft ? c->d_f : nullptr, c->d_diag, c->d_acc_cell, stage_out,
       c->d_orig, VERLET ? list_track(c) : ListTrack{nullptr, nullptr}, c->n, c->cap,
-      initial, time_ahead, si, c->d_counts, c->d_skey);
+      initial, time_ahead, si, c->d_counts, c->d_skey, c->d_srank);
 }
 
 // Touching spheres of the binned spheres (skin 0) or candidate spheres within sep + skin.
@@ -367,16 +369,16 @@ void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bo
   if (i == 0)
     k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
         c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-        c->n, c->cap, 1, ListTrack{nullptr, nullptr});
+        c->d_srank, c->n, c->cap, 1, ListTrack{nullptr, nullptr});
   mark(AVF_K_SCAN_A);
   k_count_sum<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
   mark(AVF_K_SCAN_B);
   k_count_scan<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_starts,
-                                                c->d_cursor, c->d_ncells, c->d_P, c->d_acc_cell,
+                                                c->d_ncells, c->d_P, c->d_acc_cell,
                                                 c->ninst, i > 0, 1);
   mark(AVF_K_SCAN_C);
   mark(AVF_K_SCATTER);
-  launch_scatter(c, c->d_skey, cur, c->d_spos, c->d_sid, c->d_ccnt);
+  launch_scatter(c, c->d_skey, c->d_srank, c->d_starts, cur, c->d_spos, c->d_sid, c->d_ccnt);
   mark(AVF_K_DETECT);
   launch_detect(c);
   mark(AVF_K_FORCE);
@@ -410,18 +412,19 @@ void launch_vstep(avf_ctx *c, bool rebuild, hipEvent_t *ev = nullptr, bool ft = 
   if (rebuild || i == 0)
     k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
         c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-        c->n, c->cap, rebuild ? 1 : 2, list_track(c));
+        c->d_srank, c->n, c->cap, rebuild ? 1 : 2, list_track(c));
   mark(AVF_K_SCAN_A);
   if (rebuild) k_count_sum<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
   mark(AVF_K_SCAN_B);
   if (rebuild)  // also advances the time (between the predict and the force stage)
     k_count_scan<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_starts,
-                                                  c->d_cursor, c->d_ncells, c->d_P,
+                                                  c->d_ncells, c->d_P,
                                                   c->d_acc_cell, c->ninst, 0, 1);
   mark(AVF_K_SCAN_C);
   mark(AVF_K_SCATTER);
   if (rebuild)
-    launch_scatter(c, c->d_skey, c->d_stage, c->d_spos, c->d_sid, c->d_ccnt);
+    launch_scatter(c, c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid,
+                   c->d_ccnt);
   mark(AVF_K_DETECT);
   if (rebuild) {
     launch_detect(c, c->opt.verlet_skin);
@@ -897,7 +900,6 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
       (rc = dalloc(c, &c->d_nbricks, 1)) || (rc = dalloc(c, &c->d_stage2, N * kStageDoubles)) ||
       (rc = dalloc(c, &c->d_tmp64, N)) || (rc = dalloc(c, &c->d_counts, c->cell_cap)) ||
       (rc = dalloc(c, &c->d_starts, c->cell_cap + 1)) ||
-      (rc = dalloc(c, &c->d_cursor, c->cell_cap)) ||
       (rc = dalloc(c, &c->d_bsum, grid_for(c->cell_cap, kScanTile) + 1)) ||
       (rc = dalloc(c, &c->d_skey, 4 * N)) || (rc = dalloc(c, &c->d_srank, 4 * N)) ||
       (rc = dalloc(c, &c->d_sid, 4 * N)) || (rc = dalloc(c, &c->d_spos, 4 * N)) ||
@@ -1017,7 +1019,7 @@ int avf_destroy(avf_ctx *c) {
   void *bufs[] = {c->d_P,      c->d_st,    c->d_st2,   c->d_sta,   c->d_sta2,  c->d_stage,
                   c->d_f,      c->d_io,    c->d_io2,   c->d_orig,  c->d_orig2, c->d_inst,
                   c->d_inst2,  c->d_inv,   c->d_ncells, c->d_nbricks, c->d_tmp64, c->d_counts,
-                  c->d_starts, c->d_cursor,
+                  c->d_starts,
                   c->d_bsum,   c->d_skey,  c->d_srank, c->d_sid,   c->d_spos,  c->d_acc_cell,
                   c->d_acc_sg, c->d_diag,  c->d_count, c->d_clist, c->d_ccnt,
                   c->d_vl_m,   c->d_vl_cnt, c->d_vl_mask, c->d_dsp, c->d_wreach,
@@ -1659,9 +1661,9 @@ void win_bin(avf_ctx *c, const double *st) {
   WinBuf *w = c->win;
   k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
       const_cast<double *>(st), c->d_sta, w->bstage, c->d_inst, c->d_orig, c->d_P, c->d_counts,
-      w->bkey, c->n, c->cap, 0, ListTrack{nullptr, nullptr});
+      w->bkey, w->brank, c->n, c->cap, 0, ListTrack{nullptr, nullptr});
   launch_scan(c, w->bstarts);
-  launch_scatter(c, w->bkey, w->bstage, w->bspos, w->bsid, nullptr);
+  launch_scatter(c, w->bkey, w->brank, w->bstarts, w->bstage, w->bspos, w->bsid, nullptr);
 }
 
 // Adds the touching sphere pairs of state st that the frozen list lacks to w->missed.

@@ -183,18 +183,20 @@ __device__ __forceinline__ void track_update(ListTrack lt, int64_t c, d3 r0, d3 
   if (d > 0.0) lt.dsp[c] += d;
 }
 
-// Counting-sort counts of a clump's four sphere-cell keys (one atomic per binned sphere,
-// whose result is not used: no wait on its return; the positions within a cell are taken
-// later by k_scatter) and the keys themselves.  Measured against staging the counts of a
-// workgroup in an LDS hash table first (one atomic per distinct cell): the staging cost more
-// than the atomics it saved, in the force kernel (49.5 -> 63 us per launch at 250,000 clumps)
-// and in the scatter (23.5 -> 42.6 us).
+// Counting-sort counts of a clump's four sphere-cell keys, one atomic per binned sphere
+// whose return value is the sphere's rank within its cell (k_scatter then places it at the
+// cell's start + rank, with no atomics of its own), and the keys and ranks themselves.
+// Measured against staging the counts of a workgroup in an LDS hash table first (one atomic
+// per distinct cell): the staging cost more than the atomics it saved, in the force kernel
+// (49.5 -> 63 us per launch at 250,000 clumps) and in the scatter (23.5 -> 42.6 us).
 __device__ __forceinline__ void count_keys(const uint32_t *keys, uint32_t *__restrict__ counts,
-                                           uint32_t *__restrict__ skey, int64_t c) {
+                                           uint32_t *__restrict__ skey,
+                                           uint32_t *__restrict__ srank, int64_t c) {
+  uint32_t r[4];
 #pragma unroll
-  for (int s = 0; s < 4; s++)
-    if (keys[s] != kInvalidKey) atomicAdd(&counts[keys[s]], 1u);
+  for (int s = 0; s < 4; s++) r[s] = keys[s] != kInvalidKey ? atomicAdd(&counts[keys[s]], 1u) : 0u;
   reinterpret_cast<uint4 *>(skey)[c] = make_uint4(keys[0], keys[1], keys[2], keys[3]);
+  reinterpret_cast<uint4 *>(srank)[c] = make_uint4(r[0], r[1], r[2], r[3]);
 }
 
 // Sphere-cell keys of a clump's four spheres at the force-stage time t_new.  Sphere centres
@@ -307,7 +309,7 @@ __global__ __launch_bounds__(kPredictBlock) void k_predict(
     double *__restrict__ st, double *__restrict__ sta, double *__restrict__ stage,
     const int32_t *__restrict__ inst_of, const int32_t *__restrict__ orig,
     const InstParams *__restrict__ P, uint32_t *__restrict__ counts, uint32_t *__restrict__ skey,
-    int64_t n, int64_t cap, int mode, ListTrack lt) {
+    uint32_t *__restrict__ srank, int64_t n, int64_t cap, int mode, ListTrack lt) {
   __shared__ double rec[kPredictBlock * kStagePad];
   const int64_t c0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kPredictBlock;
   const int64_t c = c0 + threadIdx.x;
@@ -323,7 +325,7 @@ __global__ __launch_bounds__(kPredictBlock) void k_predict(
       }
     }
   }
-  if (mode != 2 && c < n) count_keys(keys, counts, skey, c);
+  if (mode != 2 && c < n) count_keys(keys, counts, skey, srank, c);
   __syncthreads();
   const int64_t rows = n - c0 < kPredictBlock ? n - c0 : kPredictBlock;
   double *dst = stage + c0 * kStageDoubles;
@@ -403,16 +405,16 @@ __global__ __launch_bounds__(kScanBlock) void k_count_sum(const uint32_t *__rest
 
 // ----------------------------------------------------------------------- scatter ----
 
-// One lane per sphere: the record (x, y, z, r) + id 4 m + s of each binned sphere at the next
-// free position of its cell, taken from cursor (the scanned starts, copied by k_count_scan)
-// with an atomic.  The centre is the predict's (sphere_of on the stage record's centre and
+// One lane per sphere: the record (x, y, z, r) + id 4 m + s of each binned sphere at its
+// cell's start + its rank within the cell (from the count's atomic).  The centre is the predict's (sphere_of on the stage record's centre and
 // rotation is bit-identical to sphere_centres).  The sweep lists the touching spheres of
 // binned spheres only; ccnt (if given) of a sphere that is not binned (its clump is past
 // side.x) is set to 0 here, so every per-sphere count is written each binning.
 constexpr int kScatterBlock = 256;
 __global__ __launch_bounds__(kScatterBlock) void k_scatter(
-    const uint32_t *__restrict__ skey, uint32_t *__restrict__ cursor,
-    const double *__restrict__ stage, double4 *__restrict__ spos, uint32_t *__restrict__ sid,
+    const uint32_t *__restrict__ skey, const uint32_t *__restrict__ srank,
+    const uint32_t *__restrict__ starts, const double *__restrict__ stage,
+    double4 *__restrict__ spos, uint32_t *__restrict__ sid,
     const int32_t *__restrict__ inst_of, const InstParams *__restrict__ P,
     int32_t *__restrict__ ccnt, int64_t nsph) {
   const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kScatterBlock + threadIdx.x;
@@ -422,7 +424,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter(
     if (ccnt) ccnt[i] = 0;
     return;
   }
-  const uint32_t dst = atomicAdd(&cursor[key], 1u);
+  const uint32_t dst = starts[key] + srank[i];
   const double *g = stage + (i >> 2) * kStageDoubles;
   const double rad = g[G_RAD];  // binned: not frozen, the pre-freeze radius
   const d3 q = rec_sphere(g, rad / P[inst_of[i >> 2]].sphere_div, (int)(i & 3));
@@ -1390,7 +1392,8 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
     const int32_t *__restrict__ vl_cnt, double *__restrict__ fout, DevDiag *__restrict__ diag,
     long long *__restrict__ acc, double *__restrict__ stage_out,
     const int32_t *__restrict__ orig, ListTrack lt, int64_t n, int64_t cap, int initial,
-    int time_ahead, int si, uint32_t *__restrict__ counts, uint32_t *__restrict__ skey) {
+    int time_ahead, int si, uint32_t *__restrict__ counts, uint32_t *__restrict__ skey,
+    uint32_t *__restrict__ srank) {
   static_assert(!NEXT || VERLET || CELLS, "the fused predict needs the record-resident state");
   static_assert(sizeof(PPShared<MAXN, BS>) >= sizeof(double) * BS * kStagePad,
                 "stage records of the workgroup are assembled in the pair scratch");
@@ -1483,7 +1486,7 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
       }
     }
     if constexpr (CELLS)  // sphere-cell counts of the next step
-      if (c < n) count_keys(keys, counts, skey, c);
+      if (c < n) count_keys(keys, counts, skey, srank, c);
     __syncthreads();
     const int64_t rows = n - c0 < BS ? n - c0 : BS;
     double *dst = stage_out + c0 * kStageDoubles;
@@ -1622,13 +1625,11 @@ __device__ __forceinline__ void step_params(InstParams &p, long long *acc, int f
   if (advance) p.time = p.time + p.dt;
 }
 
-// Exclusive scan of the counts into starts and cursor (the next free position of each cell,
-// for k_scatter), the counts back to zero; starts[ncells] receives the total.  Workgroup 0's first lane also applies step_params to every instance
+// Exclusive scan of the counts into starts, the counts back to zero; starts[ncells] receives the total.  Workgroup 0's first lane also applies step_params to every instance
 // (no other kernel runs meanwhile; this kernel reads no parameter).
 __global__ __launch_bounds__(kScanBlock) void k_count_scan(
     uint32_t *__restrict__ counts, const uint32_t *__restrict__ bsum,
-    uint32_t *__restrict__ starts, uint32_t *__restrict__ cursor,
-    const int64_t *__restrict__ ncells_p,
+    uint32_t *__restrict__ starts, const int64_t *__restrict__ ncells_p,
     InstParams *__restrict__ P, long long *__restrict__ acc, int32_t ninst, int finalize_prev,
     int advance) {
   __shared__ uint32_t lds[kScanBlock / 64];
@@ -1674,7 +1675,6 @@ __global__ __launch_bounds__(kScanBlock) void k_count_scan(
       if (i < hi) {
         const uint32_t st0 = woff + inc[k] - v[k];
         starts[i] = st0;
-        cursor[i] = st0;
         counts[i] = 0u;
       }
       woff += __shfl(inc[k], 63, 64);
