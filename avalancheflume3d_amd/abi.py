@@ -223,6 +223,7 @@ def load_library() -> C.CDLL:
             "avf_step_profiled": (C.c_int, [vp, i64, vp, vp]),
             "avf_step_timed": (C.c_int, [vp, i64, i32, i32, vp, vp]),
             "avf_observe": (C.c_int, [vp, i32, vp, vp, vp]),
+            "avf_observe_noalloc": (C.c_int, [vp, i32, vp, vp, vp]),
             "avf_window_run": (C.c_int, [vp] * 15),
         }
         for name, (res, args) in sig.items():
@@ -243,7 +244,8 @@ EXPORTED = [
     "avf_regrid", "avf_get_cell", "avf_initial_forces", "avf_step", "avf_get_forces",
     "avf_get_clump_cells", "avf_check_cells", "avf_get_contacts", "avf_get_cell_table",
     "avf_get_sphere_keys", "avf_get_diag", "avf_sync",
-    "avf_step_profiled", "avf_step_timed", "avf_observe", "avf_window_run",
+    "avf_step_profiled", "avf_step_timed", "avf_observe", "avf_observe_noalloc",
+    "avf_window_run",
 ]
 KERNELS = ["predict", "scan_a", "scan_b", "scan_c", "scatter", "detect", "force", "cellreduce",
            "finalize"]

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -75,7 +76,9 @@ struct avf_ctx {
   struct HostMail {
     unsigned long long umax;
     DevDiag diag;
+    double obs[kObsScalars + kMaxBins];  // avf_observe_noalloc's sums
   } *h_mail = nullptr;
+  double *d_scratch = nullptr;  // this device's address of scratch (avf_observe_noalloc)
   InstParams *h_P = nullptr;
   unsigned long long seen_below = 0, seen_overflow = 0;  // counters at the last check
   bool list_valid = false;
@@ -98,6 +101,15 @@ struct avf_ctx {
   int32_t prof_every = 1;        // avf_step_timed: events on every prof_every-th step
   int64_t prof_seq = 0;          // steps launched under the profiler
 };
+
+// Block partial sums and totals of avf_observe_noalloc: a fixed device array of
+// kObsScratchDoubles doubles (8 MiB), one copy on each GPU the library runs on, held from
+// the code object's load until the process ends.  Every call on a GPU shares that GPU's copy
+// (calls that overlap on one GPU race); it keeps the last call's contents, and nothing in it
+// needs to start at zero (k_observe writes every partial that k_observe_final reads).  Host
+// code uses the address hipGetSymbolAddress gives for the current device, never the name.
+constexpr int64_t kObsScratchDoubles = int64_t{1} << 20;
+static __device__ __align__(256) double scratch[kObsScratchDoubles];
 
 namespace {
 
@@ -123,6 +135,27 @@ int fail(avf_ctx *ctx, int code, const char *fmt, ...) {
   } while (0)
 
 inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// scratch's address on each device, looked up once per device by the first context created
+// on it (that device current).
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kScratchDevices] = {};
+
+int scratch_address(avf_ctx *c) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (c->device < 0 || c->device >= kScratchDevices)
+    return fail(c, AVF_ERR_ARG, "device %d out of range", c->device);
+  if (!g_scratch[c->device]) {
+    void *p = nullptr;
+    const hipError_t he = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (he != hipSuccess || !p)
+      return fail(c, AVF_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(he));
+    g_scratch[c->device] = static_cast<double *>(p);
+  }
+  c->d_scratch = g_scratch[c->device];
+  return AVF_OK;
+}
 
 // One k_copy_set launch: {source, destination, 4-byte words} per range (unused ranges empty).
 struct CopyRange {
@@ -913,6 +946,7 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
     fail(c, AVF_ERR_NOMEM, "hipHostMalloc: %s", hipGetErrorString(he));
     return bail(AVF_ERR_NOMEM);
   }
+  if ((rc = scratch_address(c))) return bail(rc);
   if (c->opt.verlet_skin > 0.0 &&
       ((rc = dalloc(c, &c->d_vl_m, kVMax * N)) || (rc = dalloc(c, &c->d_vl_mask, kVMax * N)) ||
        (rc = dalloc(c, &c->d_vl_cnt, N)) || (rc = dalloc(c, &c->d_dsp, N)) || (rc = dalloc(c, &c->d_wreach, N)) ||
@@ -1482,7 +1516,7 @@ int profile_params(avf_ctx *c, const avf_profile_spec *spec, int *nbins, double 
   return AVF_OK;
 }
 
-void fill_observables(avf_ctx *c, int32_t inst, const std::vector<double> &h, avf_observables *obs) {
+void fill_observables(avf_ctx *c, int32_t inst, const double *h, avf_observables *obs) {
   const int64_t cnt = c->desc[inst].sys.npart;
   obs->vmean = h[0] / (double)cnt;  // avalanche_hst.cu:573-574 (vmean /= nn)
   obs->wmean = h[1] / (double)cnt;
@@ -1514,8 +1548,39 @@ int avf_observe(avf_ctx *c, int32_t inst, avf_observables *obs, const avf_profil
     return rc;
   std::vector<double> h;
   if ((rc = observe_state(c, c->d_st, inst, nbins, x0, bw, sigma, width, h))) return rc;
-  if (obs) fill_observables(c, inst, h, obs);
+  if (obs) fill_observables(c, inst, h.data(), obs);
   for (int b = 0; b < nbins; b++) profile[b] = h[kObsScalars + b];
+  return AVF_OK;
+}
+
+int avf_observe_noalloc(avf_ctx *c, int32_t inst, avf_observables *obs,
+                        const avf_profile_spec *spec, double *profile) {
+  int rc = check_inst(c, inst);
+  if (rc) return rc;
+  int nbins;
+  double x0, bw, sigma, width;
+  if ((rc = profile_params(c, profile ? spec : nullptr, &nbins, &x0, &bw, &sigma, &width)))
+    return rc;
+  // the same kernels and grid as avf_observe, the block partials and the sums in scratch
+  const int64_t cnt = c->desc[inst].sys.npart;
+  const int64_t nb = (cnt + kObsBlock - 1) / kObsBlock;
+  const int64_t w = kObsScalars + nbins;
+  if ((nb + 1) * w > kObsScratchDoubles)
+    return fail(c, AVF_ERR_NOMEM,
+                "avf_observe_noalloc: %lld clumps with %d bins need %lld doubles of scratch, "
+                "more than its %lld", (long long)cnt, nbins, (long long)((nb + 1) * w),
+                (long long)kObsScratchDoubles);
+  double *part = c->d_scratch, *sums = c->d_scratch + nb * w;
+  k_observe<<<(unsigned)nb, kObsBlock, 0, c->stream>>>(c->d_st, c->d_sta, c->d_inv, c->d_P, inst,
+                                                      cnt, c->cap, nbins, x0, bw, sigma, width,
+                                                      part);
+  k_observe_final<<<1, 1, 0, c->stream>>>(part, nb, nbins, sums);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(c->h_mail->obs, sums, w * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (obs) fill_observables(c, inst, c->h_mail->obs, obs);
+  for (int b = 0; b < nbins; b++) profile[b] = c->h_mail->obs[kObsScalars + b];
   return AVF_OK;
 }
 
@@ -1863,7 +1928,7 @@ extern "C" int avf_window_run(avf_ctx *c, const avf_window_spec *spec, const dou
     std::vector<double> h;
     if ((rc = observe_state(c, fin, i, nbins, x0, bw, sigma, width, h))) return rc;
     avf_observables o;
-    fill_observables(c, i, h, &o);
+    fill_observables(c, i, h.data(), &o);
     double L = spec->w_runout * o.runout_mean;
     for (int b = 0; b < nbins; b++) {
       const double tb = target ? target[(size_t)i * nbins + b] : 0.0;

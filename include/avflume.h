@@ -331,6 +331,15 @@ typedef struct avf_profile_spec {
 /* Fills *obs and, if spec and profile are given, profile[0..nbins). */
 int avf_observe(avf_ctx *ctx, int32_t inst, avf_observables *obs, const avf_profile_spec *spec,
                 double *profile);
+/* avf_observe with no device allocation.  No counterpart in the reference (its statistics
+ * are host loops); for callers that cannot allocate device memory: monitoring loops of a
+ * serving process, code run next to graph capture, harnesses that forbid hipMalloc.  Same
+ * kernels, grid and results as avf_observe; the block partial sums live in a fixed device
+ * array of 1,048,576 doubles (8 MiB) on each GPU, held until the process ends and shared by
+ * every call on that GPU, so calls that overlap on one GPU race.  AVF_ERR_NOMEM when
+ * (ceil(npart / 256) + 1) * (9 + nbins) exceeds it (e.g. above 3.6M clumps with 64 bins). */
+int avf_observe_noalloc(avf_ctx *ctx, int32_t inst, avf_observables *obs,
+                        const avf_profile_spec *spec, double *profile);
 
 /* Kernels of one step, in launch order (indices of avf_step_profiled's kernel_ms).  Slots
  * of kernels a step does not launch read as empty (near 0): the binning kernels on
