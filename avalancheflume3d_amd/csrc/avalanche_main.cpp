@@ -303,7 +303,16 @@ int main(int argc, char **argv) {
   CK(avf_regrid(ctx));
   if (!have_acc) CK(avf_initial_forces(ctx));
 
-  double vmean = 1e300;  // set by the first output step (cc == 0)
+  // the stop tests read vmean as last set: before the first output step, the sum (not the
+  // mean) of |v| over the clumps of the start state (avalanche_hst.cu:266-300), which only a
+  // restart between output steps ever reads
+  double vmean = 0.0;
+  for (int64_t i = 0; i < nn; i++) {
+    double vm2 = pp[i].vv.x * pp[i].vv.x;
+    vm2 += pp[i].vv.y * pp[i].vv.y;
+    vm2 += pp[i].vv.z * pp[i].vv.z;
+    vmean += std::sqrt(vm2);
+  }
   const auto t_start = std::chrono::steady_clock::now();
   int64_t total = 0;
   auto budget = [&](int64_t k) { return max_steps < 0 ? k : std::min<int64_t>(k, max_steps - total); };
