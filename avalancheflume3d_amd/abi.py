@@ -248,4 +248,4 @@ EXPORTED = [
     "avf_window_run",
 ]
 KERNELS = ["predict", "scan_a", "scan_b", "scan_c", "scatter", "detect", "force", "cellreduce",
-           "finalize"]
+           "finalize", "count"]

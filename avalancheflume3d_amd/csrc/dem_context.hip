@@ -383,13 +383,14 @@ void launch_cforce(avf_ctx *c, bool ft, const double *stage_in, double *stage_ou
 
 // Step i of a chunk of S steps with the sphere cells rebuilt every step
 // (avalanche_hst.cu:669-710 or :439-467, plus the regrid and cc++).  The chunk's first step
-// predicts from st (k_predict, keys and LDS-staged counts); every step then scans the counts
+// predicts from st (k_predict, keys, counts and ranks); every step then scans the counts
 // (applying the previous step's regrid and cc++ and advancing the time), places the spheres
 // in cell order, sweeps the cells for touching spheres and runs the force stage, which also
-// predicts the next step into the other stage buffer with its keys and counts; the chunk's
-// last force stage writes the state back to st instead and one k_finalize applies its
-// regrid and cc++.  ev (optional, AVF_NKERNELS + 1 events) brackets the kernels in the slots
-// of avf_step_profiled (SCAN_A: count sums, SCAN_B: scan, SCAN_C: empty, DETECT: sweep).
+// predicts the next step into the other stage buffer with its keys, and k_count_ranks counts
+// those keys in this step's sorted order; the chunk's last force stage writes the state
+// back to st instead and one k_finalize applies its regrid and cc++.  ev (optional,
+// AVF_NKERNELS + 1 events) brackets the kernels in the slots of avf_step_profiled (SCAN_A:
+// count sums, SCAN_B: scan, SCAN_C: empty, DETECT: sweep, COUNT: the next step's counts).
 void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bool ft = true) {
   auto mark = [&](int k) {
     if (ev && (c->prof_kernel < 0 || k == c->prof_kernel || k == c->prof_kernel + 1))
@@ -421,6 +422,10 @@ void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bo
   if (i + 1 == S)
     k_finalize<<<grid_for(c->ninst, 64), 64, 0, c->stream>>>(c->d_P, c->d_acc_cell, c->ninst, 0,
                                                              1, 0, 1);
+  mark(AVF_K_COUNT);
+  if (i + 1 < S)  // the next step's counts and ranks, in this step's sorted order
+    k_count_ranks<<<std::max(1u, grid_for(4 * c->n, 256)), 256, 0, c->stream>>>(
+        c->d_skey, c->d_sid, c->d_starts, c->d_ncells, c->d_counts, c->d_srank);
   mark(AVF_NKERNELS);
 }
 
@@ -480,6 +485,7 @@ void launch_vstep(avf_ctx *c, bool rebuild, hipEvent_t *ev = nullptr, bool ft = 
   else if (i + 1 == S)
     k_finalize<<<grid_for(c->ninst, 64), 64, 0, c->stream>>>(c->d_P, c->d_acc_cell, c->ninst, 0,
                                                              1, 1, (int)S);
+  mark(AVF_K_COUNT);  // per-step binning only
   mark(AVF_NKERNELS);
 }
 
@@ -771,6 +777,7 @@ int csteps_profiled(avf_ctx *c, int64_t S, StepProf *prof) {
       if (j == AVF_K_SCAN_C || j == AVF_K_CELLREDUCE) continue;
       if (j == AVF_K_PREDICT && timed[t] != 0) continue;
       if (j == AVF_K_FINALIZE && timed[t] + 1 != S) continue;
+      if (j == AVF_K_COUNT && timed[t] + 1 == S) continue;
       const size_t i = (size_t)timed[t];
       float ms = 0.f;
       he = hipEventElapsedTime(&ms, ev[i * kEv + j], ev[i * kEv + j + 1]);

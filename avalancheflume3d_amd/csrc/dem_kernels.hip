@@ -403,6 +403,28 @@ __global__ __launch_bounds__(kScanBlock) void k_count_sum(const uint32_t *__rest
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
+// The next step's cell counts on per-step binning, in the sorted order of the step just
+// binned: lane j takes the sphere at sorted position j, reads the key the force stage gave it
+// for the next step and counts it with a returning atomic whose old value is the sphere's
+// rank in its cell.  Spheres move little in a step, so the lanes of a wavefront count into a
+// few adjacent counters and their atomics share cache lines (measured at 250,000 clumps:
+// 9.2 us, against 17.0 us counting in clump order and about 16 us inside the force kernel).
+// Every sphere with a key for the next step was binned in this one: a clump that is not
+// binned is frozen (past side.x after t = 0), and a frozen clump neither moves nor thaws
+// within a chunk (side.x only changes between chunks), so it stays unbinned.
+__global__ __launch_bounds__(256) void k_count_ranks(const uint32_t *__restrict__ skey,
+                                                     const uint32_t *__restrict__ sid,
+                                                     const uint32_t *__restrict__ starts,
+                                                     const int64_t *__restrict__ ncells_p,
+                                                     uint32_t *__restrict__ counts,
+                                                     uint32_t *__restrict__ srank) {
+  const int64_t j = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  if (j >= (int64_t)starts[*ncells_p]) return;
+  const uint32_t id = sid[j];
+  const uint32_t key = skey[id];
+  if (key != kInvalidKey) srank[id] = atomicAdd(&counts[key], 1u);
+}
+
 // ----------------------------------------------------------------------- scatter ----
 
 // One lane per sphere: the record (x, y, z, r) + id 4 m + s of each binned sphere at its
@@ -1485,8 +1507,9 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
         }
       }
     }
-    if constexpr (CELLS)  // sphere-cell counts of the next step
-      if (c < n) count_keys(keys, counts, skey, srank, c);
+    if constexpr (CELLS)  // sphere-cell keys of the next step (counted by k_count_ranks)
+      if (c < n)
+        reinterpret_cast<uint4 *>(skey)[c] = make_uint4(keys[0], keys[1], keys[2], keys[3]);
     __syncthreads();
     const int64_t rows = n - c0 < BS ? n - c0 : BS;
     double *dst = stage_out + c0 * kStageDoubles;

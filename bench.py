@@ -49,11 +49,10 @@ DEFAULT_REORDER = 400
 # Algorithmic bytes per clump and launch of k_force on a per-step-binning step (the chunk's
 # steps but its last): the clump's stage record (192 B), the angular acceleration (24 B),
 # mass and inertia (32 B), the instance index (4 B), the four per-sphere contact counts
-# (16 B) read; the next stage record (192 B), the angular acceleration (24 B), the four
-# sphere-cell keys (16 B) and the spheres' ranks within their cells (16 B) written.  The
-# contact entries (4 B each), the neighbours' records (gathered, cache hits) and the count
-# increments are not counted.
-CELLS_FORCE_BYTES = 516.0
+# (16 B) read; the next stage record (192 B), the angular acceleration (24 B) and the four
+# sphere-cell keys (16 B) written.  The contact entries (4 B each) and the neighbours'
+# records (gathered, cache hits) are not counted; the keys are counted by their own kernel.
+CELLS_FORCE_BYTES = 500.0
 # Algorithmic bytes per clump of the contact sweep: each binned sphere's sorted record
 # (x, y, z, r: 32 B) and id (4 B) read once, two cell starts per sphere (the grid holds about
 # two cells per sphere: 8 B), its contact count (4 B) written; contact entries not counted.

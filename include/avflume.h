@@ -344,11 +344,12 @@ int avf_observe_noalloc(avf_ctx *ctx, int32_t inst, avf_observables *obs,
 /* Kernels of one step, in launch order (indices of avf_step_profiled's kernel_ms).  Slots
  * of kernels a step does not launch read as empty (near 0): the binning kernels on
  * candidate-list steps, the predict after the first step of a chunk (it runs inside the
- * previous step's force kernel), the regrid reduction (inside the force kernel) and the
- * finalize before the last step of a chunk. */
+ * previous step's force kernel), the regrid reduction (inside the force kernel), the
+ * finalize before the last step of a chunk, and the next step's cell counts (per-step
+ * binning only, after every step of a chunk but its last). */
 enum {
   AVF_K_PREDICT, AVF_K_SCAN_A, AVF_K_SCAN_B, AVF_K_SCAN_C, AVF_K_SCATTER, AVF_K_DETECT,
-  AVF_K_FORCE, AVF_K_CELLREDUCE, AVF_K_FINALIZE, AVF_NKERNELS
+  AVF_K_FORCE, AVF_K_CELLREDUCE, AVF_K_FINALIZE, AVF_K_COUNT, AVF_NKERNELS
 };
 /* Like avf_step, but launches eagerly with HIP events between consecutive kernels on the
  * context's stream: kernel_ms[k] receives the mean duration of kernel k per step and
