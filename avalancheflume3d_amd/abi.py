@@ -99,7 +99,8 @@ class Options(C.Structure):
 class Diag(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "cell_below", "cell_beyond", "neigh_overflow", "max_neighbours", "max_cell_spheres",
-        "steps", "list_builds", "steps_undone", "closed_instances")]
+        "steps", "list_builds", "steps_undone", "closed_instances", "sweep_global_bricks",
+        "sweep_fp64_pairs")]
 
 
 class Observables(C.Structure):

@@ -1608,6 +1608,8 @@ int avf_get_diag(avf_ctx *c, avf_diag *d) {
   d->neigh_overflow = (int64_t)h.neigh_overflow;
   d->max_neighbours = (int64_t)h.max_neighbours;
   d->max_cell_spheres = (int64_t)h.max_cell;
+  d->sweep_global_bricks = (int64_t)h.sweep_global;
+  d->sweep_fp64_pairs = (int64_t)h.sweep_fp64;
   d->steps = c->steps_total;
   d->list_builds = c->list_builds;
   d->steps_undone = c->steps_undone;

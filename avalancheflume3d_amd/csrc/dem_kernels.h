@@ -68,7 +68,8 @@ constexpr int kScanTile = kScanBlock * kScanItems;
 
 // Device-side diagnostics (avf_diag).
 struct DevDiag {
-  unsigned long long cell_below, cell_beyond, neigh_overflow, max_neighbours, max_cell, pad;
+  unsigned long long cell_below, cell_beyond, neigh_overflow, max_neighbours, max_cell;
+  unsigned long long sweep_global, sweep_fp64;  // bricks swept from global memory, FP64 pairs
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8),

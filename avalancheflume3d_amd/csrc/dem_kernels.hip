@@ -1845,9 +1845,11 @@ __device__ __forceinline__ void sweep_lds(const SweepLds &sh, const double4 *__r
     const double4 q64 = spos[sh.lo[hr] + (uint32_t)t - sh.hoff[hr]];
     const int n = cnt < CAP ? cnt : CAP;
     int w = 0;
+    unsigned long long decided = 0;
     for (int e = 0; e < n; e++) {
       uint32_t v = out[e];
       if (v & kSwUndecided) {
+        decided++;
         const uint32_t jg = v & ~kSwUndecided;
         if (!touch_fp64(q64, spos[jg], skin)) continue;
         v = sid[jg];
@@ -1855,6 +1857,7 @@ __device__ __forceinline__ void sweep_lds(const SweepLds &sh, const double4 *__r
       out[w++] = v;
     }
     cnt -= n - w;
+    atomicAdd(&diag->sweep_fp64, decided);
   }
   if (cnt > CAP) {
     atomicAdd(&diag->neigh_overflow, 1ull);
@@ -2022,6 +2025,7 @@ __global__ __launch_bounds__(kSwBlock) void k_sweep(
       }
     }
     if (!lds) {
+      if (tid == 0) atomicAdd(&diag->sweep_global, 1ull);
       for (int t = tid; t < kSwRows * kSwRowStarts; t += kSwBlock) {
         const int hr = t / kSwRowStarts, k = t % kSwRowStarts;
         const int y = y0 - 1 + hr % kSwHY, z = z0 - 1 + hr / kSwHY;
