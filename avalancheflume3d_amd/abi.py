@@ -93,7 +93,7 @@ class Options(C.Structure):
                 ("sphere_cell_factor", C.c_double), ("block_clumps", C.c_int32),
                 ("max_neighbours", C.c_int32), ("device", C.c_int32), ("lean_state", C.c_int32),
                 ("stream", C.c_void_p), ("verlet_skin", C.c_double),
-                ("closed_forms", C.c_int32)]
+                ("closed_forms", C.c_int32), ("step_binning", C.c_int32)]
 
 
 class Diag(C.Structure):

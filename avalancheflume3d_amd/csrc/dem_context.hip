@@ -61,6 +61,13 @@ struct avf_ctx {
   uint32_t *d_skey = nullptr, *d_srank = nullptr, *d_sid = nullptr, *d_clist = nullptr;
   int32_t *d_ccnt = nullptr;
   double4 *d_spos = nullptr;
+  // per-step binning by clump (avf_options.step_binning 0): one key, rank and sorted
+  // record per clump, the clump grid's cell and brick counts, the records' largest
+  // coordinate; the lists go to the candidate-list arrays below
+  bool clump_bins = false;
+  uint32_t *d_ckey = nullptr, *d_crank = nullptr, *d_csid = nullptr;
+  float4 *d_crec = nullptr;
+  int64_t *d_cgrid = nullptr;  // [0] clump cells over all instances, [1] clump-sweep bricks
   // candidate lists (verlet_skin > 0): per clump up to kVMax neighbour clumps with their
   // candidate sphere pairings (entry-major), displacement tracking, chunk checkpoint
   int32_t *d_vl_m = nullptr, *d_vl_cnt = nullptr;
@@ -90,7 +97,8 @@ struct avf_ctx {
   int64_t io_cap = 0;
 
   bool sgrid_valid = false;
-  bool binned = false;  // sorted sphere arrays describe the current positions
+  bool binned = false;   // sorted sphere arrays describe the current positions
+  bool stepped = false;  // st holds the state after steps (the exports may bin it)
   int64_t steps_since_reorder = 0;
   int64_t steps_total = 0;
   int64_t list_builds = 0, steps_undone = 0;  // avf_diag
@@ -283,7 +291,7 @@ void launch_scatter(avf_ctx *c, const uint32_t *skey, const uint32_t *srank,
 void launch_bin(avf_ctx *c, int mode) {
   k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
       c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-      c->d_srank, c->n, c->cap, mode, ListTrack{nullptr, nullptr});
+      c->d_srank, c->n, c->cap, mode, ListTrack{nullptr, nullptr}, 0);
   launch_scan(c, c->d_starts, 0, mode == 1);
   launch_scatter(c, c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid,
                  c->d_ccnt);
@@ -299,8 +307,9 @@ void launch_force_bs(avf_ctx *c, int initial, bool ft, int time_ahead, const dou
       <<<grid_for(c->n, BS), BS, 0, c->stream>>>(
       c->d_st, c->d_sta, stage_in, c->d_inst, c->d_P, c->d_clist, c->d_ccnt, c->d_vl_m,
       c->d_vl_mask, c->d_vl_cnt, ft ? c->d_f : nullptr, c->d_diag, c->d_acc_cell, stage_out,
-      c->d_orig, VERLET ? list_track(c) : ListTrack{nullptr, nullptr}, c->n, c->cap,
-      initial, time_ahead, si, c->d_counts, c->d_skey, c->d_srank);
+      c->d_orig, (VERLET && !CELLS) ? list_track(c) : ListTrack{nullptr, nullptr}, c->n, c->cap,
+      initial, time_ahead, si, c->d_counts, (VERLET && CELLS) ? c->d_ckey : c->d_skey,
+      c->d_srank);
 }
 
 // Touching spheres of the binned spheres (skin 0) or candidate spheres within sep + skin.
@@ -367,6 +376,21 @@ void launch_force(avf_ctx *c, int initial, bool ft = true, bool verlet = false,
 // spheres from the sweep; with stage_out, the next step's predict into that buffer with its
 // sphere-cell counts and ranks, otherwise the state back to st.
 void launch_cforce(avf_ctx *c, bool ft, const double *stage_in, double *stage_out) {
+  if (c->clump_bins) {  // touching clumps from the clump sweep's lists
+    if (stage_out && c->closed)
+      launch_force_bs<kForceBlock, true, true, true, true, false, true>(c, 0, ft, 0, stage_in,
+                                                                        stage_out, 0);
+    else if (stage_out)
+      launch_force_bs<kForceBlock, true, true, true, false, false, true>(c, 0, ft, 0, stage_in,
+                                                                         stage_out, 0);
+    else if (c->closed)
+      launch_force_bs<kForceBlock, true, false, true, true, false, true>(c, 0, ft, 0, stage_in,
+                                                                         nullptr, 0);
+    else
+      launch_force_bs<kForceBlock, true, false, true, false, false, true>(c, 0, ft, 0, stage_in,
+                                                                          nullptr, 0);
+    return;
+  }
   if (stage_out && c->closed)
     launch_force_bs<kForceBlock, false, true, true, true, false, true>(c, 0, ft, 0, stage_in,
                                                                        stage_out, 0);
@@ -399,22 +423,37 @@ void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bo
   double *buf[2] = {c->d_stage, c->d_stage2};
   const double *cur = buf[i & 1];
   const unsigned G = scan_grid(c);
+  // the keys, ranks and cell count of the binning: one per clump in the clump grid, or
+  // four per clump in the sphere grid
+  const bool cb = c->clump_bins;
+  uint32_t *key = cb ? c->d_ckey : c->d_skey, *rank = cb ? c->d_crank : c->d_srank;
+  uint32_t *sid = cb ? c->d_csid : c->d_sid;
+  int64_t *ncells = cb ? c->d_cgrid : c->d_ncells;
   mark(AVF_K_PREDICT);
   if (i == 0)
     k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
-        c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-        c->d_srank, c->n, c->cap, 1, ListTrack{nullptr, nullptr});
+        c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, key, rank,
+        c->n, c->cap, 1, ListTrack{nullptr, nullptr}, cb ? 1 : 0);
   mark(AVF_K_SCAN_A);
-  k_count_sum<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
+  k_count_sum<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, ncells);
   mark(AVF_K_SCAN_B);
-  k_count_scan<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_starts,
-                                                c->d_ncells, c->d_P, c->d_acc_cell,
-                                                c->ninst, i > 0, 1);
+  k_count_scan<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_starts, ncells,
+                                                c->d_P, c->d_acc_cell, c->ninst, i > 0, 1);
   mark(AVF_K_SCAN_C);
   mark(AVF_K_SCATTER);
-  launch_scatter(c, c->d_skey, c->d_srank, c->d_starts, cur, c->d_spos, c->d_sid, c->d_ccnt);
+  if (cb)
+    k_cscatter<<<grid_for(c->n, 256), 256, 0, c->stream>>>(
+        c->d_ckey, c->d_crank, c->d_starts, cur, c->d_crec, c->d_csid, c->d_inst, c->d_P,
+        c->d_vl_cnt, c->n);
+  else
+    launch_scatter(c, c->d_skey, c->d_srank, c->d_starts, cur, c->d_spos, c->d_sid, c->d_ccnt);
   mark(AVF_K_DETECT);
-  launch_detect(c);
+  if (cb)
+    k_csweep<<<kCbGrid, kCbBlock, 0, c->stream>>>(c->d_P, c->ninst, c->d_cgrid + 1, c->d_starts,
+                                                  c->d_crec, c->d_csid, c->d_vl_m,
+                                                  c->d_vl_mask, c->d_vl_cnt, c->d_diag, c->cap);
+  else
+    launch_detect(c);
   mark(AVF_K_FORCE);
   launch_cforce(c, ft, cur, i + 1 < S ? buf[(i + 1) & 1] : nullptr);
   mark(AVF_K_CELLREDUCE);  // the regrid reduction runs in k_force
@@ -424,8 +463,8 @@ void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bo
                                                              1, 0, 1);
   mark(AVF_K_COUNT);
   if (i + 1 < S)  // the next step's counts and ranks, in this step's sorted order
-    k_count_ranks<<<std::max(1u, grid_for(4 * c->n, 256)), 256, 0, c->stream>>>(
-        c->d_skey, c->d_sid, c->d_starts, c->d_ncells, c->d_counts, c->d_srank);
+    k_count_ranks<<<std::max(1u, grid_for((cb ? 1 : 4) * c->n, 256)), 256, 0, c->stream>>>(
+        key, sid, c->d_starts, ncells, c->d_counts, rank);
   mark(AVF_NKERNELS);
 }
 
@@ -450,7 +489,7 @@ void launch_vstep(avf_ctx *c, bool rebuild, hipEvent_t *ev = nullptr, bool ft = 
   if (rebuild || i == 0)
     k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
         c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-        c->d_srank, c->n, c->cap, rebuild ? 1 : 2, list_track(c));
+        c->d_srank, c->n, c->cap, rebuild ? 1 : 2, list_track(c), 0);
   mark(AVF_K_SCAN_A);
   if (rebuild) k_count_sum<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
   mark(AVF_K_SCAN_B);
@@ -497,7 +536,8 @@ int rebuild_grid(avf_ctx *c, bool reorder) {
   const double horizon = 2.0 * std::max(1, c->opt.reorder_interval);
   k_sgrid_finalize<<<1, 1, 0, c->stream>>>(c->d_P, c->d_acc_sg, c->ninst, c->cell_cap,
                                            c->opt.sphere_cell_factor, horizon, c->d_ncells,
-                                           c->opt.verlet_skin, c->d_nbricks);
+                                           c->opt.verlet_skin, c->d_nbricks,
+                                           c->clump_bins ? c->d_cgrid : nullptr);
   if (reorder) {
     k_reorder_key<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(
         c->d_st, c->d_inst, c->d_P, c->d_counts, c->d_skey, c->d_srank, c->n, c->cap);
@@ -740,6 +780,7 @@ int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
     c->steps_total += S;
     c->since_build += S;
     c->binned = false;
+    c->stepped = true;
     const double rate = maxd / (double)c->since_build, room = 0.49 * skin - maxd;
     int64_t next = rate > 0.0 ? (int64_t)(0.8 * room / rate) : kMaxChunk;
     if (next < 1) {
@@ -826,11 +867,22 @@ int run_cells(avf_ctx *c, int64_t nsteps, StepProf *prof) {
       for (int64_t i = 0; i < S; i++) launch_cstep(c, i, S, nullptr, i + 1 == S);
     }
     HIPCHK(c, hipGetLastError());
-    c->binned = true;
+    c->binned = !c->clump_bins;  // the sphere arrays are the last step's with sphere keys
+    c->stepped = true;
     c->steps_since_reorder += S;
     c->steps_total += S;
     left -= S;
   }
+  return AVF_OK;
+}
+
+// The sphere binning of the state in st for the exports that read it, when the steps since
+// it was last binned did not keep one (per-step binning by clump, candidate-list steps).
+int bin_current(avf_ctx *c) {
+  if (c->binned || !c->stepped || !c->sgrid_valid) return AVF_OK;
+  launch_bin(c, 0);
+  HIPCHK(c, hipGetLastError());
+  c->binned = true;
   return AVF_OK;
 }
 
@@ -872,6 +924,7 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
     avf_options_default(&c->opt);
   if (c->opt.sphere_cell_factor < 1.0) c->opt.sphere_cell_factor = 1.0;
   if (!(c->opt.verlet_skin > 0.0)) c->opt.verlet_skin = 0.0;
+  c->clump_bins = c->opt.verlet_skin == 0.0 && c->opt.step_binning == 0;
   c->opt.block_clumps = c->opt.block_clumps >= 128 ? 128 : 64;
   c->device = c->opt.device;
   c->ninst = ninst;
@@ -954,9 +1007,17 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
     return bail(AVF_ERR_NOMEM);
   }
   if ((rc = scratch_address(c))) return bail(rc);
-  if (c->opt.verlet_skin > 0.0 &&
+  if ((c->opt.verlet_skin > 0.0 || c->clump_bins) &&
       ((rc = dalloc(c, &c->d_vl_m, kVMax * N)) || (rc = dalloc(c, &c->d_vl_mask, kVMax * N)) ||
-       (rc = dalloc(c, &c->d_vl_cnt, N)) || (rc = dalloc(c, &c->d_dsp, N)) || (rc = dalloc(c, &c->d_wreach, N)) ||
+       (rc = dalloc(c, &c->d_vl_cnt, N))))
+    return bail(rc);
+  if (c->clump_bins &&
+      ((rc = dalloc(c, &c->d_ckey, N)) || (rc = dalloc(c, &c->d_crank, N)) ||
+       (rc = dalloc(c, &c->d_csid, N)) || (rc = dalloc(c, &c->d_crec, 4 * N)) ||
+       (rc = dalloc(c, &c->d_cgrid, 2))))
+    return bail(rc);
+  if (c->opt.verlet_skin > 0.0 &&
+      ((rc = dalloc(c, &c->d_dsp, N)) || (rc = dalloc(c, &c->d_wreach, N)) ||
        (rc = dalloc(c, &c->d_ck_st, N * S_NFIELDS)) || (rc = dalloc(c, &c->d_ck_rad, N)) ||
        (rc = dalloc(c, &c->d_ck_P, ninst)) || (rc = dalloc(c, &c->d_ck_diag, 1)) ||
        (rc = dalloc(c, &c->d_umax, 1))))
@@ -998,7 +1059,8 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
       (he = hipMemset(c->d_f, 0, N * F_NFIELDS * sizeof(double))) != hipSuccess ||
       (he = hipMemset(c->d_diag, 0, sizeof(DevDiag))) != hipSuccess ||
       (he = hipMemset(c->d_ncells, 0, sizeof(int64_t))) != hipSuccess ||
-      (he = hipMemset(c->d_nbricks, 0, sizeof(int64_t))) != hipSuccess) {
+      (he = hipMemset(c->d_nbricks, 0, sizeof(int64_t))) != hipSuccess ||
+      (c->d_cgrid && (he = hipMemset(c->d_cgrid, 0, 2 * sizeof(int64_t))) != hipSuccess)) {
     fail(c, AVF_ERR_HIP, "hipMemset: %s", hipGetErrorString(he));
     return bail(AVF_ERR_HIP);
   }
@@ -1065,7 +1127,7 @@ int avf_destroy(avf_ctx *c) {
                   c->d_acc_sg, c->d_diag,  c->d_count, c->d_clist, c->d_ccnt,
                   c->d_vl_m,   c->d_vl_cnt, c->d_vl_mask, c->d_dsp, c->d_wreach,
                   c->d_ck_st,  c->d_ck_rad, c->d_ck_P, c->d_ck_diag, c->d_umax,
-                  c->d_stage2};
+                  c->d_stage2, c->d_ckey, c->d_crank, c->d_csid, c->d_crec, c->d_cgrid};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
@@ -1130,6 +1192,7 @@ int avf_upload(avf_ctx *c, int32_t inst, const avf_particle *pp, const avf_doubl
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->sgrid_valid = false;
   c->binned = false;
+  c->stepped = false;
   c->list_valid = false;
   return AVF_OK;
 }
@@ -1388,10 +1451,10 @@ int avf_get_sphere_keys(avf_ctx *c, int32_t inst, uint32_t *keys, int64_t *order
                         int64_t *nbinned, avf_sphere_grid *grid) {
   int rc = check_inst(c, inst);
   if (rc) return rc;
+  if ((rc = bin_current(c))) return rc;
   if (!c->binned)
     return fail(c, AVF_ERR_ARG,
-                "no sphere binning of the current state (call avf_initial_forces, or step with "
-                "verlet_skin = 0)");
+                "no sphere binning of the current state (call avf_initial_forces or step)");
   InstParams p;
   HIPCHK(c, hipMemcpy(&p, c->d_P + inst, sizeof p, hipMemcpyDeviceToHost));
   if (grid) {
@@ -1434,6 +1497,7 @@ int avf_get_sphere_keys(avf_ctx *c, int32_t inst, uint32_t *keys, int64_t *order
 int avf_get_contacts(avf_ctx *c, int32_t inst, avf_contact *out, int64_t capn, int64_t *count) {
   int rc = check_inst(c, inst);
   if (rc) return rc;
+  if ((rc = bin_current(c))) return rc;
   if (!c->binned)
     return fail(c, AVF_ERR_ARG, "no force stage since the last upload (call avf_initial_forces)");
   int64_t room = std::max<int64_t>(capn, 0);
@@ -1735,7 +1799,7 @@ void win_bin(avf_ctx *c, const double *st) {
   WinBuf *w = c->win;
   k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
       const_cast<double *>(st), c->d_sta, w->bstage, c->d_inst, c->d_orig, c->d_P, c->d_counts,
-      w->bkey, w->brank, c->n, c->cap, 0, ListTrack{nullptr, nullptr});
+      w->bkey, w->brank, c->n, c->cap, 0, ListTrack{nullptr, nullptr}, 0);
   launch_scan(c, w->bstarts);
   launch_scatter(c, w->bkey, w->brank, w->bstarts, w->bstage, w->bspos, w->bsid, nullptr);
 }

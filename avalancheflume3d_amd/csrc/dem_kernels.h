@@ -30,6 +30,13 @@ struct InstParams {
   int32_t sg_n[3], pad0;
   int64_t sg_base;
   int64_t sg_bbase;  // first brick of this instance in the contact sweep's brick range
+  // clump-cell grid of this instance (per-step binning by clump: one key per clump centre,
+  // cell side >= the largest clump's bounding diameter); cg_mh: the float test's margin
+  // (length), cg_bfac: bounding radius / sphere radius, rounded up
+  double cg_lo[3], cg_inv_h, cg_h, cg_mh, cg_bfac;
+  int32_t cg_n[3], pad1;
+  int64_t cg_base;
+  int64_t cg_bbase;  // first brick of this instance in the clump sweep's brick range
   // storage range and bookkeeping
   int64_t begin, count;
   int64_t cc;

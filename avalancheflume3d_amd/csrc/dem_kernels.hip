@@ -32,6 +32,16 @@ __device__ __forceinline__ uint32_t sphere_key(const InstParams &p, d3 q) {
   return (uint32_t)(p.sg_base + ix + (int64_t)p.sg_n[0] * (iy + (int64_t)p.sg_n[1] * iz));
 }
 
+// Clump-cell key of a clump centre for per-step binning by clump (kInvalidKey for a clump
+// past side.x at t_new, which is not binned: cells.cu:65).
+__device__ __forceinline__ uint32_t clump_key(const InstParams &p, d3 rr, double t_new) {
+  if (frozen(t_new, rr.x, p.side_x)) return kInvalidKey;  // not binned (cells.cu:65)
+  const int ix = grid_index(rr.x, p.cg_lo[0], p.cg_inv_h, p.cg_n[0]);
+  const int iy = grid_index(rr.y, p.cg_lo[1], p.cg_inv_h, p.cg_n[1]);
+  const int iz = grid_index(rr.z, p.cg_lo[2], p.cg_inv_h, p.cg_n[2]);
+  return (uint32_t)(p.cg_base + ix + (int64_t)p.cg_n[0] * (iy + (int64_t)p.cg_n[1] * iz));
+}
+
 // Sphere centres rr + (rad/div) R (+-1,+-1,+-1), init_Verlet.cu:130-142.
 __device__ __forceinline__ void sphere_centres(d3 rr, const double *R, double pos, d3 *s) {
   s[0] = make_d3(rr.x + pos * (R[0] + R[1] + R[2]), rr.y + pos * (R[3] + R[4] + R[5]),
@@ -222,7 +232,7 @@ __device__ __forceinline__ void predict_clump(double *__restrict__ st, double *_
                                               double *__restrict__ g, const InstParams &p,
                                               const int32_t *__restrict__ orig, uint32_t *keys,
                                               int64_t c, int64_t cap, int mode, ListTrack lt,
-                                              double t_old) {
+                                              double t_old, bool ckeys = false) {
   const double t_new = mode ? t_old + p.dt : t_old;
 
   d3 rr = make_d3(st[S_RX * cap + c], st[S_RY * cap + c], st[S_RZ * cap + c]);
@@ -265,7 +275,10 @@ __device__ __forceinline__ void predict_clump(double *__restrict__ st, double *_
     return;
   }
 
-  clump_keys(p, rr, R, rad0, t_new, keys);
+  if (ckeys)
+    keys[0] = clump_key(p, rr, t_new);
+  else
+    clump_keys(p, rr, R, rad0, t_new, keys);
 }
 
 // A clump's state in registers: its stage record as read at the start of a list step
@@ -309,7 +322,7 @@ __global__ __launch_bounds__(kPredictBlock) void k_predict(
     double *__restrict__ st, double *__restrict__ sta, double *__restrict__ stage,
     const int32_t *__restrict__ inst_of, const int32_t *__restrict__ orig,
     const InstParams *__restrict__ P, uint32_t *__restrict__ counts, uint32_t *__restrict__ skey,
-    uint32_t *__restrict__ srank, int64_t n, int64_t cap, int mode, ListTrack lt) {
+    uint32_t *__restrict__ srank, int64_t n, int64_t cap, int mode, ListTrack lt, int ckeys) {
   __shared__ double rec[kPredictBlock * kStagePad];
   const int64_t c0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kPredictBlock;
   const int64_t c = c0 + threadIdx.x;
@@ -320,12 +333,19 @@ __global__ __launch_bounds__(kPredictBlock) void k_predict(
       const int cur = __builtin_amdgcn_readfirstlane(inst);
       if (inst == cur) {
         predict_clump(st, sta, rec + threadIdx.x * kStagePad, P[cur], orig, keys, c, cap, mode,
-                      lt, P[cur].time);
+                      lt, P[cur].time, ckeys != 0);
         break;
       }
     }
   }
-  if (mode != 2 && c < n) count_keys(keys, counts, skey, srank, c);
+  if (mode != 2 && c < n) {
+    if (ckeys) {  // one clump key (clump-cell binning): skey / srank hold one word per clump
+      skey[c] = keys[0];
+      srank[c] = keys[0] != kInvalidKey ? atomicAdd(&counts[keys[0]], 1u) : 0u;
+    } else {
+      count_keys(keys, counts, skey, srank, c);
+    }
+  }
   __syncthreads();
   const int64_t rows = n - c0 < kPredictBlock ? n - c0 : kPredictBlock;
   double *dst = stage + c0 * kStageDoubles;
@@ -986,7 +1006,7 @@ __device__ __forceinline__ void force_clump(
     double *__restrict__ fout, DevDiag *__restrict__ diag, int64_t cap, int initial,
     double time, const ClumpRegs &own, ClumpRegs &out) {
   static_assert(!NEXT || VERLET || CELLS, "the fused predict needs the record-resident state");
-  static_assert(!CELLS || (!VERLET && LEAN), "per-step binning keeps the state in the records");
+  static_assert(!CELLS || LEAN, "per-step binning keeps the state in the records");
   constexpr bool kRecState = VERLET || CELLS;  // the state lives in the stage records
   constexpr bool kOwnStage = kRecState && (LEAN || !NEXT);
   constexpr bool kRegsOut = NEXT && LEAN;
@@ -1015,7 +1035,8 @@ __device__ __forceinline__ void force_clump(
                    (initial ? (AVF_MOD_PP | AVF_MOD_HOP) : ~0);
   // on candidate-list steps, the wall modules this clump cannot reach before the lists are
   // rebuilt are skipped (each would return at its own early-out test)
-  const uint32_t reach = VERLET ? (uint32_t)wreach[c] : (uint32_t)kReachAll;
+  // (per-step binning by clump reads its lists the same way but has no reach bits)
+  const uint32_t reach = (VERLET && !CELLS) ? (uint32_t)wreach[c] : (uint32_t)kReachAll;
   const double massw = kSphFact * rad * rad * rad * rho;  // force_hop.cu:45 order
   d3 ff = make_d3(0.0, 0.0, 0.0), tq = make_d3(0.0, 0.0, 0.0);
 
@@ -1502,14 +1523,22 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
           else
             predict_clump(st, sta, rec + lane * kStagePad, q, orig, nullptr, c, cap, 2, lt,
                           t_old);
-          if constexpr (CELLS) clump_keys(q, cr.rr, cr.R, rad0, t_old + q.dt, keys);
+          if constexpr (CELLS && VERLET)
+            keys[0] = clump_key(q, cr.rr, t_old + q.dt);
+          else if constexpr (CELLS)
+            clump_keys(q, cr.rr, cr.R, rad0, t_old + q.dt, keys);
           break;
         }
       }
     }
-    if constexpr (CELLS)  // sphere-cell keys of the next step (counted by k_count_ranks)
+    // the next step's keys (counted by k_count_ranks): the clump-cell key, or the four
+    // sphere-cell keys
+    if constexpr (CELLS && VERLET) {
+      if (c < n) skey[c] = keys[0];
+    } else if constexpr (CELLS) {
       if (c < n)
         reinterpret_cast<uint4 *>(skey)[c] = make_uint4(keys[0], keys[1], keys[2], keys[3]);
+    }
     __syncthreads();
     const int64_t rows = n - c0 < BS ? n - c0 : BS;
     double *dst = stage_out + c0 * kStageDoubles;
@@ -1648,8 +1677,9 @@ __device__ __forceinline__ void step_params(InstParams &p, long long *acc, int f
   if (advance) p.time = p.time + p.dt;
 }
 
-// Exclusive scan of the counts into starts, the counts back to zero; starts[ncells] receives the total.  Workgroup 0's first lane also applies step_params to every instance
-// (no other kernel runs meanwhile; this kernel reads no parameter).
+// Exclusive scan of the counts into starts, the counts back to zero; starts[ncells] receives
+// the total.  Workgroup 0's first lane also applies step_params to every instance (no other
+// kernel runs meanwhile; this kernel reads no parameter).
 __global__ __launch_bounds__(kScanBlock) void k_count_scan(
     uint32_t *__restrict__ counts, const uint32_t *__restrict__ bsum,
     uint32_t *__restrict__ starts, const int64_t *__restrict__ ncells_p,
@@ -2069,6 +2099,321 @@ __global__ __launch_bounds__(kSwBlock) void k_sweep(
   }
 }
 
+// ----------------------------------------------------------------- clump sweep -------
+//
+// Per-step binning by clump: one key per clump (the clump-cell grid cell of its centre, cell
+// side >= the largest clump's bounding diameter, so the clumps of any touching pair sit in
+// neighbouring cells), counted, scanned and scattered like the sphere keys (a quarter of the
+// keys, a fifth of the cells).  k_cscatter writes each binned clump's sorted record: its
+// centre, radius and four sphere centres as floats relative to the grid origin (64 B) and
+// its storage index.  k_csweep then lists, for every binned clump, the clumps of its 27
+// neighbour cells that may touch it, with the pairings that may: a float test with a margin
+// far above its rounding error (centre distance against the bounding radii, then the 16
+// sphere pairs), so the listed pairings are a superset of the touching ones.  The force
+// kernel re-tests every listed pairing with force_part.cu:140-157's FP64 test on the stage
+// records' centres (as on candidate-list steps), so the contacts, their order and every bit
+// of the forces are those of the sphere-level binning.
+
+constexpr int kCbX = 8, kCbY = 8, kCbZ = 4;  // brick of clump cells
+constexpr int kCbHX = kCbX + 2, kCbHY = kCbY + 2, kCbHZ = kCbZ + 2;
+constexpr int kCbRows = kCbHY * kCbHZ;        // halo rows of a brick
+constexpr int kCbOwnRows = kCbY * kCbZ;       // rows of the brick itself
+constexpr int kCbRowStarts = kCbHX + 1;       // first clump of each halo cell, and the end
+constexpr int kCbBlock = 256;
+constexpr int kCbCap = 512;                   // halo clumps staged in LDS
+constexpr int kCbGrid = 4096;                 // workgroups (bricks in strides of kCbGrid)
+static_assert(kCbRows <= 64 && kCbOwnRows <= 64, "one wavefront scans the rows");
+
+__device__ __forceinline__ int64_t cbricks_of(const InstParams &p) {
+  return (int64_t)((p.cg_n[0] + kCbX - 1) / kCbX) * ((p.cg_n[1] + kCbY - 1) / kCbY) *
+         ((p.cg_n[2] + kCbZ - 1) / kCbZ);
+}
+
+// One lane per binned clump (storage order): its sorted record at its cell's start + its
+// rank (from the count).  Record: (centre, radius), then the four sphere centres packed in
+// three float4; coordinates relative to the instance's grid origin, rounded from the FP64
+// centres of the predict (sphere_centres on the stage record).  A clump that is not binned
+// (past side.x) gets an empty list.
+__global__ __launch_bounds__(256) void k_cscatter(
+    const uint32_t *__restrict__ ckey, const uint32_t *__restrict__ crank,
+    const uint32_t *__restrict__ starts, const double *__restrict__ stage,
+    float4 *__restrict__ crec, uint32_t *__restrict__ csid, const int32_t *__restrict__ inst_of,
+    const InstParams *__restrict__ P, int32_t *__restrict__ vl_cnt, int64_t n) {
+  const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  if (c < n) {
+    const uint32_t key = ckey[c];
+    if (key == kInvalidKey) {
+      vl_cnt[c] = 0;
+    } else {
+      const uint32_t dst = starts[key] + crank[c];
+      const InstParams &p = P[inst_of[c]];
+      const double2 *g2 = reinterpret_cast<const double2 *>(stage + c * kStageDoubles);
+      const double2 a = g2[0], b = g2[1];  // rr.x, rr.y | rr.z, rad
+      const double2 r0 = g2[6], r1 = g2[7], r2 = g2[8], r3 = g2[9], r4 = g2[10];
+      const double R[9] = {r0.x, r0.y, r1.x, r1.y, r2.x, r2.y, r3.x, r3.y, r4.x};
+      const d3 rr = make_d3(a.x, a.y, b.x);
+      const double rad = b.y;  // binned: not frozen, the pre-freeze radius
+      d3 s[4];
+      sphere_centres(rr, R, rad / p.sphere_div, s);
+      const double ox = p.cg_lo[0], oy = p.cg_lo[1], oz = p.cg_lo[2];
+      float f[15];
+      f[0] = (float)(rr.x - ox);
+      f[1] = (float)(rr.y - oy);
+      f[2] = (float)(rr.z - oz);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        f[3 + 3 * k] = (float)(s[k].x - ox);
+        f[4 + 3 * k] = (float)(s[k].y - oy);
+        f[5 + 3 * k] = (float)(s[k].z - oz);
+      }
+      float4 *o = crec + 4 * (int64_t)dst;
+      o[0] = make_float4(f[0], f[1], f[2], (float)rad);
+      o[1] = make_float4(f[3], f[4], f[5], f[6]);
+      o[2] = make_float4(f[7], f[8], f[9], f[10]);
+      o[3] = make_float4(f[11], f[12], f[13], f[14]);
+      csid[dst] = (uint32_t)c;
+    }
+  }
+}
+
+struct CSweepLds {
+  uint32_t gs[kCbRows][kCbRowStarts];  // global sorted index of each halo cell's first clump
+  uint32_t hoff[kCbRows + 1];          // LDS index of each halo row's first clump
+  uint32_t ownp[kCbOwnRows + 1];       // prefix of the brick rows' clump counts
+  uint32_t xmax;                       // largest coordinate magnitude of the halo (float bits)
+  float4 ca[kCbCap];                   // centre, radius
+  float4 cs[kCbCap][3];                // the four sphere centres
+  uint32_t id[kCbCap];                 // storage index
+};
+
+// Largest coordinate magnitude of a record (the float test's rounding error scales with it).
+__device__ __forceinline__ float rec_extent(float4 a, float4 b, float4 c, float4 d) {
+  const float m0 = fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(b.x)));
+  const float m1 = fmaxf(fmaxf(fabsf(b.y), fabsf(b.z)), fmaxf(fabsf(b.w), fabsf(c.x)));
+  const float m2 = fmaxf(fmaxf(fabsf(c.y), fabsf(c.z)), fmaxf(fabsf(c.w), fabsf(d.x)));
+  const float m3 = fmaxf(fmaxf(fabsf(d.y), fabsf(d.z)), fabsf(d.w));
+  return fmaxf(fmaxf(m0, m1), fmaxf(m2, m3));
+}
+
+// Candidate pairings of two clumps whose bounding spheres overlap: bit (sm * 4 + sl) for
+// sphere sm of o and sl of q when their centres are closer than lim (r_q + r_o + margin).
+__device__ __forceinline__ uint32_t cand_mask(const float *qs, const float4 *os, float lim) {
+  const float o[12] = {os[0].x, os[0].y, os[0].z, os[0].w, os[1].x, os[1].y,
+                       os[1].z, os[1].w, os[2].x, os[2].y, os[2].z, os[2].w};
+  const float lim2 = lim * lim;
+  uint32_t mask = 0u;
+#pragma unroll
+  for (int sm = 0; sm < 4; sm++)
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++) {
+      const float ex = o[3 * sm] - qs[3 * sl], ey = o[3 * sm + 1] - qs[3 * sl + 1],
+                  ez = o[3 * sm + 2] - qs[3 * sl + 2];
+      if (__fmaf_rn(ex, ex, __fmaf_rn(ey, ey, ez * ez)) < lim2) mask |= 1u << (sm * 4 + sl);
+    }
+  return mask;
+}
+
+// The candidate list of one own clump (storage index me, record qa / qs, largest coordinate
+// magnitude xq) from the clumps of the 9 neighbour rows' cells lx - 1 .. lx + 1 of halo row
+// hr; records from LDS (GLOBAL false: indices into the staged halo, mh covers every record
+// of the halo) or from the sorted arrays (GLOBAL true: mh is widened per pair for the two
+// records' magnitudes).  self: the clump's own index in the same numbering.  Entries go to
+// vl_m / vl_mask (entry-major, as the candidate lists of k_vmerge), at most kVMax (the rest
+// are counted in diag.neigh_overflow).
+template <bool GLOBAL>
+__device__ __forceinline__ void csweep_clump(const CSweepLds &sh, const float4 *__restrict__ crec,
+                                             const uint32_t *__restrict__ csid, float4 qa,
+                                             const float *qs, float xq, uint32_t self,
+                                             uint32_t me, int hr, int lx, float bfac, float mh0,
+                                             int32_t *__restrict__ vl_m,
+                                             uint16_t *__restrict__ vl_mask,
+                                             int32_t *__restrict__ vl_cnt, int64_t cap,
+                                             DevDiag *__restrict__ diag) {
+  int cnt = 0;
+#pragma unroll 1
+  for (int r9 = 0; r9 < 9; r9++) {
+    const int nr = hr + (r9 % 3 - 1) + kCbHY * (r9 / 3 - 1);
+    uint32_t j = sh.gs[nr][lx - 1], e = sh.gs[nr][lx + 2];
+    if (!GLOBAL) {
+      const uint32_t base = sh.hoff[nr] - sh.gs[nr][0];
+      j += base;
+      e += base;
+    }
+    for (; j < e; j++) {
+      if (j == self) continue;
+      float4 oa, os[3];
+      float mh = mh0;
+      if (GLOBAL) {
+        const float4 *r = crec + 4 * (int64_t)j;
+        oa = r[0];
+        os[0] = r[1];
+        os[1] = r[2];
+        os[2] = r[3];
+        mh = fmaxf(mh0, (xq + rec_extent(oa, os[0], os[1], os[2])) * 0x1p-21f);
+      } else {
+        oa = sh.ca[j];
+      }
+      // the bounding spheres (radius r * bfac) first, then the 16 sphere pairs
+      const float dx = oa.x - qa.x, dy = oa.y - qa.y, dz = oa.z - qa.z;
+      const float rs = qa.w + oa.w, bb = __fmaf_rn(rs, bfac, mh);
+      if (!(__fmaf_rn(dx, dx, __fmaf_rn(dy, dy, dz * dz)) < bb * bb)) continue;
+      if (!GLOBAL) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) os[k] = sh.cs[j][k];
+      }
+      const uint32_t mask = cand_mask(qs, os, rs + mh);
+      if (!mask) continue;
+      if (cnt < kVMax) {
+        vl_m[cnt * cap + me] = (int32_t)(GLOBAL ? csid[j] : sh.id[j]);
+        vl_mask[cnt * cap + me] = (uint16_t)mask;
+      }
+      cnt++;
+    }
+  }
+  if (cnt > kVMax) {
+    atomicAdd(&diag->neigh_overflow, 1ull);
+    cnt = kVMax;
+  }
+  vl_cnt[me] = cnt;
+}
+
+// The candidate lists of every binned clump, by bricks of kCbX x kCbY x kCbZ clump cells: a
+// workgroup loads the first clump of every cell of the brick's halo rows (each row a
+// contiguous range of the sorted clumps), stages the halo's records in LDS (from global
+// memory in place when more than kCbCap), then each lane takes one of the brick's own clumps
+// and walks the 27 cells around it.  Brick b of all instances' bricks goes to logical
+// workgroup b mod G as in k_sweep.
+__global__ __launch_bounds__(kCbBlock) void k_csweep(
+    const InstParams *__restrict__ P, int32_t ninst, const int64_t *__restrict__ nbricks_p,
+    const uint32_t *__restrict__ starts, const float4 *__restrict__ crec,
+    const uint32_t *__restrict__ csid, int32_t *__restrict__ vl_m, uint16_t *__restrict__ vl_mask, int32_t *__restrict__ vl_cnt,
+    DevDiag *__restrict__ diag, int64_t cap) {
+  __shared__ CSweepLds sh;
+  const int tid = threadIdx.x;
+  const int64_t nb = *nbricks_p;
+  const int64_t G = gridDim.x, L = xcd_block_runs(blockIdx.x, gridDim.x, 8);
+#pragma unroll 1
+  for (int64_t b = L; b < nb; b += G) {
+    __syncthreads();  // the previous brick's LDS is no longer read
+    if (tid == 0) sh.xmax = 0u;
+    int inst = 0;
+    if (ninst > 1) {  // last instance whose first brick is <= b
+      int lo = 0, hi = ninst - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (P[mid].cg_bbase <= b)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      inst = lo;
+    }
+    const InstParams &p = P[inst];
+    const int nx = p.cg_n[0], ny = p.cg_n[1], nz = p.cg_n[2];
+    const int64_t lb = b - p.cg_bbase;
+    const int nbx = (nx + kCbX - 1) / kCbX, nby = (ny + kCbY - 1) / kCbY;
+    const int x0 = (int)(lb % nbx) * kCbX, y0 = (int)((lb / nbx) % nby) * kCbY;
+    const int z0 = (int)(lb / ((int64_t)nbx * nby)) * kCbZ;
+    // first clump of cells x0 - 1 .. x0 + kCbX of every halo row, clamped into [0, nx]
+    // (cells outside the grid have no width; rows outside it are empty)
+    for (int t = tid; t < kCbRows * kCbRowStarts; t += kCbBlock) {
+      const int hr = t / kCbRowStarts, k = t % kCbRowStarts;
+      const int y = y0 - 1 + hr % kCbHY, z = z0 - 1 + hr / kCbHY;
+      uint32_t v = 0u;
+      if (y >= 0 && y < ny && z >= 0 && z < nz) {
+        int x = x0 - 1 + k;
+        x = x < 0 ? 0 : (x > nx ? nx : x);
+        v = starts[p.cg_base + (int64_t)nx * (y + (int64_t)ny * z) + x];
+      }
+      sh.gs[hr][k] = v;
+    }
+    __syncthreads();
+    if (tid < 64) {  // LDS offsets of the halo rows
+      const uint32_t len = tid < kCbRows ? sh.gs[tid][kCbRowStarts - 1] - sh.gs[tid][0] : 0u;
+      const uint32_t inc = wave_incl_scan(len);
+      if (tid < kCbRows) sh.hoff[tid + 1] = inc;
+      if (tid == 0) sh.hoff[0] = 0u;
+    } else if (tid < 128) {  // clumps of the brick rows (cells 1 .. kCbX of the halo frame)
+      const int o = tid - 64;
+      uint32_t len = 0u;
+      if (o < kCbOwnRows) {
+        const int hr = (1 + o % kCbY) + kCbHY * (1 + o / kCbY);
+        len = sh.gs[hr][kCbX + 1] - sh.gs[hr][1];
+      }
+      const uint32_t inc = wave_incl_scan(len);
+      if (o < kCbOwnRows) sh.ownp[o + 1] = inc;
+      if (o == 0) sh.ownp[0] = 0u;
+    }
+    __syncthreads();
+    const uint32_t H = sh.hoff[kCbRows], K = sh.ownp[kCbOwnRows];
+    if (K == 0u) continue;
+    const bool lds = H <= (uint32_t)kCbCap;
+    if (lds) {  // the halo's records, one clump per thread (five loads in flight)
+      float ext = 0.0f;
+      for (uint32_t t = tid; t < H; t += kCbBlock) {
+        const int hr = upper_row(sh.hoff, kCbRows, t);
+        const int64_t j = (int64_t)sh.gs[hr][0] + (t - sh.hoff[hr]);
+        const float4 *r = crec + 4 * j;
+        const float4 v0 = r[0], v1 = r[1], v2 = r[2], v3 = r[3];
+        const uint32_t id = csid[j];
+        sh.ca[t] = v0;
+        sh.cs[t][0] = v1;
+        sh.cs[t][1] = v2;
+        sh.cs[t][2] = v3;
+        sh.id[t] = id;
+        ext = fmaxf(ext, rec_extent(v0, v1, v2, v3));
+      }
+      // the halo's largest coordinate magnitude (NaN coordinates leave it alone; they touch
+      // nothing either)
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) ext = fmaxf(ext, __shfl_xor(ext, off, 64));
+      if ((tid & 63) == 0) atomicMax(&sh.xmax, __float_as_uint(ext));
+      __syncthreads();
+    } else if (tid == 0) {
+      atomicAdd(&diag->sweep_global, 1ull);
+    }
+    const float h = (float)p.cg_h;
+    // the float test's margin: above its rounding error for every record of the halo (for
+    // records swept from global memory, per pair)
+    const float mh = lds ? fmaxf((float)p.cg_mh, (__uint_as_float(sh.xmax) + h) * 0x1p-20f)
+                         : (float)p.cg_mh;
+    const float bfac = (float)p.cg_bfac;
+    for (uint32_t k = tid; k < K; k += kCbBlock) {
+      const int o = upper_row(sh.ownp, kCbOwnRows, k);
+      const int hr = (1 + o % kCbY) + kCbHY * (1 + o / kCbY);
+      const uint32_t jg = sh.gs[hr][1] + (k - sh.ownp[o]);
+      int lx = 1;  // the clump's cell along x in the halo frame (1 .. kCbX)
+#pragma unroll
+      for (int x = 2; x <= kCbX; x++)
+        if (sh.gs[hr][x] <= jg) lx = x;
+      const uint32_t t = sh.hoff[hr] + (jg - sh.gs[hr][0]);
+      float4 qa, q1, q2, q3;
+      uint32_t me;
+      if (lds) {
+        qa = sh.ca[t];
+        q1 = sh.cs[t][0];
+        q2 = sh.cs[t][1];
+        q3 = sh.cs[t][2];
+        me = sh.id[t];
+      } else {
+        const float4 *r = crec + 4 * (int64_t)jg;
+        qa = r[0];
+        q1 = r[1];
+        q2 = r[2];
+        q3 = r[3];
+        me = csid[jg];
+      }
+      const float qs[12] = {q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+      if (lds)
+        csweep_clump<false>(sh, crec, csid, qa, qs, 0.0f, t, me, hr, lx, bfac, mh, vl_m,
+                            vl_mask, vl_cnt, cap, diag);
+      else
+        csweep_clump<true>(sh, crec, csid, qa, qs, rec_extent(qa, q1, q2, q3), jg, me, hr, lx,
+                           bfac, mh, vl_m, vl_mask, vl_cnt, cap, diag);
+    }
+  }
+}
+
 // --------------------------------------------------------------- sphere-cell grid ---
 
 // AABB of clump centres and max |v| per instance (acc: 8 per instance as in
@@ -2105,7 +2450,7 @@ __global__ __launch_bounds__(256) void k_sgrid_reduce(const double *__restrict__
 __global__ void k_sgrid_finalize(InstParams *__restrict__ P, long long *__restrict__ acc,
                                  int32_t ninst, int64_t cell_cap, double factor, double horizon,
                                  int64_t *__restrict__ ncells_out, double skin,
-                                 int64_t *__restrict__ nbricks_out) {
+                                 int64_t *__restrict__ nbricks_out, int64_t *__restrict__ cg_out) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   double scale = 1.0;
   bool fit = false;
@@ -2158,6 +2503,56 @@ __global__ void k_sgrid_finalize(InstParams *__restrict__ P, long long *__restri
     bricks += bricks_of(P[i]);
   }
   *nbricks_out = bricks;
+  if (cg_out) {
+    // the clump-cell grid over the same box: cell side >= twice the largest bounding radius
+    // (sg_reach), so the clumps of a touching pair sit in neighbouring cells
+    scale = 1.0;
+    fit = false;
+    for (int attempt = 0; attempt < 64 && !fit; attempt++) {
+      int64_t total = 0;
+      for (int i = 0; i < ninst; i++) {
+        InstParams &p = P[i];
+        const double h = fmax(2.0 * p.sg_reach, p.sg_h) * scale * (1.0 + 1e-9);
+        double ext = h;
+        for (int k = 0; k < 3; k++) {
+          const double len = (double)p.sg_n[k] * p.sg_h;
+          int64_t m = (int64_t)ceil(len / h);
+          if (m < 1) m = 1;
+          if (m > (1 << 20)) m = 1 << 20;
+          p.cg_n[k] = (int32_t)m;
+          p.cg_lo[k] = p.sg_lo[k];
+          ext = fmax(ext, (double)m * h);
+        }
+        p.cg_h = h;
+        p.cg_inv_h = 1.0 / h;
+        // the float test's margin: far above its rounding error for coordinates within the
+        // grid (the sweep widens it for coordinates beyond, from the step's largest one)
+        p.cg_mh = fmax(h * 0x1p-12, (ext + h) * 0x1p-20);
+        p.cg_bfac = (1.0 + sqrt(3.0) / p.sphere_div) * (1.0 + 0x1p-16);
+        p.cg_base = total;
+        total += (int64_t)p.cg_n[0] * p.cg_n[1] * p.cg_n[2];
+      }
+      if (total <= cell_cap) {
+        cg_out[0] = total;
+        fit = true;
+      } else {
+        scale *= cbrt((double)total / (double)cell_cap) * 1.05;
+      }
+    }
+    if (!fit) {
+      for (int i = 0; i < ninst; i++) {
+        P[i].cg_n[0] = P[i].cg_n[1] = P[i].cg_n[2] = 1;
+        P[i].cg_base = i;
+      }
+      cg_out[0] = ninst;
+    }
+    int64_t cb = 0;
+    for (int i = 0; i < ninst; i++) {
+      P[i].cg_bbase = cb;
+      cb += cbricks_of(P[i]);
+    }
+    cg_out[1] = cb;
+  }
   for (int i = 0; i < ninst; i++) {
     long long *a = acc + 8 * i;
     for (int k = 0; k < 3; k++) {

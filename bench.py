@@ -57,7 +57,17 @@ CELLS_FORCE_BYTES = 500.0
 # (x, y, z, r: 32 B) and id (4 B) read once, two cell starts per sphere (the grid holds about
 # two cells per sphere: 8 B), its contact count (4 B) written; contact entries not counted.
 SWEEP_BYTES_PER_CLUMP = 4 * (32.0 + 4.0 + 8.0 + 4.0)
-KERNEL_NAMES = {"force": "k_force", "detect": "k_sweep"}
+# Per-step binning by clump (the default): k_force reads the stage record (192 B), the
+# angular acceleration (24 B), mass and inertia (32 B), the instance index and list length
+# (8 B), and writes the next stage record (192 B), the angular acceleration (24 B) and the
+# next clump-cell key (4 B): 476 B; list entries and the neighbours' records not counted.
+# k_csweep reads each binned clump's sorted record (centre, radius, four sphere centres as
+# floats: 64 B) and storage index (4 B) once and two cell starts (8 B), and writes its list
+# length (4 B): 80 B; list entries not counted.
+CLUMP_FORCE_BYTES = 476.0
+CSWEEP_BYTES_PER_CLUMP = 80.0
+KERNEL_NAMES = {"sphere": {"force": "k_force", "detect": "k_sweep"},
+                "clump": {"force": "k_force", "detect": "k_csweep"}}
 
 
 def cpu_batch_baseline(budget_s: float, cores: int) -> dict:
@@ -279,12 +289,13 @@ def cpu_window(inst, n_sample: int, steps: int) -> dict:
                       f"({dt:.1f} s)"}
 
 
-def kernel_bytes(name: str, n: int, diag: dict) -> float:
+def kernel_bytes(name: str, n: int, diag: dict, binning: str = "clump") -> float:
     """Algorithmic bytes per launch of the kernels a roofline is quoted for."""
+    mi = MASS_INERTIA_BYTES if diag["closed_instances"] else 0.0
     if name == "force":
-        return (CELLS_FORCE_BYTES - (MASS_INERTIA_BYTES if diag["closed_instances"] else 0.0)) * n
+        return ((CLUMP_FORCE_BYTES if binning == "clump" else CELLS_FORCE_BYTES) - mi) * n
     if name == "detect":
-        return SWEEP_BYTES_PER_CLUMP * n
+        return (CSWEEP_BYTES_PER_CLUMP if binning == "clump" else SWEEP_BYTES_PER_CLUMP) * n
     raise ValueError(name)
 
 
@@ -297,6 +308,9 @@ def main() -> int:
     ap.add_argument("--cpu-clumps", type=int, default=25000,
                     help="clumps of the workload in the CPU baseline's sample")
     ap.add_argument("--cpu-steps", type=int, default=200, help="steps of the CPU window")
+    ap.add_argument("--binning", default="clump", choices=["clump", "sphere"],
+                    help="per-step binning: one key per clump in clump cells (default) or four "
+                         "per clump in sphere cells (same results)")
     ap.add_argument("--reorder", type=int, default=50,
                     help="spatial reorder interval in steps (the headline configuration: 50)")
     ap.add_argument("--list-steps", type=int, default=20000,
@@ -354,6 +368,7 @@ def main() -> int:
     opt = dem.default_options()
     opt.verlet_skin = 0.0
     opt.reorder_interval = args.reorder
+    opt.step_binning = 0 if args.binning == "clump" else 1
     ctx = dem.Context([inst], opt)
     ctx.regrid()
     ctx.initial_forces()
@@ -395,7 +410,7 @@ def main() -> int:
     wall_graph = time.perf_counter() - t0
     ctx.close()
     value = n * args.steps / wall
-    kb = kernel_bytes(dominant, n, diag)
+    kb = kernel_bytes(dominant, n, diag, args.binning)
     achieved = kb / (kernel_ms * 1e-3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
@@ -407,13 +422,13 @@ def main() -> int:
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {n} clumps ({4 * n} spheres), FP64, sphere "
-                               f"cells rebuilt every step, spatial reorder every "
-                               f"{opt.reorder_interval} steps",
+        "config": {"workload": f"{args.workload}: {n} clumps ({4 * n} spheres), FP64, cell "
+                               f"list rebuilt every step ({args.binning} cells), spatial "
+                               f"reorder every {opt.reorder_interval} steps",
                    "clumps": n, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": KERNEL_NAMES[dominant], "kernel_ms": kernel_ms,
+                     "kernel": KERNEL_NAMES[args.binning][dominant], "kernel_ms": kernel_ms,
                      "bytes_per_launch": kb},
         # the whole step at 480 B per clump-step (algorithmic), against the HBM peak
         "roofline_step": {"achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

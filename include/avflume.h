@@ -154,6 +154,11 @@ typedef struct avf_options {
                             /* the closed forms bit for bit (fewer loads), -1 = always  */
                             /* loaded, 0 = recomputed for working sets beyond the 256   */
                             /* MiB last-level cache; same results                      */
+  int32_t step_binning;     /* per-step binning (verlet_skin 0): 0 = one key per clump   */
+                            /* in cells of the largest clump's bounding diameter and a  */
+                            /* list of candidate sphere pairings per clump, re-tested  */
+                            /* in FP64 by the force stage; 1 = four keys per clump in  */
+                            /* sphere cells and per-sphere lists; same results          */
 } avf_options;
 
 /* Status codes.  Nothing in the library calls exit(). */

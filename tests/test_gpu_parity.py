@@ -18,8 +18,9 @@ T_RTOL = 1e-9    # trajectories over 100 steps
 STATE = ["rr", "vv", "ww", "Rmatx", "Rmaty", "Rmatz", "rr_a", "rr_b", "rr_c", "rr_d", "radius"]
 
 
-def opts(graph=1, reorder=50, factor=1.0, skin=0.0, lean=0, closed=0):
+def opts(graph=1, reorder=50, factor=1.0, skin=0.0, lean=0, closed=0, binning=0):
     o = dem.default_options()
+    o.step_binning = binning
     o.use_graph = graph
     o.reorder_interval = reorder
     o.sphere_cell_factor = factor
@@ -388,13 +389,15 @@ def test_batch_equals_separate_runs(gpu):
 
 
 def test_headline_size_lists_equal_per_step_binning(gpu):
-    """At the headline size (250,000 clumps, 1M spheres): 150 steps on candidate lists, in
-    both state layouts and with a list rebuild forced by a reorder every 64 steps, equal
-    per-step binning bit for bit (state, accelerations, last forces, time and counter)."""
+    """At the headline size (250,000 clumps, 1M spheres): 150 steps with per-step binning by
+    sphere, and on candidate lists in both state layouts with a list rebuild forced by a
+    reorder every 64 steps, equal per-step binning by clump bit for bit (state,
+    accelerations, last forces, time and counter)."""
     from avalancheflume3d_amd import workloads
     inst = workloads.headline()
     outs = []
-    for kw in ({}, {"skin": 0.01, "reorder": 64, "lean": -1}, {"skin": 0.01, "reorder": 64, "lean": 1}):
+    for kw in ({}, {"binning": 1}, {"skin": 0.01, "reorder": 64, "lean": -1},
+               {"skin": 0.01, "reorder": 64, "lean": 1}):
         ctx = gpu_ctx(inst, **kw)
         ctx.initial_forces()
         ctx.step(150)
