@@ -66,7 +66,7 @@ struct avf_ctx {
   // coordinate; the lists go to the candidate-list arrays below
   bool clump_bins = false;
   uint32_t *d_ckey = nullptr, *d_crank = nullptr, *d_csid = nullptr;
-  float4 *d_crec = nullptr;
+  float4 *d_frec = nullptr;  // float records of the clump sweep, in storage order
   int64_t *d_cgrid = nullptr;  // [0] clump cells over all instances, [1] clump-sweep bricks
   // candidate lists (verlet_skin > 0): per clump up to kVMax neighbour clumps with their
   // candidate sphere pairings (entry-major), displacement tracking, chunk checkpoint
@@ -291,7 +291,7 @@ void launch_scatter(avf_ctx *c, const uint32_t *skey, const uint32_t *srank,
 void launch_bin(avf_ctx *c, int mode) {
   k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
       c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-      c->d_srank, c->n, c->cap, mode, ListTrack{nullptr, nullptr}, 0);
+      c->d_srank, c->n, c->cap, mode, ListTrack{nullptr, nullptr}, 0, nullptr);
   launch_scan(c, c->d_starts, 0, mode == 1);
   launch_scatter(c, c->d_skey, c->d_srank, c->d_starts, c->d_stage, c->d_spos, c->d_sid,
                  c->d_ccnt);
@@ -309,7 +309,7 @@ void launch_force_bs(avf_ctx *c, int initial, bool ft, int time_ahead, const dou
       c->d_vl_mask, c->d_vl_cnt, ft ? c->d_f : nullptr, c->d_diag, c->d_acc_cell, stage_out,
       c->d_orig, (VERLET && !CELLS) ? list_track(c) : ListTrack{nullptr, nullptr}, c->n, c->cap,
       initial, time_ahead, si, c->d_counts, (VERLET && CELLS) ? c->d_ckey : c->d_skey,
-      c->d_srank);
+      c->d_srank, (VERLET && CELLS) ? c->d_frec : nullptr);
 }
 
 // Touching spheres of the binned spheres (skin 0) or candidate spheres within sep + skin.
@@ -433,7 +433,7 @@ void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bo
   if (i == 0)
     k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
         c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, key, rank,
-        c->n, c->cap, 1, ListTrack{nullptr, nullptr}, cb ? 1 : 0);
+        c->n, c->cap, 1, ListTrack{nullptr, nullptr}, cb ? 1 : 0, cb ? c->d_frec : nullptr);
   mark(AVF_K_SCAN_A);
   k_count_sum<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, ncells);
   mark(AVF_K_SCAN_B);
@@ -443,14 +443,13 @@ void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bo
   mark(AVF_K_SCATTER);
   if (cb)
     k_cscatter<<<grid_for(c->n, 256), 256, 0, c->stream>>>(
-        c->d_ckey, c->d_crank, c->d_starts, cur, c->d_crec, c->d_csid, c->d_inst, c->d_P,
-        c->d_vl_cnt, c->n);
+        c->d_ckey, c->d_crank, c->d_starts, c->d_csid, c->d_vl_cnt, c->n);
   else
     launch_scatter(c, c->d_skey, c->d_srank, c->d_starts, cur, c->d_spos, c->d_sid, c->d_ccnt);
   mark(AVF_K_DETECT);
   if (cb)
     k_csweep<<<kCbGrid, kCbBlock, 0, c->stream>>>(c->d_P, c->ninst, c->d_cgrid + 1, c->d_starts,
-                                                  c->d_crec, c->d_csid, c->d_vl_m,
+                                                  c->d_frec, c->d_csid, c->d_vl_m,
                                                   c->d_vl_mask, c->d_vl_cnt, c->d_diag, c->cap);
   else
     launch_detect(c);
@@ -489,7 +488,7 @@ void launch_vstep(avf_ctx *c, bool rebuild, hipEvent_t *ev = nullptr, bool ft = 
   if (rebuild || i == 0)
     k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
         c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, c->d_skey,
-        c->d_srank, c->n, c->cap, rebuild ? 1 : 2, list_track(c), 0);
+        c->d_srank, c->n, c->cap, rebuild ? 1 : 2, list_track(c), 0, nullptr);
   mark(AVF_K_SCAN_A);
   if (rebuild) k_count_sum<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
   mark(AVF_K_SCAN_B);
@@ -1013,7 +1012,7 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
     return bail(rc);
   if (c->clump_bins &&
       ((rc = dalloc(c, &c->d_ckey, N)) || (rc = dalloc(c, &c->d_crank, N)) ||
-       (rc = dalloc(c, &c->d_csid, N)) || (rc = dalloc(c, &c->d_crec, 4 * N)) ||
+       (rc = dalloc(c, &c->d_csid, N)) || (rc = dalloc(c, &c->d_frec, 4 * N)) ||
        (rc = dalloc(c, &c->d_cgrid, 2))))
     return bail(rc);
   if (c->opt.verlet_skin > 0.0 &&
@@ -1127,7 +1126,8 @@ int avf_destroy(avf_ctx *c) {
                   c->d_acc_sg, c->d_diag,  c->d_count, c->d_clist, c->d_ccnt,
                   c->d_vl_m,   c->d_vl_cnt, c->d_vl_mask, c->d_dsp, c->d_wreach,
                   c->d_ck_st,  c->d_ck_rad, c->d_ck_P, c->d_ck_diag, c->d_umax,
-                  c->d_stage2, c->d_ckey, c->d_crank, c->d_csid, c->d_crec, c->d_cgrid};
+                  c->d_stage2, c->d_ckey, c->d_crank, c->d_csid, c->d_frec,
+                  c->d_cgrid};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
@@ -1799,7 +1799,7 @@ void win_bin(avf_ctx *c, const double *st) {
   WinBuf *w = c->win;
   k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
       const_cast<double *>(st), c->d_sta, w->bstage, c->d_inst, c->d_orig, c->d_P, c->d_counts,
-      w->bkey, w->brank, c->n, c->cap, 0, ListTrack{nullptr, nullptr}, 0);
+      w->bkey, w->brank, c->n, c->cap, 0, ListTrack{nullptr, nullptr}, 0, nullptr);
   launch_scan(c, w->bstarts);
   launch_scatter(c, w->bkey, w->brank, w->bstarts, w->bstage, w->bspos, w->bsid, nullptr);
 }

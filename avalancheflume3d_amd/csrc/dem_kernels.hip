@@ -65,6 +65,36 @@ __device__ __forceinline__ d3 sphere_of(d3 rr, const double *R, double pos, int 
                  rr.z + pos * (sx * R[6] + sy * R[7] + sz * R[8]));
 }
 
+// The clump sweep's 64-B float record of a binned clump from its stage record g: (centre,
+// radius), then the four sphere centres (sphere_centres on the record's centre and rotation:
+// the predict's FP64 centres) packed in three float4, coordinates relative to the
+// instance's clump-grid origin.  The radius is the record's (a binned clump is not frozen).
+__device__ __forceinline__ void clump_frec(const double *g, const InstParams &p,
+                                           float4 *__restrict__ out) {
+  const d3 rr = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
+  const double rad = g[G_RAD];
+  double R[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) R[k] = g[G_R + k];
+  d3 s[4];
+  sphere_centres(rr, R, rad / p.sphere_div, s);
+  const double ox = p.cg_lo[0], oy = p.cg_lo[1], oz = p.cg_lo[2];
+  float f[15];
+  f[0] = (float)(rr.x - ox);
+  f[1] = (float)(rr.y - oy);
+  f[2] = (float)(rr.z - oz);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    f[3 + 3 * k] = (float)(s[k].x - ox);
+    f[4 + 3 * k] = (float)(s[k].y - oy);
+    f[5 + 3 * k] = (float)(s[k].z - oz);
+  }
+  out[0] = make_float4(f[0], f[1], f[2], (float)rad);
+  out[1] = make_float4(f[3], f[4], f[5], f[6]);
+  out[2] = make_float4(f[7], f[8], f[9], f[10]);
+  out[3] = make_float4(f[11], f[12], f[13], f[14]);
+}
+
 // Mass and moment of inertia (isotropic) of a clump of four spheres of radius rad and density
 // rho, in the operation order of the library's setup (calculate_particle_parameters.cu:27-33;
 // clump_mode 1: I = 4 m_s (2/5 r^2 + 2/3 d^2) for the sphere offset d = sqrt(3) r / div).
@@ -322,7 +352,8 @@ __global__ __launch_bounds__(kPredictBlock) void k_predict(
     double *__restrict__ st, double *__restrict__ sta, double *__restrict__ stage,
     const int32_t *__restrict__ inst_of, const int32_t *__restrict__ orig,
     const InstParams *__restrict__ P, uint32_t *__restrict__ counts, uint32_t *__restrict__ skey,
-    uint32_t *__restrict__ srank, int64_t n, int64_t cap, int mode, ListTrack lt, int ckeys) {
+    uint32_t *__restrict__ srank, int64_t n, int64_t cap, int mode, ListTrack lt, int ckeys,
+    float4 *__restrict__ frec) {
   __shared__ double rec[kPredictBlock * kStagePad];
   const int64_t c0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kPredictBlock;
   const int64_t c = c0 + threadIdx.x;
@@ -347,6 +378,8 @@ __global__ __launch_bounds__(kPredictBlock) void k_predict(
     }
   }
   __syncthreads();
+  if (frec && c < n && keys[0] != kInvalidKey)  // the clump sweep's record (clump keys)
+    clump_frec(rec + threadIdx.x * kStagePad, P[inst_of[c]], frec + 4 * c);
   const int64_t rows = n - c0 < kPredictBlock ? n - c0 : kPredictBlock;
   double *dst = stage + c0 * kStageDoubles;
   for (int64_t k = threadIdx.x; k < rows * kStageDoubles; k += kPredictBlock)
@@ -1436,7 +1469,7 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
     long long *__restrict__ acc, double *__restrict__ stage_out,
     const int32_t *__restrict__ orig, ListTrack lt, int64_t n, int64_t cap, int initial,
     int time_ahead, int si, uint32_t *__restrict__ counts, uint32_t *__restrict__ skey,
-    uint32_t *__restrict__ srank) {
+    uint32_t *__restrict__ srank, float4 *__restrict__ frec) {
   static_assert(!NEXT || VERLET || CELLS, "the fused predict needs the record-resident state");
   static_assert(sizeof(PPShared<MAXN, BS>) >= sizeof(double) * BS * kStagePad,
                 "stage records of the workgroup are assembled in the pair scratch");
@@ -1540,6 +1573,8 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
         reinterpret_cast<uint4 *>(skey)[c] = make_uint4(keys[0], keys[1], keys[2], keys[3]);
     }
     __syncthreads();
+    if constexpr (CELLS && VERLET)  // the clump sweep's float record of the next step
+      if (c < n && keys[0] != kInvalidKey) clump_frec(rec + lane * kStagePad, P[inst], frec + 4 * c);
     const int64_t rows = n - c0 < BS ? n - c0 : BS;
     double *dst = stage_out + c0 * kStageDoubles;
     for (int64_t k = lane; k < rows * kStageDoubles; k += BS)
@@ -2104,9 +2139,10 @@ __global__ __launch_bounds__(kSwBlock) void k_sweep(
 // Per-step binning by clump: one key per clump (the clump-cell grid cell of its centre, cell
 // side >= the largest clump's bounding diameter, so the clumps of any touching pair sit in
 // neighbouring cells), counted, scanned and scattered like the sphere keys (a quarter of the
-// keys, a fifth of the cells).  k_cscatter writes each binned clump's sorted record: its
-// centre, radius and four sphere centres as floats relative to the grid origin (64 B) and
-// its storage index.  k_csweep then lists, for every binned clump, the clumps of its 27
+// keys, a fifth of the cells).  The predict writes each binned clump's float record in
+// storage order (its centre, radius and four sphere centres as floats relative to the grid
+// origin: 64 B); k_cscatter places the storage indices in cell order, through which the
+// sweep reads the records.  k_csweep then lists, for every binned clump, the clumps of its 27
 // neighbour cells that may touch it, with the pairings that may: a float test with a margin
 // far above its rounding error (centre distance against the bounding radii, then the 16
 // sphere pairs), so the listed pairings are a superset of the touching ones.  The force
@@ -2121,59 +2157,30 @@ constexpr int kCbOwnRows = kCbY * kCbZ;       // rows of the brick itself
 constexpr int kCbRowStarts = kCbHX + 1;       // first clump of each halo cell, and the end
 constexpr int kCbBlock = 256;
 constexpr int kCbCap = 512;                   // halo clumps staged in LDS
+constexpr int kCq = 8;                        // candidates queued per lane between the passes
 constexpr int kCbGrid = 4096;                 // workgroups (bricks in strides of kCbGrid)
 static_assert(kCbRows <= 64 && kCbOwnRows <= 64, "one wavefront scans the rows");
+static_assert(kCbCap <= 1024 && kCbRows <= 64, "queue entries: 6-bit row, 10-bit index");
 
 __device__ __forceinline__ int64_t cbricks_of(const InstParams &p) {
   return (int64_t)((p.cg_n[0] + kCbX - 1) / kCbX) * ((p.cg_n[1] + kCbY - 1) / kCbY) *
          ((p.cg_n[2] + kCbZ - 1) / kCbZ);
 }
 
-// One lane per binned clump (storage order): its sorted record at its cell's start + its
-// rank (from the count).  Record: (centre, radius), then the four sphere centres packed in
-// three float4; coordinates relative to the instance's grid origin, rounded from the FP64
-// centres of the predict (sphere_centres on the stage record).  A clump that is not binned
-// (past side.x) gets an empty list.
-__global__ __launch_bounds__(256) void k_cscatter(
-    const uint32_t *__restrict__ ckey, const uint32_t *__restrict__ crank,
-    const uint32_t *__restrict__ starts, const double *__restrict__ stage,
-    float4 *__restrict__ crec, uint32_t *__restrict__ csid, const int32_t *__restrict__ inst_of,
-    const InstParams *__restrict__ P, int32_t *__restrict__ vl_cnt, int64_t n) {
+// One lane per binned clump (storage order): its storage index to its cell's start + its rank
+// (from the count).  A clump that is not binned (past side.x) gets an empty list.
+__global__ __launch_bounds__(256) void k_cscatter(const uint32_t *__restrict__ ckey,
+                                                  const uint32_t *__restrict__ crank,
+                                                  const uint32_t *__restrict__ starts,
+                                                  uint32_t *__restrict__ csid,
+                                                  int32_t *__restrict__ vl_cnt, int64_t n) {
   const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-  if (c < n) {
-    const uint32_t key = ckey[c];
-    if (key == kInvalidKey) {
-      vl_cnt[c] = 0;
-    } else {
-      const uint32_t dst = starts[key] + crank[c];
-      const InstParams &p = P[inst_of[c]];
-      const double2 *g2 = reinterpret_cast<const double2 *>(stage + c * kStageDoubles);
-      const double2 a = g2[0], b = g2[1];  // rr.x, rr.y | rr.z, rad
-      const double2 r0 = g2[6], r1 = g2[7], r2 = g2[8], r3 = g2[9], r4 = g2[10];
-      const double R[9] = {r0.x, r0.y, r1.x, r1.y, r2.x, r2.y, r3.x, r3.y, r4.x};
-      const d3 rr = make_d3(a.x, a.y, b.x);
-      const double rad = b.y;  // binned: not frozen, the pre-freeze radius
-      d3 s[4];
-      sphere_centres(rr, R, rad / p.sphere_div, s);
-      const double ox = p.cg_lo[0], oy = p.cg_lo[1], oz = p.cg_lo[2];
-      float f[15];
-      f[0] = (float)(rr.x - ox);
-      f[1] = (float)(rr.y - oy);
-      f[2] = (float)(rr.z - oz);
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        f[3 + 3 * k] = (float)(s[k].x - ox);
-        f[4 + 3 * k] = (float)(s[k].y - oy);
-        f[5 + 3 * k] = (float)(s[k].z - oz);
-      }
-      float4 *o = crec + 4 * (int64_t)dst;
-      o[0] = make_float4(f[0], f[1], f[2], (float)rad);
-      o[1] = make_float4(f[3], f[4], f[5], f[6]);
-      o[2] = make_float4(f[7], f[8], f[9], f[10]);
-      o[3] = make_float4(f[11], f[12], f[13], f[14]);
-      csid[dst] = (uint32_t)c;
-    }
-  }
+  if (c >= n) return;
+  const uint32_t key = ckey[c];
+  if (key == kInvalidKey)
+    vl_cnt[c] = 0;
+  else
+    csid[starts[key] + crank[c]] = (uint32_t)c;
 }
 
 struct CSweepLds {
@@ -2183,7 +2190,7 @@ struct CSweepLds {
   uint32_t xmax;                       // largest coordinate magnitude of the halo (float bits)
   float4 ca[kCbCap];                   // centre, radius
   float4 cs[kCbCap][3];                // the four sphere centres
-  uint32_t id[kCbCap];                 // storage index
+  uint16_t q[kCq][kCbBlock];           // per lane: queued candidates (row << 10 | index)
 };
 
 // Largest coordinate magnitude of a record (the float test's rounding error scales with it).
@@ -2213,58 +2220,89 @@ __device__ __forceinline__ uint32_t cand_mask(const float *qs, const float4 *os,
   return mask;
 }
 
-// The candidate list of one own clump (storage index me, record qa / qs, largest coordinate
-// magnitude xq) from the clumps of the 9 neighbour rows' cells lx - 1 .. lx + 1 of halo row
-// hr; records from LDS (GLOBAL false: indices into the staged halo, mh covers every record
-// of the halo) or from the sorted arrays (GLOBAL true: mh is widened per pair for the two
-// records' magnitudes).  self: the clump's own index in the same numbering.  Entries go to
-// vl_m / vl_mask (entry-major, as the candidate lists of k_vmerge), at most kVMax (the rest
-// are counted in diag.neigh_overflow).
-template <bool GLOBAL>
-__device__ __forceinline__ void csweep_clump(const CSweepLds &sh, const float4 *__restrict__ crec,
-                                             const uint32_t *__restrict__ csid, float4 qa,
-                                             const float *qs, float xq, uint32_t self,
-                                             uint32_t me, int hr, int lx, float bfac, float mh0,
-                                             int32_t *__restrict__ vl_m,
-                                             uint16_t *__restrict__ vl_mask,
-                                             int32_t *__restrict__ vl_cnt, int64_t cap,
-                                             DevDiag *__restrict__ diag) {
+// The candidate list of one own clump (storage index me, record qa / qs, LDS index self) from
+// the staged halo: the clumps of the 9 neighbour rows' cells lx - 1 .. lx + 1 of halo row hr.
+// Two passes, so that the lanes of a wavefront test sphere pairs together instead of each
+// whenever one of them meets a candidate: the bounding-sphere test queues each candidate
+// that passes (in LDS, kCq per lane; a full queue is emptied on the spot), then the 16
+// sphere pairs of the queued candidates.  Entries go to vl_m / vl_mask (entry-major, as the
+// candidate lists of k_vmerge; the storage index from csid), at most kVMax (the rest are
+// counted in diag.neigh_overflow).  mh covers every record of the halo.
+__device__ __forceinline__ void csweep_lds(CSweepLds &sh, const uint32_t *__restrict__ csid,
+                                           float4 qa, const float *qs, uint32_t self,
+                                           uint32_t me, int hr, int lx, float bfac, float mh,
+                                           int tid, int32_t *__restrict__ vl_m,
+                                           uint16_t *__restrict__ vl_mask,
+                                           int32_t *__restrict__ vl_cnt, int64_t cap,
+                                           DevDiag *__restrict__ diag) {
+  int cnt = 0, nq = 0;
+  auto drain = [&]() {
+    for (int k = 0; k < nq; k++) {
+      const uint32_t v = sh.q[k][tid], j = v & 0x3ffu, row = v >> 10;
+      const float4 os[3] = {sh.cs[j][0], sh.cs[j][1], sh.cs[j][2]};
+      const uint32_t mask = cand_mask(qs, os, qa.w + sh.ca[j].w + mh);
+      if (!mask) continue;
+      if (cnt < kVMax) {
+        vl_m[cnt * cap + me] = (int32_t)csid[sh.gs[row][0] + (j - sh.hoff[row])];
+        vl_mask[cnt * cap + me] = (uint16_t)mask;
+      }
+      cnt++;
+    }
+    nq = 0;
+  };
+#pragma unroll 1
+  for (int r9 = 0; r9 < 9; r9++) {
+    const int nr = hr + (r9 % 3 - 1) + kCbHY * (r9 / 3 - 1);
+    const uint32_t base = sh.hoff[nr] - sh.gs[nr][0];
+    const uint32_t e = sh.gs[nr][lx + 2] + base;
+    for (uint32_t j = sh.gs[nr][lx - 1] + base; j < e; j++) {
+      if (j == self) continue;
+      const float4 oa = sh.ca[j];
+      const float dx = oa.x - qa.x, dy = oa.y - qa.y, dz = oa.z - qa.z;
+      const float bb = __fmaf_rn(qa.w + oa.w, bfac, mh);
+      if (!(__fmaf_rn(dx, dx, __fmaf_rn(dy, dy, dz * dz)) < bb * bb)) continue;
+      if (nq == kCq) drain();
+      sh.q[nq++][tid] = (uint16_t)((nr << 10) | j);
+    }
+  }
+  drain();
+  if (cnt > kVMax) {
+    atomicAdd(&diag->neigh_overflow, 1ull);
+    cnt = kVMax;
+  }
+  vl_cnt[me] = cnt;
+}
+
+// The same list from the sorted records in global memory (halos too full for LDS): global
+// sorted indices, own index self; mh is widened per pair for the two records' largest
+// coordinate magnitudes (xq: the own clump's).
+__device__ __forceinline__ void csweep_global(const CSweepLds &sh, const float4 *__restrict__ frec,
+                                              const uint32_t *__restrict__ csid, float4 qa,
+                                              const float *qs, float xq, uint32_t self,
+                                              uint32_t me, int hr, int lx, float bfac,
+                                              float mh0, int32_t *__restrict__ vl_m,
+                                              uint16_t *__restrict__ vl_mask,
+                                              int32_t *__restrict__ vl_cnt, int64_t cap,
+                                              DevDiag *__restrict__ diag) {
   int cnt = 0;
 #pragma unroll 1
   for (int r9 = 0; r9 < 9; r9++) {
     const int nr = hr + (r9 % 3 - 1) + kCbHY * (r9 / 3 - 1);
-    uint32_t j = sh.gs[nr][lx - 1], e = sh.gs[nr][lx + 2];
-    if (!GLOBAL) {
-      const uint32_t base = sh.hoff[nr] - sh.gs[nr][0];
-      j += base;
-      e += base;
-    }
-    for (; j < e; j++) {
+    const uint32_t e = sh.gs[nr][lx + 2];
+    for (uint32_t j = sh.gs[nr][lx - 1]; j < e; j++) {
       if (j == self) continue;
-      float4 oa, os[3];
-      float mh = mh0;
-      if (GLOBAL) {
-        const float4 *r = crec + 4 * (int64_t)j;
-        oa = r[0];
-        os[0] = r[1];
-        os[1] = r[2];
-        os[2] = r[3];
-        mh = fmaxf(mh0, (xq + rec_extent(oa, os[0], os[1], os[2])) * 0x1p-21f);
-      } else {
-        oa = sh.ca[j];
-      }
+      const uint32_t m = csid[j];
+      const float4 *r = frec + 4 * (int64_t)m;
+      const float4 oa = r[0], os[3] = {r[1], r[2], r[3]};
+      const float mh = fmaxf(mh0, (xq + rec_extent(oa, os[0], os[1], os[2])) * 0x1p-21f);
       // the bounding spheres (radius r * bfac) first, then the 16 sphere pairs
       const float dx = oa.x - qa.x, dy = oa.y - qa.y, dz = oa.z - qa.z;
       const float rs = qa.w + oa.w, bb = __fmaf_rn(rs, bfac, mh);
       if (!(__fmaf_rn(dx, dx, __fmaf_rn(dy, dy, dz * dz)) < bb * bb)) continue;
-      if (!GLOBAL) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) os[k] = sh.cs[j][k];
-      }
       const uint32_t mask = cand_mask(qs, os, rs + mh);
       if (!mask) continue;
       if (cnt < kVMax) {
-        vl_m[cnt * cap + me] = (int32_t)(GLOBAL ? csid[j] : sh.id[j]);
+        vl_m[cnt * cap + me] = (int32_t)m;
         vl_mask[cnt * cap + me] = (uint16_t)mask;
       }
       cnt++;
@@ -2285,7 +2323,7 @@ __device__ __forceinline__ void csweep_clump(const CSweepLds &sh, const float4 *
 // workgroup b mod G as in k_sweep.
 __global__ __launch_bounds__(kCbBlock) void k_csweep(
     const InstParams *__restrict__ P, int32_t ninst, const int64_t *__restrict__ nbricks_p,
-    const uint32_t *__restrict__ starts, const float4 *__restrict__ crec,
+    const uint32_t *__restrict__ starts, const float4 *__restrict__ frec,
     const uint32_t *__restrict__ csid, int32_t *__restrict__ vl_m, uint16_t *__restrict__ vl_mask, int32_t *__restrict__ vl_cnt,
     DevDiag *__restrict__ diag, int64_t cap) {
   __shared__ CSweepLds sh;
@@ -2348,19 +2386,17 @@ __global__ __launch_bounds__(kCbBlock) void k_csweep(
     const uint32_t H = sh.hoff[kCbRows], K = sh.ownp[kCbOwnRows];
     if (K == 0u) continue;
     const bool lds = H <= (uint32_t)kCbCap;
-    if (lds) {  // the halo's records, one clump per thread (five loads in flight)
+    if (lds) {  // the halo's records through their storage indices, one clump per thread
       float ext = 0.0f;
       for (uint32_t t = tid; t < H; t += kCbBlock) {
         const int hr = upper_row(sh.hoff, kCbRows, t);
         const int64_t j = (int64_t)sh.gs[hr][0] + (t - sh.hoff[hr]);
-        const float4 *r = crec + 4 * j;
+        const float4 *r = frec + 4 * (int64_t)csid[j];
         const float4 v0 = r[0], v1 = r[1], v2 = r[2], v3 = r[3];
-        const uint32_t id = csid[j];
         sh.ca[t] = v0;
         sh.cs[t][0] = v1;
         sh.cs[t][1] = v2;
         sh.cs[t][2] = v3;
-        sh.id[t] = id;
         ext = fmaxf(ext, rec_extent(v0, v1, v2, v3));
       }
       // the halo's largest coordinate magnitude (NaN coordinates leave it alone; they touch
@@ -2388,28 +2424,26 @@ __global__ __launch_bounds__(kCbBlock) void k_csweep(
         if (sh.gs[hr][x] <= jg) lx = x;
       const uint32_t t = sh.hoff[hr] + (jg - sh.gs[hr][0]);
       float4 qa, q1, q2, q3;
-      uint32_t me;
+      const uint32_t me = csid[jg];
       if (lds) {
         qa = sh.ca[t];
         q1 = sh.cs[t][0];
         q2 = sh.cs[t][1];
         q3 = sh.cs[t][2];
-        me = sh.id[t];
       } else {
-        const float4 *r = crec + 4 * (int64_t)jg;
+        const float4 *r = frec + 4 * (int64_t)me;
         qa = r[0];
         q1 = r[1];
         q2 = r[2];
         q3 = r[3];
-        me = csid[jg];
       }
       const float qs[12] = {q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
       if (lds)
-        csweep_clump<false>(sh, crec, csid, qa, qs, 0.0f, t, me, hr, lx, bfac, mh, vl_m,
-                            vl_mask, vl_cnt, cap, diag);
+        csweep_lds(sh, csid, qa, qs, t, me, hr, lx, bfac, mh, tid, vl_m, vl_mask, vl_cnt, cap,
+                   diag);
       else
-        csweep_clump<true>(sh, crec, csid, qa, qs, rec_extent(qa, q1, q2, q3), jg, me, hr, lx,
-                           bfac, mh, vl_m, vl_mask, vl_cnt, cap, diag);
+        csweep_global(sh, frec, csid, qa, qs, rec_extent(qa, q1, q2, q3), jg, me, hr, lx, bfac,
+                      mh, vl_m, vl_mask, vl_cnt, cap, diag);
     }
   }
 }
