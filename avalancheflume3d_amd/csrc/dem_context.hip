@@ -58,6 +58,7 @@ struct avf_ctx {
   int32_t *d_orig = nullptr, *d_orig2 = nullptr, *d_inst = nullptr, *d_inst2 = nullptr;
   int64_t *d_inv = nullptr, *d_ncells = nullptr, *d_nbricks = nullptr, *d_tmp64 = nullptr;
   uint32_t *d_counts = nullptr, *d_starts = nullptr, *d_bsum = nullptr;
+  uint32_t *d_bsum2 = nullptr;  // per-step binning: share sums added up by the count pass (x2)
   uint32_t *d_skey = nullptr, *d_srank = nullptr, *d_sid = nullptr, *d_clist = nullptr;
   int32_t *d_ccnt = nullptr;
   double4 *d_spos = nullptr;
@@ -88,6 +89,7 @@ struct avf_ctx {
   double *d_scratch = nullptr;  // this device's address of scratch (avf_observe_noalloc)
   InstParams *h_P = nullptr;
   unsigned long long seen_below = 0, seen_overflow = 0;  // counters at the last check
+  unsigned long long seen_broken = 0;
   bool list_valid = false;
   int64_t vchunk = 8;             // next chunk of candidate-list steps
   int64_t since_build = 0;        // steps since the last list build
@@ -273,7 +275,7 @@ void launch_scan(avf_ctx *c, uint32_t *starts, int finalize_prev = 0, int advanc
   k_count_sum<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_ncells);
   k_count_scan<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, starts,
                                                 c->d_ncells, c->d_P, c->d_acc_cell, c->ninst,
-                                                finalize_prev, advance);
+                                                finalize_prev, advance, 1, nullptr);
 }
 
 // Spheres into cell order at their cell's start (from the scan) + their rank (from the
@@ -283,7 +285,7 @@ void launch_scatter(avf_ctx *c, const uint32_t *skey, const uint32_t *srank,
                     const uint32_t *starts, const double *stage, double4 *spos, uint32_t *sid,
                     int32_t *ccnt) {
   k_scatter<<<std::max(1u, grid_for(4 * c->n, kScatterBlock)), kScatterBlock, 0, c->stream>>>(
-      skey, srank, starts, stage, spos, sid, c->d_inst, c->d_P, ccnt, 4 * c->n);
+      skey, srank, starts, stage, spos, sid, c->d_inst, c->d_P, ccnt, c->d_diag, 4 * c->n);
 }
 
 // Binning of the staged spheres: predict (mode 1, advancing the time) or stage (mode 0),
@@ -414,7 +416,8 @@ void launch_cforce(avf_ctx *c, bool ft, const double *stage_in, double *stage_ou
 // those keys in this step's sorted order; the chunk's last force stage writes the state
 // back to st instead and one k_finalize applies its regrid and cc++.  ev (optional,
 // AVF_NKERNELS + 1 events) brackets the kernels in the slots of avf_step_profiled (SCAN_A:
-// count sums, SCAN_B: scan, SCAN_C: empty, DETECT: sweep, COUNT: the next step's counts).
+// count sums of a chunk's first step, SCAN_B: scan, SCAN_C: empty, DETECT: sweep, COUNT: the
+// next step's counts with the scan's share sums).
 void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bool ft = true) {
   auto mark = [&](int k) {
     if (ev && (c->prof_kernel < 0 || k == c->prof_kernel || k == c->prof_kernel + 1))
@@ -435,22 +438,29 @@ void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bo
         c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, key, rank,
         c->n, c->cap, 1, ListTrack{nullptr, nullptr}, cb ? 1 : 0, cb ? c->d_frec : nullptr);
   mark(AVF_K_SCAN_A);
-  k_count_sum<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, ncells);
+  // the share sums of the scan: k_count_sum on a chunk's first step, after that added up by
+  // the previous step's count pass into the step's half of d_bsum2 (the scan clears the
+  // other half for the next count)
+  uint32_t *bs = i == 0 ? c->d_bsum : c->d_bsum2 + (size_t)(i & 1) * kScanGrid * kBsumStride;
+  uint32_t *bs_next = c->d_bsum2 + (size_t)((i + 1) & 1) * kScanGrid * kBsumStride;
+  if (i == 0) k_count_sum<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, ncells);
   mark(AVF_K_SCAN_B);
-  k_count_scan<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_starts, ncells,
-                                                c->d_P, c->d_acc_cell, c->ninst, i > 0, 1);
+  k_count_scan<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, bs, c->d_starts, ncells, c->d_P,
+                                                c->d_acc_cell, c->ninst, i > 0, 1,
+                                                i == 0 ? 1 : kBsumStride, bs_next);
   mark(AVF_K_SCAN_C);
   mark(AVF_K_SCATTER);
   if (cb)
     k_cscatter<<<grid_for(c->n, 256), 256, 0, c->stream>>>(
-        c->d_ckey, c->d_crank, c->d_starts, c->d_csid, c->d_vl_cnt, c->n);
+        c->d_ckey, c->d_crank, c->d_starts, c->d_csid, c->d_vl_cnt, c->d_diag, c->n);
   else
     launch_scatter(c, c->d_skey, c->d_srank, c->d_starts, cur, c->d_spos, c->d_sid, c->d_ccnt);
   mark(AVF_K_DETECT);
   if (cb)
     k_csweep<<<kCbGrid, kCbBlock, 0, c->stream>>>(c->d_P, c->ninst, c->d_cgrid + 1, c->d_starts,
                                                   c->d_frec, c->d_csid, c->d_vl_m,
-                                                  c->d_vl_mask, c->d_vl_cnt, c->d_diag, c->cap);
+                                                  c->d_vl_mask, c->d_vl_cnt, c->d_diag, c->cap,
+                                                  c->n);
   else
     launch_detect(c);
   mark(AVF_K_FORCE);
@@ -463,7 +473,8 @@ void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bo
   mark(AVF_K_COUNT);
   if (i + 1 < S)  // the next step's counts and ranks, in this step's sorted order
     k_count_ranks<<<std::max(1u, grid_for((cb ? 1 : 4) * c->n, 256)), 256, 0, c->stream>>>(
-        key, sid, c->d_starts, ncells, c->d_counts, rank);
+        key, sid, c->d_starts, ncells, c->d_counts, rank, bs_next, (int32_t)G,
+        (cb ? 1 : 4) * c->n);
   mark(AVF_NKERNELS);
 }
 
@@ -495,7 +506,7 @@ void launch_vstep(avf_ctx *c, bool rebuild, hipEvent_t *ev = nullptr, bool ft = 
   if (rebuild)  // also advances the time (between the predict and the force stage)
     k_count_scan<<<G, kScanBlock, 0, c->stream>>>(c->d_counts, c->d_bsum, c->d_starts,
                                                   c->d_ncells, c->d_P,
-                                                  c->d_acc_cell, c->ninst, 0, 1);
+                                                  c->d_acc_cell, c->ninst, 0, 1, 1, nullptr);
   mark(AVF_K_SCAN_C);
   mark(AVF_K_SCATTER);
   if (rebuild)
@@ -658,8 +669,13 @@ int step_status(avf_ctx *c) {
   const unsigned long long below = c->h_mail->diag.cell_below;
   const unsigned long long over = c->h_mail->diag.neigh_overflow;
   const unsigned long long nb = below - c->seen_below, no = over - c->seen_overflow;
+  const unsigned long long broken = c->h_mail->diag.bin_broken - c->seen_broken;
   c->seen_below = below;
   c->seen_overflow = over;
+  c->seen_broken = c->h_mail->diag.bin_broken;
+  if (broken)
+    return fail(c, AVF_ERR_HIP, "%llu binned entries beyond the binned count (binning broken)",
+                broken);
   if (nb)
     return fail(c, AVF_ERR_CELL_NEG,
                 "%llu clump visit(s) outside the clump-cell grid (the reference exits)", nb);
@@ -815,7 +831,7 @@ int csteps_profiled(avf_ctx *c, int64_t S, StepProf *prof) {
       if (c->prof_kernel >= 0 && j != c->prof_kernel) continue;
       // slots of kernels this step does not launch stay out of the sums
       if (j == AVF_K_SCAN_C || j == AVF_K_CELLREDUCE) continue;
-      if (j == AVF_K_PREDICT && timed[t] != 0) continue;
+      if ((j == AVF_K_PREDICT || j == AVF_K_SCAN_A) && timed[t] != 0) continue;
       if (j == AVF_K_FINALIZE && timed[t] + 1 != S) continue;
       if (j == AVF_K_COUNT && timed[t] + 1 == S) continue;
       const size_t i = (size_t)timed[t];
@@ -993,6 +1009,7 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
       (rc = dalloc(c, &c->d_tmp64, N)) || (rc = dalloc(c, &c->d_counts, c->cell_cap)) ||
       (rc = dalloc(c, &c->d_starts, c->cell_cap + 1)) ||
       (rc = dalloc(c, &c->d_bsum, grid_for(c->cell_cap, kScanTile) + 1)) ||
+      (rc = dalloc(c, &c->d_bsum2, 2 * (size_t)kScanGrid * kBsumStride)) ||
       (rc = dalloc(c, &c->d_skey, 4 * N)) || (rc = dalloc(c, &c->d_srank, 4 * N)) ||
       (rc = dalloc(c, &c->d_sid, 4 * N)) || (rc = dalloc(c, &c->d_spos, 4 * N)) ||
       (rc = dalloc(c, &c->d_clist, 4 * N * (c->opt.verlet_skin > 0.0 ? kSphListV : kSphList))) ||
@@ -1053,13 +1070,16 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
   UP(c->d_acc_sg, acc.data(), acc.size() * sizeof(long long));
 #undef UP
   if ((he = hipMemset(c->d_counts, 0, c->cell_cap * sizeof(uint32_t))) != hipSuccess ||
+      (he = hipMemset(c->d_bsum2, 0, 2 * (size_t)kScanGrid * kBsumStride * sizeof(uint32_t))) !=
+          hipSuccess ||
       (he = hipMemset(c->d_st, 0, N * S_NFIELDS * sizeof(double))) != hipSuccess ||
       (he = hipMemset(c->d_sta, 0, N * C_NFIELDS * sizeof(double))) != hipSuccess ||
       (he = hipMemset(c->d_f, 0, N * F_NFIELDS * sizeof(double))) != hipSuccess ||
       (he = hipMemset(c->d_diag, 0, sizeof(DevDiag))) != hipSuccess ||
       (he = hipMemset(c->d_ncells, 0, sizeof(int64_t))) != hipSuccess ||
       (he = hipMemset(c->d_nbricks, 0, sizeof(int64_t))) != hipSuccess ||
-      (c->d_cgrid && (he = hipMemset(c->d_cgrid, 0, 2 * sizeof(int64_t))) != hipSuccess)) {
+      (c->d_cgrid && (he = hipMemset(c->d_cgrid, 0, 2 * sizeof(int64_t))) != hipSuccess) ||
+      (c->d_csid && (he = hipMemset(c->d_csid, 0, N * sizeof(uint32_t))) != hipSuccess)) {
     fail(c, AVF_ERR_HIP, "hipMemset: %s", hipGetErrorString(he));
     return bail(AVF_ERR_HIP);
   }
@@ -1122,7 +1142,7 @@ int avf_destroy(avf_ctx *c) {
                   c->d_f,      c->d_io,    c->d_io2,   c->d_orig,  c->d_orig2, c->d_inst,
                   c->d_inst2,  c->d_inv,   c->d_ncells, c->d_nbricks, c->d_tmp64, c->d_counts,
                   c->d_starts,
-                  c->d_bsum,   c->d_skey,  c->d_srank, c->d_sid,   c->d_spos,  c->d_acc_cell,
+                  c->d_bsum,   c->d_bsum2, c->d_skey,  c->d_srank, c->d_sid,   c->d_spos,  c->d_acc_cell,
                   c->d_acc_sg, c->d_diag,  c->d_count, c->d_clist, c->d_ccnt,
                   c->d_vl_m,   c->d_vl_cnt, c->d_vl_mask, c->d_dsp, c->d_wreach,
                   c->d_ck_st,  c->d_ck_rad, c->d_ck_P, c->d_ck_diag, c->d_umax,
@@ -1664,7 +1684,7 @@ int avf_get_diag(avf_ctx *c, avf_diag *d) {
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const DevDiag h = c->h_mail->diag;
-  c->seen_below = c->seen_overflow = 0;  // the device counters restart from zero
+  c->seen_below = c->seen_overflow = c->seen_broken = 0;  // the device counters restart from zero
   int64_t closed = 0;
   for (int32_t i = 0; i < c->ninst; i++) closed += c->h_P[i].mass_closed != 0;
   d->cell_below = (int64_t)h.cell_below;

@@ -77,6 +77,7 @@ constexpr int kScanTile = kScanBlock * kScanItems;
 struct DevDiag {
   unsigned long long cell_below, cell_beyond, neigh_overflow, max_neighbours, max_cell;
   unsigned long long sweep_global, sweep_fp64;  // bricks swept from global memory, FP64 pairs
+  unsigned long long bin_broken;  // scattered entries beyond the binned count (never expected)
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8),

@@ -394,6 +394,7 @@ __global__ __launch_bounds__(kPredictBlock) void k_predict(
 // share of the cells tile by tile (writing starts, zeroing the counts for the next binning).
 
 constexpr int kScanGrid = 1024;
+constexpr int kBsumStride = 32;  // words between the share sums the count pass adds into
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
   const int lane = threadIdx.x & 63;
@@ -465,17 +466,50 @@ __global__ __launch_bounds__(kScanBlock) void k_count_sum(const uint32_t *__rest
 // Every sphere with a key for the next step was binned in this one: a clump that is not
 // binned is frozen (past side.x after t = 0), and a frozen clump neither moves nor thaws
 // within a chunk (side.x only changes between chunks), so it stays unbinned.
+//
+// The same pass adds the counts of each share of cells of the scan that follows into
+// bsum[share * kBsumStride] (the shares of k_count_sum, nb scan workgroups), so that scan
+// needs no k_count_sum: in sorted order the lanes of a wavefront fall in one or two shares,
+// and each wavefront adds once per share it meets.  The sums sit a cache line apart: atomics
+// to one line are performed one after the other.  bsum must hold zeros (k_count_scan clears
+// the buffer of the next step).
 __global__ __launch_bounds__(256) void k_count_ranks(const uint32_t *__restrict__ skey,
                                                      const uint32_t *__restrict__ sid,
                                                      const uint32_t *__restrict__ starts,
                                                      const int64_t *__restrict__ ncells_p,
                                                      uint32_t *__restrict__ counts,
-                                                     uint32_t *__restrict__ srank) {
+                                                     uint32_t *__restrict__ srank,
+                                                     uint32_t *__restrict__ bsum, int32_t nb,
+                                                     int64_t nitems) {
   const int64_t j = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-  if (j >= (int64_t)starts[*ncells_p]) return;
-  const uint32_t id = sid[j];
-  const uint32_t key = skey[id];
-  if (key != kInvalidKey) srank[id] = atomicAdd(&counts[key], 1u);
+  const int64_t ncells = *ncells_p;
+  bool todo = false;
+  uint32_t key = kInvalidKey;
+  // the binned items of this step (never more than nitems, the length of sid)
+  const int64_t nbin = (int64_t)starts[ncells] < nitems ? (int64_t)starts[ncells] : nitems;
+  if (j < nbin) {
+    const uint32_t id = sid[j];
+    key = skey[id];
+    if (key != kInvalidKey) {
+      srank[id] = atomicAdd(&counts[key], 1u);
+      todo = true;
+    }
+  }
+  // the scan workgroup whose share of cells holds key (scan_share's shares)
+  const int64_t tiles = (ncells + kScanTile - 1) / kScanTile;
+  const int64_t span = ((tiles + nb - 1) / nb) * kScanTile;
+  const uint32_t w = todo ? (uint32_t)((int64_t)key / span) : 0u;
+  const int lane = threadIdx.x & 63;
+  while (true) {
+    const unsigned long long b = __ballot(todo);
+    if (!b) break;
+    const int first = __ffsll((long long)b) - 1;
+    const uint32_t wf = __shfl(w, first, 64);
+    const bool mine = todo && w == wf;
+    const unsigned long long mm = __ballot(mine);
+    if (lane == first) atomicAdd(&bsum[(size_t)wf * kBsumStride], (uint32_t)__popcll(mm));
+    todo = todo && !mine;
+  }
 }
 
 // ----------------------------------------------------------------------- scatter ----
@@ -491,7 +525,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter(
     const uint32_t *__restrict__ starts, const double *__restrict__ stage,
     double4 *__restrict__ spos, uint32_t *__restrict__ sid,
     const int32_t *__restrict__ inst_of, const InstParams *__restrict__ P,
-    int32_t *__restrict__ ccnt, int64_t nsph) {
+    int32_t *__restrict__ ccnt, DevDiag *__restrict__ diag, int64_t nsph) {
   const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kScatterBlock + threadIdx.x;
   if (i >= nsph) return;
   const uint32_t key = skey[i];
@@ -500,6 +534,10 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter(
     return;
   }
   const uint32_t dst = starts[key] + srank[i];
+  if (dst >= (uint64_t)nsph) {  // counts and starts disagree: never expected; avf_step reports it
+    atomicAdd(&diag->bin_broken, 1ull);
+    return;
+  }
   const double *g = stage + (i >> 2) * kStageDoubles;
   const double rad = g[G_RAD];  // binned: not frozen, the pre-freeze radius
   const d3 q = rec_sphere(g, rad / P[inst_of[i >> 2]].sphere_div, (int)(i & 3));
@@ -1713,19 +1751,23 @@ __device__ __forceinline__ void step_params(InstParams &p, long long *acc, int f
 }
 
 // Exclusive scan of the counts into starts, the counts back to zero; starts[ncells] receives
-// the total.  Workgroup 0's first lane also applies step_params to every instance (no other
-// kernel runs meanwhile; this kernel reads no parameter).
+// the total.  bsum: each workgroup's share sums, bstride words apart (k_count_sum: 1, or added
+// up by the count pass: kBsumStride).
+// Workgroup 0's first lane also applies step_params to every instance (no other kernel runs
+// meanwhile; this kernel reads no parameter).  bsum_next (optional): the share sums the next
+// count pass adds into (kBsumStride apart), zeroed here (each workgroup its own).
 __global__ __launch_bounds__(kScanBlock) void k_count_scan(
     uint32_t *__restrict__ counts, const uint32_t *__restrict__ bsum,
     uint32_t *__restrict__ starts, const int64_t *__restrict__ ncells_p,
     InstParams *__restrict__ P, long long *__restrict__ acc, int32_t ninst, int finalize_prev,
-    int advance) {
+    int advance, int bstride, uint32_t *__restrict__ bsum_next) {
   __shared__ uint32_t lds[kScanBlock / 64];
+  if (bsum_next && threadIdx.x == 0) bsum_next[(size_t)blockIdx.x * kBsumStride] = 0u;
   if (blockIdx.x == 0 && threadIdx.x == 0 && (finalize_prev || advance))
     for (int32_t i = 0; i < ninst; i++) step_params(P[i], acc + 8 * i, finalize_prev, advance);
   const int64_t ncells = *ncells_p;
   uint32_t s = 0u;  // sum of the earlier workgroups' counts
-  for (int k = threadIdx.x; k < (int)blockIdx.x; k += kScanBlock) s += bsum[k];
+  for (int k = threadIdx.x; k < (int)blockIdx.x; k += kScanBlock) s += bsum[(size_t)k * bstride];
   uint32_t off;
   block_excl_scan(s, &off, lds);
   int64_t lo, hi;
@@ -2173,14 +2215,20 @@ __global__ __launch_bounds__(256) void k_cscatter(const uint32_t *__restrict__ c
                                                   const uint32_t *__restrict__ crank,
                                                   const uint32_t *__restrict__ starts,
                                                   uint32_t *__restrict__ csid,
-                                                  int32_t *__restrict__ vl_cnt, int64_t n) {
+                                                  int32_t *__restrict__ vl_cnt,
+                                                  DevDiag *__restrict__ diag, int64_t n) {
   const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (c >= n) return;
   const uint32_t key = ckey[c];
-  if (key == kInvalidKey)
+  if (key == kInvalidKey) {
     vl_cnt[c] = 0;
-  else
-    csid[starts[key] + crank[c]] = (uint32_t)c;
+    return;
+  }
+  const uint32_t dst = starts[key] + crank[c];
+  if (dst < (uint64_t)n)
+    csid[dst] = (uint32_t)c;
+  else  // counts and starts disagree: never expected; avf_step reports it
+    atomicAdd(&diag->bin_broken, 1ull);
 }
 
 struct CSweepLds {
@@ -2324,8 +2372,9 @@ __device__ __forceinline__ void csweep_global(const CSweepLds &sh, const float4 
 __global__ __launch_bounds__(kCbBlock) void k_csweep(
     const InstParams *__restrict__ P, int32_t ninst, const int64_t *__restrict__ nbricks_p,
     const uint32_t *__restrict__ starts, const float4 *__restrict__ frec,
-    const uint32_t *__restrict__ csid, int32_t *__restrict__ vl_m, uint16_t *__restrict__ vl_mask, int32_t *__restrict__ vl_cnt,
-    DevDiag *__restrict__ diag, int64_t cap) {
+    const uint32_t *__restrict__ csid, int32_t *__restrict__ vl_m,
+    uint16_t *__restrict__ vl_mask, int32_t *__restrict__ vl_cnt, DevDiag *__restrict__ diag,
+    int64_t cap, int64_t n) {
   __shared__ CSweepLds sh;
   const int tid = threadIdx.x;
   const int64_t nb = *nbricks_p;
@@ -2362,6 +2411,7 @@ __global__ __launch_bounds__(kCbBlock) void k_csweep(
         int x = x0 - 1 + k;
         x = x < 0 ? 0 : (x > nx ? nx : x);
         v = starts[p.cg_base + (int64_t)nx * (y + (int64_t)ny * z) + x];
+        if (v > (uint64_t)n) v = (uint32_t)n;  // within the sorted array whatever happens
       }
       sh.gs[hr][k] = v;
     }
