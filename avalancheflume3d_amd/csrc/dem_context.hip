@@ -541,8 +541,8 @@ void launch_vstep(avf_ctx *c, bool rebuild, hipEvent_t *ev = nullptr, bool ft = 
 // Sphere-cell grid from the current positions, then (optionally) the spatial reorder of the
 // clump arrays by the sphere cell of each clump centre.
 int rebuild_grid(avf_ctx *c, bool reorder) {
-  k_sgrid_reduce<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(c->d_st, c->d_inst,
-                                                                    c->d_acc_sg, c->n, c->cap);
+  k_sgrid_reduce<<<grid_for(c->n, kBlock * kSgItems), kBlock, 0, c->stream>>>(
+      c->d_st, c->d_inst, c->d_acc_sg, c->n, c->cap);
   const double horizon = 2.0 * std::max(1, c->opt.reorder_interval);
   k_sgrid_finalize<<<1, 1, 0, c->stream>>>(c->d_P, c->d_acc_sg, c->ninst, c->cell_cap,
                                            c->opt.sphere_cell_factor, horizon, c->d_ncells,

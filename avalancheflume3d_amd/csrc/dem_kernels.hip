@@ -2201,6 +2201,9 @@ constexpr int kCbBlock = 256;
 constexpr int kCbCap = 512;                   // halo clumps staged in LDS
 constexpr int kCq = 8;                        // candidates queued per lane between the passes
 constexpr int kCbGrid = 4096;                 // workgroups (bricks in strides of kCbGrid)
+constexpr int kCbPer = kCbCap / kCbBlock;     // staged halo clumps per thread, at most
+constexpr int kCbGsPer = (kCbRows * kCbRowStarts + kCbBlock - 1) / kCbBlock;  // cell starts
+static_assert(kCbCap % kCbBlock == 0, "whole halo clumps per thread");
 static_assert(kCbRows <= 64 && kCbOwnRows <= 64, "one wavefront scans the rows");
 static_assert(kCbCap <= 1024 && kCbRows <= 64, "queue entries: 6-bit row, 10-bit index");
 
@@ -2402,18 +2405,26 @@ __global__ __launch_bounds__(kCbBlock) void k_csweep(
     const int x0 = (int)(lb % nbx) * kCbX, y0 = (int)((lb / nbx) % nby) * kCbY;
     const int z0 = (int)(lb / ((int64_t)nbx * nby)) * kCbZ;
     // first clump of cells x0 - 1 .. x0 + kCbX of every halo row, clamped into [0, nx]
-    // (cells outside the grid have no width; rows outside it are empty)
-    for (int t = tid; t < kCbRows * kCbRowStarts; t += kCbBlock) {
-      const int hr = t / kCbRowStarts, k = t % kCbRowStarts;
+    // (cells outside the grid have no width; rows outside it are empty); a thread's loads
+    // are all issued before it stores any
+    uint32_t gv[kCbGsPer];
+#pragma unroll
+    for (int i = 0; i < kCbGsPer; i++) {
+      const int t = tid + i * kCbBlock, hr = t / kCbRowStarts, k = t % kCbRowStarts;
       const int y = y0 - 1 + hr % kCbHY, z = z0 - 1 + hr / kCbHY;
-      uint32_t v = 0u;
-      if (y >= 0 && y < ny && z >= 0 && z < nz) {
+      gv[i] = 0u;
+      if (t < kCbRows * kCbRowStarts && y >= 0 && y < ny && z >= 0 && z < nz) {
         int x = x0 - 1 + k;
         x = x < 0 ? 0 : (x > nx ? nx : x);
-        v = starts[p.cg_base + (int64_t)nx * (y + (int64_t)ny * z) + x];
-        if (v > (uint64_t)n) v = (uint32_t)n;  // within the sorted array whatever happens
+        gv[i] = starts[p.cg_base + (int64_t)nx * (y + (int64_t)ny * z) + x];
       }
-      sh.gs[hr][k] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < kCbGsPer; i++) {
+      const int t = tid + i * kCbBlock;
+      // within the sorted array whatever happens
+      if (t < kCbRows * kCbRowStarts)
+        sh.gs[t / kCbRowStarts][t % kCbRowStarts] = gv[i] > (uint64_t)n ? (uint32_t)n : gv[i];
     }
     __syncthreads();
     if (tid < 64) {  // LDS offsets of the halo rows
@@ -2436,18 +2447,31 @@ __global__ __launch_bounds__(kCbBlock) void k_csweep(
     const uint32_t H = sh.hoff[kCbRows], K = sh.ownp[kCbOwnRows];
     if (K == 0u) continue;
     const bool lds = H <= (uint32_t)kCbCap;
-    if (lds) {  // the halo's records through their storage indices, one clump per thread
+    if (lds) {  // the halo's records through their storage indices (a thread's at most
+                // kCbCap / kCbBlock indices fetched together)
       float ext = 0.0f;
-      for (uint32_t t = tid; t < H; t += kCbBlock) {
-        const int hr = upper_row(sh.hoff, kCbRows, t);
-        const int64_t j = (int64_t)sh.gs[hr][0] + (t - sh.hoff[hr]);
-        const float4 *r = frec + 4 * (int64_t)csid[j];
-        const float4 v0 = r[0], v1 = r[1], v2 = r[2], v3 = r[3];
-        sh.ca[t] = v0;
-        sh.cs[t][0] = v1;
-        sh.cs[t][1] = v2;
-        sh.cs[t][2] = v3;
-        ext = fmaxf(ext, rec_extent(v0, v1, v2, v3));
+      uint32_t m[kCbPer];
+#pragma unroll
+      for (int i = 0; i < kCbPer; i++) {
+        const uint32_t t = tid + i * kCbBlock;
+        m[i] = 0u;
+        if (t < H) {
+          const int hr = upper_row(sh.hoff, kCbRows, t);
+          m[i] = csid[sh.gs[hr][0] + (t - sh.hoff[hr])];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kCbPer; i++) {
+        const uint32_t t = tid + i * kCbBlock;
+        if (t < H) {
+          const float4 *r = frec + 4 * (int64_t)m[i];
+          const float4 v0 = r[0], v1 = r[1], v2 = r[2], v3 = r[3];
+          sh.ca[t] = v0;
+          sh.cs[t][0] = v1;
+          sh.cs[t][1] = v2;
+          sh.cs[t][2] = v3;
+          ext = fmaxf(ext, rec_extent(v0, v1, v2, v3));
+        }
       }
       // the halo's largest coordinate magnitude (NaN coordinates leave it alone; they touch
       // nothing either)
@@ -2502,29 +2526,52 @@ __global__ __launch_bounds__(kCbBlock) void k_csweep(
 
 // AABB of clump centres and max |v| per instance (acc: 8 per instance as in
 // k_cellreduce; slot 6 holds max |v|^2 as an ordered double).  256 threads per block.
+// kSgItems clumps per thread (strided by the block size): a block's clumps belong to its
+// first clump's instance but where a block straddles instances, and those of other
+// instances go straight to their own atomics, so a launch makes a few hundred atomics on
+// each instance's accumulators instead of one per block of 256 clumps (atomics on one cache
+// line are performed one after the other).
+constexpr int kSgItems = 8;
 __global__ __launch_bounds__(256) void k_sgrid_reduce(const double *__restrict__ st,
                                                       const int32_t *__restrict__ inst_of,
                                                       long long *__restrict__ acc, int64_t n,
                                                       int64_t cap) {
-  const int64_t c0 = (int64_t)blockIdx.x * blockDim.x;
-  const int64_t c = c0 + threadIdx.x;
+  const int64_t c0 = (int64_t)blockIdx.x * blockDim.x * kSgItems;
   const int inst0 = inst_of[c0];
-  const int inst = c < n ? inst_of[c] : inst0;
-  long long v[8];
-  const bool active = c < n;
-  if (active) {
+  long long v[8] = {0x7fffffffffffffffll, 0x7fffffffffffffffll, 0x7fffffffffffffffll,
+                    (long long)0x8000000000000000ull, (long long)0x8000000000000000ull,
+                    (long long)0x8000000000000000ull, (long long)0x8000000000000000ull,
+                    0x7fffffffffffffffll};
+  bool active = false;
+  for (int k = 0; k < kSgItems; k++) {
+    const int64_t c = c0 + threadIdx.x + (int64_t)k * blockDim.x;
+    if (c >= n) break;
     const double x = st[S_RX * cap + c], y = st[S_RY * cap + c], z = st[S_RZ * cap + c];
     const double vx = st[S_VX * cap + c], vy = st[S_VY * cap + c], vz = st[S_VZ * cap + c];
-    v[0] = order_key(x);
-    v[1] = order_key(y);
-    v[2] = order_key(z);
-    v[3] = v[0];
-    v[4] = v[1];
-    v[5] = v[2];
-    v[6] = order_key(vx * vx + vy * vy + vz * vz);
-    v[7] = 0x7fffffffffffffffll;
+    const long long kx = order_key(x), ky = order_key(y), kz = order_key(z);
+    const long long kv = order_key(vx * vx + vy * vy + vz * vz);
+    const int inst = inst_of[c];
+    if (inst == inst0) {
+      v[0] = kx < v[0] ? kx : v[0];
+      v[1] = ky < v[1] ? ky : v[1];
+      v[2] = kz < v[2] ? kz : v[2];
+      v[3] = kx > v[3] ? kx : v[3];
+      v[4] = ky > v[4] ? ky : v[4];
+      v[5] = kz > v[5] ? kz : v[5];
+      v[6] = kv > v[6] ? kv : v[6];
+      active = true;
+    } else {
+      long long *a = acc + 8 * inst;
+      atomicMin(&a[0], kx);
+      atomicMin(&a[1], ky);
+      atomicMin(&a[2], kz);
+      atomicMax(&a[3], kx);
+      atomicMax(&a[4], ky);
+      atomicMax(&a[5], kz);
+      atomicMax(&a[6], kv);
+    }
   }
-  block_minmax8(acc + 8 * inst0, inst0, inst, acc + 8 * inst, v, active, 0x78u);
+  block_minmax8(acc + 8 * inst0, inst0, inst0, acc + 8 * inst0, v, active, 0x78u);
 }
 
 // Lays out every instance's sphere grid (cell side h >= 2 * largest sphere radius, so any
