@@ -2301,20 +2301,26 @@ __device__ __forceinline__ void csweep_lds(CSweepLds &sh, const uint32_t *__rest
     }
     nq = 0;
   };
+  // bounding-sphere test of candidate j (LDS record oa) of row nr; queues it when it passes
+  auto test = [&](uint32_t j, float4 oa, int nr) {
+    const float dx = oa.x - qa.x, dy = oa.y - qa.y, dz = oa.z - qa.z;
+    const float bb = __fmaf_rn(qa.w + oa.w, bfac, mh);
+    if (j == self || !(__fmaf_rn(dx, dx, __fmaf_rn(dy, dy, dz * dz)) < bb * bb)) return;
+    if (nq == kCq) drain();
+    sh.q[nq++][tid] = (uint16_t)((nr << 10) | j);
+  };
 #pragma unroll 1
   for (int r9 = 0; r9 < 9; r9++) {
     const int nr = hr + (r9 % 3 - 1) + kCbHY * (r9 / 3 - 1);
     const uint32_t base = sh.hoff[nr] - sh.gs[nr][0];
+    uint32_t j = sh.gs[nr][lx - 1] + base;
     const uint32_t e = sh.gs[nr][lx + 2] + base;
-    for (uint32_t j = sh.gs[nr][lx - 1] + base; j < e; j++) {
-      if (j == self) continue;
-      const float4 oa = sh.ca[j];
-      const float dx = oa.x - qa.x, dy = oa.y - qa.y, dz = oa.z - qa.z;
-      const float bb = __fmaf_rn(qa.w + oa.w, bfac, mh);
-      if (!(__fmaf_rn(dx, dx, __fmaf_rn(dy, dy, dz * dz)) < bb * bb)) continue;
-      if (nq == kCq) drain();
-      sh.q[nq++][tid] = (uint16_t)((nr << 10) | j);
+    for (; j + 1 < e; j += 2) {  // two candidates' records in flight
+      const float4 oa = sh.ca[j], ob = sh.ca[j + 1];
+      test(j, oa, nr);
+      test(j + 1, ob, nr);
     }
+    if (j < e) test(j, sh.ca[j], nr);
   }
   drain();
   if (cnt > kVMax) {
