@@ -3,13 +3,14 @@
     python bench.py [--steps K] [--warmup W] [--workload headline|small|large]
 
 Times K steps of the headline configuration as stated (250,000 tetrahedral 4-sphere clumps,
-FP64, the sphere cells rebuilt every step, spatial reorder every 50 steps) with inputs
-resident in HBM, and prints one JSON line: clump-steps/s, the roofline of the path's
-dominant kernel (from HIP events over the timed region), the step-level roofline (480 B per
-clump-step), and the CPU baseline (the oracle pinned to one host core, a 200-step window on
-a slab of the same workload).  The same workload on candidate lists (identical results,
-fewer rebuilds), the batched ensemble and the adjoint window are reported under their own
-keys.
+FP64, the cell list rebuilt every step, spatial reorder every 50 steps) with inputs resident
+in HBM, and prints one JSON line: clump-steps/s, the roofline of the path's dominant kernel
+(from HIP events over the timed region), the step-level roofline (480 B per clump-step), and
+the CPU baseline (the oracle pinned to one host core, a 200-step window on a slab of the
+same workload).  The cell list is binned by clump (the library default; --binning sphere
+bins the four spheres of every clump instead, with the same results).  The same workload
+binned by sphere, on candidate lists (identical results, fewer rebuilds), the batched
+ensemble and the adjoint window are reported under their own keys.
 """
 from __future__ import annotations
 
@@ -46,25 +47,26 @@ DEFAULT_SKIN = 0.02
 TIMED_EVERY = 8
 # Spatial reorder interval (steps) of the candidate-list comparison runs.
 DEFAULT_REORDER = 400
-# Algorithmic bytes per clump and launch of k_force on a per-step-binning step (the chunk's
-# steps but its last): the clump's stage record (192 B), the angular acceleration (24 B),
+# Per-step binning by sphere (--binning sphere): algorithmic bytes per clump and launch of
+# k_force (a chunk's steps but its last): the clump's stage record (192 B), the angular acceleration (24 B),
 # mass and inertia (32 B), the instance index (4 B), the four per-sphere contact counts
 # (16 B) read; the next stage record (192 B), the angular acceleration (24 B) and the four
 # sphere-cell keys (16 B) written.  The contact entries (4 B each) and the neighbours'
 # records (gathered, cache hits) are not counted; the keys are counted by their own kernel.
 CELLS_FORCE_BYTES = 500.0
-# Algorithmic bytes per clump of the contact sweep: each binned sphere's sorted record
+# Per-step binning by sphere: algorithmic bytes per clump of the contact sweep (k_sweep):
+# each binned sphere's sorted record
 # (x, y, z, r: 32 B) and id (4 B) read once, two cell starts per sphere (the grid holds about
 # two cells per sphere: 8 B), its contact count (4 B) written; contact entries not counted.
 SWEEP_BYTES_PER_CLUMP = 4 * (32.0 + 4.0 + 8.0 + 4.0)
 # Per-step binning by clump (the default): k_force reads the stage record (192 B), the
 # angular acceleration (24 B), mass and inertia (32 B), the instance index and list length
-# (8 B), and writes the next stage record (192 B), the angular acceleration (24 B) and the
-# next clump-cell key (4 B): 476 B; list entries and the neighbours' records not counted.
-# k_csweep reads each binned clump's sorted record (centre, radius, four sphere centres as
-# floats: 64 B) and storage index (4 B) once and two cell starts (8 B), and writes its list
-# length (4 B): 80 B; list entries not counted.
-CLUMP_FORCE_BYTES = 476.0
+# (8 B), and writes the next stage record (192 B), the angular acceleration (24 B), the next
+# clump-cell key (4 B) and the sweep's float record (centre, radius, four sphere centres as
+# floats: 64 B): 540 B; list entries and the neighbours' records not counted.  k_csweep reads
+# each binned clump's float record (64 B) and its index in cell order (4 B) once and two
+# cell starts (8 B), and writes its list length (4 B): 80 B; list entries not counted.
+CLUMP_FORCE_BYTES = 540.0
 CSWEEP_BYTES_PER_CLUMP = 80.0
 KERNEL_NAMES = {"sphere": {"force": "k_force", "detect": "k_sweep"},
                 "clump": {"force": "k_force", "detect": "k_csweep"}}
@@ -311,6 +313,8 @@ def main() -> int:
     ap.add_argument("--binning", default="clump", choices=["clump", "sphere"],
                     help="per-step binning: one key per clump in clump cells (default) or four "
                          "per clump in sphere cells (same results)")
+    ap.add_argument("--compare-steps", type=int, default=2000,
+                    help="steps of the same workload binned the other way (0: skip)")
     ap.add_argument("--reorder", type=int, default=50,
                     help="spatial reorder interval in steps (the headline configuration: 50)")
     ap.add_argument("--list-steps", type=int, default=20000,
@@ -364,7 +368,7 @@ def main() -> int:
     make = {"headline": workloads.headline, "small": workloads.small, "large": workloads.large}
     inst = make[args.workload]()
     n = len(inst.particles)
-    # the stated configuration: sphere cells rebuilt every step, reorder every 50 steps
+    # the stated configuration: the cell list rebuilt every step, reorder every 50 steps
     opt = dem.default_options()
     opt.verlet_skin = 0.0
     opt.reorder_interval = args.reorder
@@ -446,6 +450,10 @@ def main() -> int:
                               n * args.steps / (upload_s + wall + download_s)},
         "diag": diag,
     }
+    if args.compare_steps > 0:
+        other = "sphere" if args.binning == "clump" else "clump"
+        out[f"{other}_binning"] = binning_run(inst, other, args.compare_steps, args.warmup,
+                                              args.reorder)
     if args.list_steps > 0:
         out["candidate_lists"] = list_run(inst, args.list_steps, args.warmup)
     if args.batch_steps > 0:
@@ -462,6 +470,31 @@ def main() -> int:
                 out["batched"]["batched_clump_steps_per_s"] / cb["value"])
     print(json.dumps(out), flush=True)
     return 0
+
+
+def binning_run(inst, binning: str, steps: int, warmup: int, reorder: int) -> dict:
+    """The same workload and steps with the cell list binned the other way (identical
+    results): its throughput and per-kernel breakdown."""
+    from avalancheflume3d_amd import dem
+    n = len(inst.particles)
+    opt = dem.default_options()
+    opt.reorder_interval = reorder
+    opt.step_binning = 0 if binning == "clump" else 1
+    ctx = dem.Context([inst], opt)
+    ctx.regrid()
+    ctx.initial_forces()
+    ctx.step(warmup)
+    kern, _ = ctx.step_profiled(min(steps, 400))
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ctx.close()
+    return {"workload": f"the same {n} clumps, cell list rebuilt every step ({binning} cells), "
+                        f"spatial reorder every {reorder} steps",
+            "clump_steps_per_s": n * steps / wall, "ms_per_step": 1e3 * wall / steps,
+            "steps": steps, "kernel_ms_breakdown": {k: v for k, v in kern.items() if v > 0}}
 
 
 def list_run(inst, steps: int, warmup: int) -> dict:
