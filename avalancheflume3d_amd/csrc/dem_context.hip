@@ -304,14 +304,15 @@ ListTrack list_track(avf_ctx *c) { return ListTrack{c->d_dsp, c->d_wreach}; }
 template <int BS, bool VERLET, bool NEXT, bool LEAN = false, bool CLOSED = false,
           bool DEEP = false, bool CELLS = false>
 void launch_force_bs(avf_ctx *c, int initial, bool ft, int time_ahead, const double *stage_in,
-                     double *stage_out, int si) {
+                     double *stage_out, int si, uint32_t *bsum = nullptr) {
   k_force<kMaxNeigh, BS, VERLET, NEXT, LEAN, CLOSED, DEEP, CELLS>
       <<<grid_for(c->n, BS), BS, 0, c->stream>>>(
       c->d_st, c->d_sta, stage_in, c->d_inst, c->d_P, c->d_clist, c->d_ccnt, c->d_vl_m,
       c->d_vl_mask, c->d_vl_cnt, ft ? c->d_f : nullptr, c->d_diag, c->d_acc_cell, stage_out,
       c->d_orig, (VERLET && !CELLS) ? list_track(c) : ListTrack{nullptr, nullptr}, c->n, c->cap,
       initial, time_ahead, si, c->d_counts, (VERLET && CELLS) ? c->d_ckey : c->d_skey,
-      c->d_srank, (VERLET && CELLS) ? c->d_frec : nullptr);
+      (VERLET && CELLS) ? c->d_crank : c->d_srank, (VERLET && CELLS) ? c->d_frec : nullptr,
+      bsum, (int32_t)scan_grid(c), (VERLET && CELLS) ? c->d_cgrid : c->d_ncells);
 }
 
 // Touching spheres of the binned spheres (skin 0) or candidate spheres within sep + skin.
@@ -377,14 +378,16 @@ void launch_force(avf_ctx *c, int initial, bool ft = true, bool verlet = false,
 // The force stage of per-step binning: own state from the stage record (stage_in), touching
 // spheres from the sweep; with stage_out, the next step's predict into that buffer with its
 // sphere-cell counts and ranks, otherwise the state back to st.
-void launch_cforce(avf_ctx *c, bool ft, const double *stage_in, double *stage_out) {
-  if (c->clump_bins) {  // touching clumps from the clump sweep's lists
+void launch_cforce(avf_ctx *c, bool ft, const double *stage_in, double *stage_out,
+                   uint32_t *bsum_next = nullptr) {
+  if (c->clump_bins) {  // touching clumps from the clump sweep's lists; with a next step, its
+                        // counts, ranks and share sums too
     if (stage_out && c->closed)
       launch_force_bs<kForceBlock, true, true, true, true, false, true>(c, 0, ft, 0, stage_in,
-                                                                        stage_out, 0);
+                                                                        stage_out, 0, bsum_next);
     else if (stage_out)
       launch_force_bs<kForceBlock, true, true, true, false, false, true>(c, 0, ft, 0, stage_in,
-                                                                         stage_out, 0);
+                                                                         stage_out, 0, bsum_next);
     else if (c->closed)
       launch_force_bs<kForceBlock, true, false, true, true, false, true>(c, 0, ft, 0, stage_in,
                                                                          nullptr, 0);
@@ -464,17 +467,17 @@ void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bo
   else
     launch_detect(c);
   mark(AVF_K_FORCE);
-  launch_cforce(c, ft, cur, i + 1 < S ? buf[(i + 1) & 1] : nullptr);
+  launch_cforce(c, ft, cur, i + 1 < S ? buf[(i + 1) & 1] : nullptr, bs_next);
   mark(AVF_K_CELLREDUCE);  // the regrid reduction runs in k_force
   mark(AVF_K_FINALIZE);
   if (i + 1 == S)
     k_finalize<<<grid_for(c->ninst, 64), 64, 0, c->stream>>>(c->d_P, c->d_acc_cell, c->ninst, 0,
                                                              1, 0, 1);
   mark(AVF_K_COUNT);
-  if (i + 1 < S)  // the next step's counts and ranks, in this step's sorted order
-    k_count_ranks<<<std::max(1u, grid_for((cb ? 1 : 4) * c->n, 256)), 256, 0, c->stream>>>(
-        key, sid, c->d_starts, ncells, c->d_counts, rank, bs_next, (int32_t)G,
-        (cb ? 1 : 4) * c->n);
+  if (i + 1 < S && !cb)  // the next step's counts and ranks, in this step's sorted order (by
+                         // clump, the force kernel counts them)
+    k_count_ranks<<<std::max(1u, grid_for(4 * c->n, 256)), 256, 0, c->stream>>>(
+        key, sid, c->d_starts, ncells, c->d_counts, rank, bs_next, (int32_t)G, 4 * c->n);
   mark(AVF_NKERNELS);
 }
 
@@ -833,7 +836,7 @@ int csteps_profiled(avf_ctx *c, int64_t S, StepProf *prof) {
       if (j == AVF_K_SCAN_C || j == AVF_K_CELLREDUCE) continue;
       if ((j == AVF_K_PREDICT || j == AVF_K_SCAN_A) && timed[t] != 0) continue;
       if (j == AVF_K_FINALIZE && timed[t] + 1 != S) continue;
-      if (j == AVF_K_COUNT && timed[t] + 1 == S) continue;
+      if (j == AVF_K_COUNT && (timed[t] + 1 == S || c->clump_bins)) continue;
       const size_t i = (size_t)timed[t];
       float ms = 0.f;
       he = hipEventElapsedTime(&ms, ev[i * kEv + j], ev[i * kEv + j + 1]);

@@ -1507,7 +1507,8 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
     long long *__restrict__ acc, double *__restrict__ stage_out,
     const int32_t *__restrict__ orig, ListTrack lt, int64_t n, int64_t cap, int initial,
     int time_ahead, int si, uint32_t *__restrict__ counts, uint32_t *__restrict__ skey,
-    uint32_t *__restrict__ srank, float4 *__restrict__ frec) {
+    uint32_t *__restrict__ srank, float4 *__restrict__ frec, uint32_t *__restrict__ bsum,
+    int32_t nb, const int64_t *__restrict__ ncells_p) {
   static_assert(!NEXT || VERLET || CELLS, "the fused predict needs the record-resident state");
   static_assert(sizeof(PPShared<MAXN, BS>) >= sizeof(double) * BS * kStagePad,
                 "stage records of the workgroup are assembled in the pair scratch");
@@ -1605,7 +1606,29 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
     // the next step's keys (counted by k_count_ranks): the clump-cell key, or the four
     // sphere-cell keys
     if constexpr (CELLS && VERLET) {
-      if (c < n) skey[c] = keys[0];
+      // the next step's clump key, its count and rank (the count's atomicAdd), and the next
+      // scan's share sums (wave-aggregated, as k_count_ranks adds them)
+      bool todo = false;
+      if (c < n) {
+        skey[c] = keys[0];
+        if (keys[0] != kInvalidKey) {
+          srank[c] = atomicAdd(&counts[keys[0]], 1u);
+          todo = true;
+        }
+      }
+      const int64_t tiles = (*ncells_p + kScanTile - 1) / kScanTile;
+      const int64_t span = ((tiles + nb - 1) / nb) * kScanTile;
+      const uint32_t w = todo ? (uint32_t)((int64_t)keys[0] / span) : 0u;
+      while (true) {
+        const unsigned long long b = __ballot(todo);
+        if (!b) break;
+        const int first = __ffsll((long long)b) - 1;
+        const uint32_t wf = __shfl(w, first, 64);
+        const bool mine = todo && w == wf;
+        const unsigned long long mm = __ballot(mine);
+        if (lane == first) atomicAdd(&bsum[(size_t)wf * kBsumStride], (uint32_t)__popcll(mm));
+        todo = todo && !mine;
+      }
     } else if constexpr (CELLS) {
       if (c < n)
         reinterpret_cast<uint4 *>(skey)[c] = make_uint4(keys[0], keys[1], keys[2], keys[3]);
