@@ -355,7 +355,10 @@ int avf_observe_noalloc(avf_ctx *ctx, int32_t inst, avf_observables *obs,
  * candidate-list steps, the predict after the first step of a chunk (it runs inside the
  * previous step's force kernel), the regrid reduction (inside the force kernel), the
  * finalize before the last step of a chunk, and the next step's cell counts (per-step
- * binning only, after every step of a chunk but its last). */
+ * binning by sphere only, after every step of a chunk but its last; binned by clump, the
+ * force kernel counts the next step's keys itself).  On per-step binning SCAN_A (the count
+ * sums) runs on a chunk's first step only (later steps add the sums up while counting) and
+ * SCAN_C is always empty. */
 enum {
   AVF_K_PREDICT, AVF_K_SCAN_A, AVF_K_SCAN_B, AVF_K_SCAN_C, AVF_K_SCATTER, AVF_K_DETECT,
   AVF_K_FORCE, AVF_K_CELLREDUCE, AVF_K_FINALIZE, AVF_K_COUNT, AVF_NKERNELS
