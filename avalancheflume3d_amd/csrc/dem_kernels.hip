@@ -1789,12 +1789,16 @@ __global__ __launch_bounds__(kScanBlock) void k_count_scan(
   if (blockIdx.x == 0 && threadIdx.x == 0 && (finalize_prev || advance))
     for (int32_t i = 0; i < ninst; i++) step_params(P[i], acc + 8 * i, finalize_prev, advance);
   const int64_t ncells = *ncells_p;
+  int64_t lo, hi;
+  scan_share(ncells, blockIdx.x, gridDim.x, lo, hi);
+  if (lo >= hi) {  // no cells in this share (the grid holds fewer than the launch covers)
+    if (ncells == 0 && blockIdx.x == 0 && threadIdx.x == 0) starts[0] = 0u;
+    return;
+  }
   uint32_t s = 0u;  // sum of the earlier workgroups' counts
   for (int k = threadIdx.x; k < (int)blockIdx.x; k += kScanBlock) s += bsum[(size_t)k * bstride];
   uint32_t off;
   block_excl_scan(s, &off, lds);
-  int64_t lo, hi;
-  scan_share(ncells, blockIdx.x, gridDim.x, lo, hi);
   // tile by tile: wavefront w takes 1024 cells as 16 rows of 64 (row k of lane l is cell
   // seg + 64 k + l: coalesced loads and stores), scans each row across the lanes, and the
   // rows and the 4 wavefronts are chained in order
@@ -1834,8 +1838,7 @@ __global__ __launch_bounds__(kScanBlock) void k_count_scan(
     }
     off += tile;
   }
-  if (threadIdx.x == 0 && ((lo < hi && hi == ncells) || (ncells == 0 && blockIdx.x == 0)))
-    starts[ncells] = off;
+  if (threadIdx.x == 0 && hi == ncells) starts[ncells] = off;
 }
 
 // ------------------------------------------------------------------ contact sweep ---
