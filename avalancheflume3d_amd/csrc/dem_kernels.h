@@ -70,7 +70,7 @@ enum StageSlot { G_RR = 0, G_RAD = 3, G_VV = 4, G_RHO = 7, G_WW = 8, G_ID = 11, 
 
 constexpr uint32_t kInvalidKey = 0xffffffffu;
 constexpr int kScanBlock = 256;
-constexpr int kScanItems = 16;
+constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanBlock * kScanItems;
 
 // Device-side diagnostics (avf_diag).

@@ -1795,23 +1795,30 @@ __global__ __launch_bounds__(kScanBlock) void k_count_scan(
     if (ncells == 0 && blockIdx.x == 0 && threadIdx.x == 0) starts[0] = 0u;
     return;
   }
-  uint32_t s = 0u;  // sum of the earlier workgroups' counts
-  for (int k = threadIdx.x; k < (int)blockIdx.x; k += kScanBlock) s += bsum[(size_t)k * bstride];
-  uint32_t off;
-  block_excl_scan(s, &off, lds);
-  // tile by tile: wavefront w takes 1024 cells as 16 rows of 64 (row k of lane l is cell
-  // seg + 64 k + l: coalesced loads and stores), scans each row across the lanes, and the
-  // rows and the 4 wavefronts are chained in order
+  // tile by tile: wavefront w takes kScanTile / 4 cells as kRows rows of 64 (row k of lane l is
+  // cell seg + 64 k + l: coalesced loads and stores), scans each row across the lanes, and the
+  // rows and the 4 wavefronts are chained in order.  The first tile's counts are loaded
+  // before the earlier shares' sums, so that both loads' latencies overlap.
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  constexpr int kRows = kScanTile / kScanBlock;  // 16
-  for (int64_t base = lo; base < hi; base += kScanTile) {
+  constexpr int kRows = kScanTile / kScanBlock;
+  uint32_t v[kRows];
+  auto load_tile = [&](int64_t base) {
     const int64_t seg = base + (int64_t)wid * 64 * kRows;
-    uint32_t v[kRows], inc[kRows], tot = 0u;
 #pragma unroll
     for (int k = 0; k < kRows; k++) {
       const int64_t i = seg + 64 * k + lane;
       v[k] = i < hi ? counts[i] : 0u;
     }
+  };
+  load_tile(lo);
+  uint32_t s = 0u;  // sum of the earlier workgroups' counts
+  for (int k = threadIdx.x; k < (int)blockIdx.x; k += kScanBlock) s += bsum[(size_t)k * bstride];
+  uint32_t off;
+  block_excl_scan(s, &off, lds);
+  for (int64_t base = lo; base < hi; base += kScanTile) {
+    const int64_t seg = base + (int64_t)wid * 64 * kRows;
+    if (base != lo) load_tile(base);
+    uint32_t inc[kRows], tot = 0u;
 #pragma unroll
     for (int k = 0; k < kRows; k++) {
       inc[k] = wave_incl_scan(v[k]);
