@@ -62,11 +62,12 @@ SWEEP_BYTES_PER_CLUMP = 4 * (32.0 + 4.0 + 8.0 + 4.0)
 # Per-step binning by clump (the default): k_force reads the stage record (192 B), the
 # angular acceleration (24 B), mass and inertia (32 B), the instance index and list length
 # (8 B), and writes the next stage record (192 B), the angular acceleration (24 B), the next
-# clump-cell key (4 B) and the sweep's float record (centre, radius, four sphere centres as
-# floats: 64 B): 540 B; list entries and the neighbours' records not counted.  k_csweep reads
+# clump-cell key and the clump's rank in its cell (8 B) and the sweep's float record (centre,
+# radius, four sphere centres as floats: 64 B): 544 B; list entries, the neighbours' records
+# and the cell counters (atomics in L2) not counted.  k_csweep reads
 # each binned clump's float record (64 B) and its index in cell order (4 B) once and two
 # cell starts (8 B), and writes its list length (4 B): 80 B; list entries not counted.
-CLUMP_FORCE_BYTES = 540.0
+CLUMP_FORCE_BYTES = 544.0
 CSWEEP_BYTES_PER_CLUMP = 80.0
 KERNEL_NAMES = {"sphere": {"force": "k_force", "detect": "k_sweep"},
                 "clump": {"force": "k_force", "detect": "k_csweep"}}
