@@ -36,7 +36,6 @@ struct InstParams {
   double cg_lo[3], cg_inv_h, cg_h, cg_mh, cg_bfac;
   int32_t cg_n[3], pad1;
   int64_t cg_base;
-  int64_t cg_bbase;  // first brick of this instance in the clump sweep's brick range
   // storage range and bookkeeping
   int64_t begin, count;
   int64_t cc;

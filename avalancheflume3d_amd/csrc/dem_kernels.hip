@@ -2213,69 +2213,61 @@ __global__ __launch_bounds__(kSwBlock) void k_sweep(
 //
 // Per-step binning by clump: one key per clump (the clump-cell grid cell of its centre, cell
 // side >= the largest clump's bounding diameter, so the clumps of any touching pair sit in
-// neighbouring cells), counted, scanned and scattered like the sphere keys (a quarter of the
-// keys, a fifth of the cells).  The predict writes each binned clump's float record in
-// storage order (its centre, radius and four sphere centres as floats relative to the grid
-// origin: 64 B); k_cscatter places the storage indices in cell order, through which the
-// sweep reads the records.  k_csweep then lists, for every binned clump, the clumps of its 27
-// neighbour cells that may touch it, with the pairings that may: a float test with a margin
-// far above its rounding error (centre distance against the bounding radii, then the 16
-// sphere pairs), so the listed pairings are a superset of the touching ones.  The force
-// kernel re-tests every listed pairing with force_part.cu:140-157's FP64 test on the stage
-// records' centres (as on candidate-list steps), so the contacts, their order and every bit
-// of the forces are those of the sphere-level binning.
+// neighbouring cells), counted and scanned like the sphere keys (a quarter of the keys, a
+// fifth of the cells).  The predict writes each binned clump's float record in storage order
+// (its centre, radius and four sphere centres as floats relative to the grid origin: 64 B);
+// k_cscatter moves the records into cell order, split in two arrays (centre and radius,
+// 16 B per clump, read by every bounding-sphere test; the sphere centres, 48 B, read for the
+// few candidates that pass it), with each clump's storage index and key beside them.  k_csweep then lists, for every binned clump, the clumps of its 27 neighbour
+// cells that may touch it, with the pairings that may: a float test with a margin far above
+// its rounding error (centre distance against the bounding radii, then the 16 sphere pairs),
+// so the listed pairings are a superset of the touching ones.  The force kernel re-tests
+// every listed pairing with force_part.cu:140-157's FP64 test on the stage records' centres
+// (as on candidate-list steps), so the contacts, their order and every bit of the forces are
+// those of the sphere-level binning.
 
-constexpr int kCbX = 8, kCbY = 8, kCbZ = 4;  // brick of clump cells
-constexpr int kCbHX = kCbX + 2, kCbHY = kCbY + 2, kCbHZ = kCbZ + 2;
-constexpr int kCbRows = kCbHY * kCbHZ;        // halo rows of a brick
-constexpr int kCbOwnRows = kCbY * kCbZ;       // rows of the brick itself
-constexpr int kCbRowStarts = kCbHX + 1;       // first clump of each halo cell, and the end
-constexpr int kCbBlock = 256;
-constexpr int kCbCap = 512;                   // halo clumps staged in LDS
-constexpr int kCq = 8;                        // candidates queued per lane between the passes
-constexpr int kCbGrid = 4096;                 // workgroups (bricks in strides of kCbGrid)
-constexpr int kCbPer = kCbCap / kCbBlock;     // staged halo clumps per thread, at most
-constexpr int kCbGsPer = (kCbRows * kCbRowStarts + kCbBlock - 1) / kCbBlock;  // cell starts
-static_assert(kCbCap % kCbBlock == 0, "whole halo clumps per thread");
-static_assert(kCbRows <= 64 && kCbOwnRows <= 64, "one wavefront scans the rows");
-static_assert(kCbCap <= 1024 && kCbRows <= 64, "queue entries: 6-bit row, 10-bit index");
+constexpr int kCsBlock = 256;
+constexpr int kCq = 8;  // candidates queued per lane between the sweep's two passes
 
-__device__ __forceinline__ int64_t cbricks_of(const InstParams &p) {
-  return (int64_t)((p.cg_n[0] + kCbX - 1) / kCbX) * ((p.cg_n[1] + kCbY - 1) / kCbY) *
-         ((p.cg_n[2] + kCbZ - 1) / kCbZ);
-}
-
-// One lane per binned clump (storage order): its storage index to its cell's start + its rank
-// (from the count).  A clump that is not binned (past side.x) gets an empty list.
+// One lane per quarter of a clump's float record (storage order): the quarter to the clump's
+// place in cell order (its cell's start + its rank; the first quarter to sca, the others to
+// scs), and, from the record's first lane, the clump's storage index and key to the same
+// place.  A clump that is not binned (past side.x)
+// gets an empty list.
 __global__ __launch_bounds__(256) void k_cscatter(const uint32_t *__restrict__ ckey,
                                                   const uint32_t *__restrict__ crank,
                                                   const uint32_t *__restrict__ starts,
+                                                  const float4 *__restrict__ frec,
+                                                  float4 *__restrict__ sca,
+                                                  float4 *__restrict__ scs,
                                                   uint32_t *__restrict__ csid,
+                                                  uint32_t *__restrict__ cskey,
                                                   int32_t *__restrict__ vl_cnt,
                                                   DevDiag *__restrict__ diag, int64_t n) {
-  const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  const int64_t u = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  const int64_t c = u >> 2;
+  const int qq = (int)(u & 3);
   if (c >= n) return;
   const uint32_t key = ckey[c];
+  const float4 v = frec[u];
   if (key == kInvalidKey) {
-    vl_cnt[c] = 0;
+    if (qq == 0) vl_cnt[c] = 0;
     return;
   }
   const uint32_t dst = starts[key] + crank[c];
-  if (dst < (uint64_t)n)
-    csid[dst] = (uint32_t)c;
-  else  // counts and starts disagree: never expected; avf_step reports it
+  if (dst < (uint64_t)n) {
+    if (qq == 0)
+      sca[dst] = v;
+    else
+      scs[3 * (int64_t)dst + qq - 1] = v;
+    if (qq == 0) {
+      csid[dst] = (uint32_t)c;
+      cskey[dst] = key;
+    }
+  } else if (qq == 0) {  // counts and starts disagree: never expected; avf_step reports it
     atomicAdd(&diag->bin_broken, 1ull);
+  }
 }
-
-struct CSweepLds {
-  uint32_t gs[kCbRows][kCbRowStarts];  // global sorted index of each halo cell's first clump
-  uint32_t hoff[kCbRows + 1];          // LDS index of each halo row's first clump
-  uint32_t ownp[kCbOwnRows + 1];       // prefix of the brick rows' clump counts
-  uint32_t xmax;                       // largest coordinate magnitude of the halo (float bits)
-  float4 ca[kCbCap];                   // centre, radius
-  float4 cs[kCbCap][3];                // the four sphere centres
-  uint16_t q[kCq][kCbBlock];           // per lane: queued candidates (row << 10 | index)
-};
 
 // Largest coordinate magnitude of a record (the float test's rounding error scales with it).
 __device__ __forceinline__ float rec_extent(float4 a, float4 b, float4 c, float4 d) {
@@ -2304,260 +2296,126 @@ __device__ __forceinline__ uint32_t cand_mask(const float *qs, const float4 *os,
   return mask;
 }
 
-// The candidate list of one own clump (storage index me, record qa / qs, LDS index self) from
-// the staged halo: the clumps of the 9 neighbour rows' cells lx - 1 .. lx + 1 of halo row hr.
-// Two passes, so that the lanes of a wavefront test sphere pairs together instead of each
-// whenever one of them meets a candidate: the bounding-sphere test queues each candidate
-// that passes (in LDS, kCq per lane; a full queue is emptied on the spot), then the 16
-// sphere pairs of the queued candidates.  Entries go to vl_m / vl_mask (entry-major, as the
-// candidate lists of k_vmerge; the storage index from csid), at most kVMax (the rest are
-// counted in diag.neigh_overflow).  mh covers every record of the halo.
-__device__ __forceinline__ void csweep_lds(CSweepLds &sh, const uint32_t *__restrict__ csid,
-                                           float4 qa, const float *qs, uint32_t self,
-                                           uint32_t me, int hr, int lx, float bfac, float mh,
-                                           int tid, int32_t *__restrict__ vl_m,
-                                           uint16_t *__restrict__ vl_mask,
-                                           int32_t *__restrict__ vl_cnt, int64_t cap,
-                                           DevDiag *__restrict__ diag) {
-  int cnt = 0, nq = 0;
-  auto drain = [&]() {
-    for (int k = 0; k < nq; k++) {
-      const uint32_t v = sh.q[k][tid], j = v & 0x3ffu, row = v >> 10;
-      const float4 os[3] = {sh.cs[j][0], sh.cs[j][1], sh.cs[j][2]};
-      const uint32_t mask = cand_mask(qs, os, qa.w + sh.ca[j].w + mh);
-      if (!mask) continue;
-      if (cnt < kVMax) {
-        vl_m[cnt * cap + me] = (int32_t)csid[sh.gs[row][0] + (j - sh.hoff[row])];
-        vl_mask[cnt * cap + me] = (uint16_t)mask;
-      }
-      cnt++;
-    }
-    nq = 0;
-  };
-  // bounding-sphere test of candidate j (LDS record oa) of row nr; queues it when it passes
-  auto test = [&](uint32_t j, float4 oa, int nr) {
-    const float dx = oa.x - qa.x, dy = oa.y - qa.y, dz = oa.z - qa.z;
-    const float bb = __fmaf_rn(qa.w + oa.w, bfac, mh);
-    if (j == self || !(__fmaf_rn(dx, dx, __fmaf_rn(dy, dy, dz * dz)) < bb * bb)) return;
-    if (nq == kCq) drain();
-    sh.q[nq++][tid] = (uint16_t)((nr << 10) | j);
-  };
-#pragma unroll 1
-  for (int r9 = 0; r9 < 9; r9++) {
-    const int nr = hr + (r9 % 3 - 1) + kCbHY * (r9 / 3 - 1);
-    const uint32_t base = sh.hoff[nr] - sh.gs[nr][0];
-    uint32_t j = sh.gs[nr][lx - 1] + base;
-    const uint32_t e = sh.gs[nr][lx + 2] + base;
-    for (; j + 1 < e; j += 2) {  // two candidates' records in flight
-      const float4 oa = sh.ca[j], ob = sh.ca[j + 1];
-      test(j, oa, nr);
-      test(j + 1, ob, nr);
-    }
-    if (j < e) test(j, sh.ca[j], nr);
-  }
-  drain();
-  if (cnt > kVMax) {
-    atomicAdd(&diag->neigh_overflow, 1ull);
-    cnt = kVMax;
-  }
-  vl_cnt[me] = cnt;
+// The float test's margin for the own clump (largest coordinate magnitude xq) and a
+// candidate with first record quarter oa: above the test's rounding error for the two
+// records.  Every sphere centre of the candidate lies within r (1 + sqrt(3) / div) of its
+// centre, so |centre| + r * bfac bounds the candidate's coordinates.
+__device__ __forceinline__ float pair_margin(float mh0, float xq, float4 oa, float bfac) {
+  const float eo = fmaxf(fmaxf(fabsf(oa.x), fabsf(oa.y)), fabsf(oa.z)) + oa.w * bfac;
+  return fmaxf(mh0, (xq + eo) * 0x1p-21f);
 }
 
-// The same list from the sorted records in global memory (halos too full for LDS): global
-// sorted indices, own index self; mh is widened per pair for the two records' largest
-// coordinate magnitudes (xq: the own clump's).
-__device__ __forceinline__ void csweep_global(const CSweepLds &sh, const float4 *__restrict__ frec,
-                                              const uint32_t *__restrict__ csid, float4 qa,
-                                              const float *qs, float xq, uint32_t self,
-                                              uint32_t me, int hr, int lx, float bfac,
-                                              float mh0, int32_t *__restrict__ vl_m,
-                                              uint16_t *__restrict__ vl_mask,
-                                              int32_t *__restrict__ vl_cnt, int64_t cap,
-                                              DevDiag *__restrict__ diag) {
-  int cnt = 0;
-#pragma unroll 1
-  for (int r9 = 0; r9 < 9; r9++) {
-    const int nr = hr + (r9 % 3 - 1) + kCbHY * (r9 / 3 - 1);
-    const uint32_t e = sh.gs[nr][lx + 2];
-    for (uint32_t j = sh.gs[nr][lx - 1]; j < e; j++) {
-      if (j == self) continue;
-      const uint32_t m = csid[j];
-      const float4 *r = frec + 4 * (int64_t)m;
-      const float4 oa = r[0], os[3] = {r[1], r[2], r[3]};
-      const float mh = fmaxf(mh0, (xq + rec_extent(oa, os[0], os[1], os[2])) * 0x1p-21f);
-      // the bounding spheres (radius r * bfac) first, then the 16 sphere pairs
-      const float dx = oa.x - qa.x, dy = oa.y - qa.y, dz = oa.z - qa.z;
-      const float rs = qa.w + oa.w, bb = __fmaf_rn(rs, bfac, mh);
-      if (!(__fmaf_rn(dx, dx, __fmaf_rn(dy, dy, dz * dz)) < bb * bb)) continue;
-      const uint32_t mask = cand_mask(qs, os, rs + mh);
-      if (!mask) continue;
-      if (cnt < kVMax) {
-        vl_m[cnt * cap + me] = (int32_t)m;
-        vl_mask[cnt * cap + me] = (uint16_t)mask;
-      }
-      cnt++;
-    }
-  }
-  if (cnt > kVMax) {
-    atomicAdd(&diag->neigh_overflow, 1ull);
-    cnt = kVMax;
-  }
-  vl_cnt[me] = cnt;
-}
-
-// The candidate lists of every binned clump, by bricks of kCbX x kCbY x kCbZ clump cells: a
-// workgroup loads the first clump of every cell of the brick's halo rows (each row a
-// contiguous range of the sorted clumps), stages the halo's records in LDS (from global
-// memory in place when more than kCbCap), then each lane takes one of the brick's own clumps
-// and walks the 27 cells around it.  Brick b of all instances' bricks goes to logical
-// workgroup b mod G as in k_sweep.
-__global__ __launch_bounds__(kCbBlock) void k_csweep(
-    const InstParams *__restrict__ P, int32_t ninst, const int64_t *__restrict__ nbricks_p,
-    const uint32_t *__restrict__ starts, const float4 *__restrict__ frec,
-    const uint32_t *__restrict__ csid, int32_t *__restrict__ vl_m,
-    uint16_t *__restrict__ vl_mask, int32_t *__restrict__ vl_cnt, DevDiag *__restrict__ diag,
-    int64_t cap, int64_t n) {
-  __shared__ CSweepLds sh;
+// The candidate lists of every binned clump, one lane per clump in cell order.  A lane reads
+// its clump's record, storage index and key (consecutive lanes: consecutive addresses), the
+// first and last clump of the three cells x - 1 .. x + 1 of each of its 9 neighbour rows
+// (each a contiguous range of the cell order), then walks the 9 ranges reading the records
+// from global memory (lanes of a wavefront sit in the same or neighbouring cells, so they
+// read the same ranges).  Two passes, so that the lanes of a wavefront test sphere pairs
+// together instead of each whenever one of them meets a candidate: the bounding-sphere test
+// queues each candidate that passes (kCq per lane in LDS; a full queue is emptied on the
+// spot), then the 16 sphere pairs of the queued candidates.  Entries go to vl_m / vl_mask
+// (entry-major, as the candidate lists of k_vmerge), at most kVMax (the rest are counted in
+// diag.neigh_overflow).
+__global__ __launch_bounds__(kCsBlock) void k_csweep(
+    const InstParams *__restrict__ P, int32_t ninst, const int64_t *__restrict__ ncells_p,
+    const uint32_t *__restrict__ starts, const float4 *__restrict__ sca,
+    const float4 *__restrict__ scs, const uint32_t *__restrict__ csid,
+    const uint32_t *__restrict__ cskey,
+    int32_t *__restrict__ vl_m, uint16_t *__restrict__ vl_mask, int32_t *__restrict__ vl_cnt,
+    DevDiag *__restrict__ diag, int64_t cap, int64_t n) {
+  __shared__ uint32_t q[kCq][kCsBlock];
+  __shared__ uint32_t rb[9][kCsBlock], re[9][kCsBlock];
   const int tid = threadIdx.x;
-  const int64_t nb = *nbricks_p;
-  const int64_t G = gridDim.x, L = xcd_block_runs(blockIdx.x, gridDim.x, 8);
-#pragma unroll 1
-  for (int64_t b = L; b < nb; b += G) {
-    __syncthreads();  // the previous brick's LDS is no longer read
-    if (tid == 0) sh.xmax = 0u;
-    int inst = 0;
-    if (ninst > 1) {  // last instance whose first brick is <= b
-      int lo = 0, hi = ninst - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (P[mid].cg_bbase <= b)
-          lo = mid;
-        else
-          hi = mid - 1;
-      }
-      inst = lo;
-    }
-    const InstParams &p = P[inst];
-    const int nx = p.cg_n[0], ny = p.cg_n[1], nz = p.cg_n[2];
-    const int64_t lb = b - p.cg_bbase;
-    const int nbx = (nx + kCbX - 1) / kCbX, nby = (ny + kCbY - 1) / kCbY;
-    const int x0 = (int)(lb % nbx) * kCbX, y0 = (int)((lb / nbx) % nby) * kCbY;
-    const int z0 = (int)(lb / ((int64_t)nbx * nby)) * kCbZ;
-    // first clump of cells x0 - 1 .. x0 + kCbX of every halo row, clamped into [0, nx]
-    // (cells outside the grid have no width; rows outside it are empty); a thread's loads
-    // are all issued before it stores any
-    uint32_t gv[kCbGsPer];
-#pragma unroll
-    for (int i = 0; i < kCbGsPer; i++) {
-      const int t = tid + i * kCbBlock, hr = t / kCbRowStarts, k = t % kCbRowStarts;
-      const int y = y0 - 1 + hr % kCbHY, z = z0 - 1 + hr / kCbHY;
-      gv[i] = 0u;
-      if (t < kCbRows * kCbRowStarts && y >= 0 && y < ny && z >= 0 && z < nz) {
-        int x = x0 - 1 + k;
-        x = x < 0 ? 0 : (x > nx ? nx : x);
-        gv[i] = starts[p.cg_base + (int64_t)nx * (y + (int64_t)ny * z) + x];
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < kCbGsPer; i++) {
-      const int t = tid + i * kCbBlock;
-      // within the sorted array whatever happens
-      if (t < kCbRows * kCbRowStarts)
-        sh.gs[t / kCbRowStarts][t % kCbRowStarts] = gv[i] > (uint64_t)n ? (uint32_t)n : gv[i];
-    }
-    __syncthreads();
-    if (tid < 64) {  // LDS offsets of the halo rows
-      const uint32_t len = tid < kCbRows ? sh.gs[tid][kCbRowStarts - 1] - sh.gs[tid][0] : 0u;
-      const uint32_t inc = wave_incl_scan(len);
-      if (tid < kCbRows) sh.hoff[tid + 1] = inc;
-      if (tid == 0) sh.hoff[0] = 0u;
-    } else if (tid < 128) {  // clumps of the brick rows (cells 1 .. kCbX of the halo frame)
-      const int o = tid - 64;
-      uint32_t len = 0u;
-      if (o < kCbOwnRows) {
-        const int hr = (1 + o % kCbY) + kCbHY * (1 + o / kCbY);
-        len = sh.gs[hr][kCbX + 1] - sh.gs[hr][1];
-      }
-      const uint32_t inc = wave_incl_scan(len);
-      if (o < kCbOwnRows) sh.ownp[o + 1] = inc;
-      if (o == 0) sh.ownp[0] = 0u;
-    }
-    __syncthreads();
-    const uint32_t H = sh.hoff[kCbRows], K = sh.ownp[kCbOwnRows];
-    if (K == 0u) continue;
-    const bool lds = H <= (uint32_t)kCbCap;
-    if (lds) {  // the halo's records through their storage indices (a thread's at most
-                // kCbCap / kCbBlock indices fetched together)
-      float ext = 0.0f;
-      uint32_t m[kCbPer];
-#pragma unroll
-      for (int i = 0; i < kCbPer; i++) {
-        const uint32_t t = tid + i * kCbBlock;
-        m[i] = 0u;
-        if (t < H) {
-          const int hr = upper_row(sh.hoff, kCbRows, t);
-          m[i] = csid[sh.gs[hr][0] + (t - sh.hoff[hr])];
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < kCbPer; i++) {
-        const uint32_t t = tid + i * kCbBlock;
-        if (t < H) {
-          const float4 *r = frec + 4 * (int64_t)m[i];
-          const float4 v0 = r[0], v1 = r[1], v2 = r[2], v3 = r[3];
-          sh.ca[t] = v0;
-          sh.cs[t][0] = v1;
-          sh.cs[t][1] = v2;
-          sh.cs[t][2] = v3;
-          ext = fmaxf(ext, rec_extent(v0, v1, v2, v3));
-        }
-      }
-      // the halo's largest coordinate magnitude (NaN coordinates leave it alone; they touch
-      // nothing either)
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) ext = fmaxf(ext, __shfl_xor(ext, off, 64));
-      if ((tid & 63) == 0) atomicMax(&sh.xmax, __float_as_uint(ext));
-      __syncthreads();
-    } else if (tid == 0) {
-      atomicAdd(&diag->sweep_global, 1ull);
-    }
-    const float h = (float)p.cg_h;
-    // the float test's margin: above its rounding error for every record of the halo (for
-    // records swept from global memory, per pair)
-    const float mh = lds ? fmaxf((float)p.cg_mh, (__uint_as_float(sh.xmax) + h) * 0x1p-20f)
-                         : (float)p.cg_mh;
-    const float bfac = (float)p.cg_bfac;
-    for (uint32_t k = tid; k < K; k += kCbBlock) {
-      const int o = upper_row(sh.ownp, kCbOwnRows, k);
-      const int hr = (1 + o % kCbY) + kCbHY * (1 + o / kCbY);
-      const uint32_t jg = sh.gs[hr][1] + (k - sh.ownp[o]);
-      int lx = 1;  // the clump's cell along x in the halo frame (1 .. kCbX)
-#pragma unroll
-      for (int x = 2; x <= kCbX; x++)
-        if (sh.gs[hr][x] <= jg) lx = x;
-      const uint32_t t = sh.hoff[hr] + (jg - sh.gs[hr][0]);
-      float4 qa, q1, q2, q3;
-      const uint32_t me = csid[jg];
-      if (lds) {
-        qa = sh.ca[t];
-        q1 = sh.cs[t][0];
-        q2 = sh.cs[t][1];
-        q3 = sh.cs[t][2];
-      } else {
-        const float4 *r = frec + 4 * (int64_t)me;
-        qa = r[0];
-        q1 = r[1];
-        q2 = r[2];
-        q3 = r[3];
-      }
-      const float qs[12] = {q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
-      if (lds)
-        csweep_lds(sh, csid, qa, qs, t, me, hr, lx, bfac, mh, tid, vl_m, vl_mask, vl_cnt, cap,
-                   diag);
+  const int64_t k = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kCsBlock + tid;
+  const uint32_t sk = starts[*ncells_p];
+  const int64_t nbin = sk < (uint64_t)n ? (int64_t)sk : n;  // clumps binned this step
+  if (k >= nbin) return;
+  const uint32_t key = cskey[k], me = csid[k];
+  const float4 qa = sca[k], q1 = scs[3 * k], q2 = scs[3 * k + 1], q3 = scs[3 * k + 2];
+  int inst = 0;
+  if (ninst > 1) {  // last instance whose first cell is <= key
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (P[mid].cg_base <= (int64_t)key)
+        lo = mid;
       else
-        csweep_global(sh, frec, csid, qa, qs, rec_extent(qa, q1, q2, q3), jg, me, hr, lx, bfac,
-                      mh, vl_m, vl_mask, vl_cnt, cap, diag);
+        hi = mid - 1;
     }
+    inst = lo;
+  }
+  while (true) {  // instance parameters wave-uniform (one pass per instance in the wavefront)
+    const int cur = __builtin_amdgcn_readfirstlane(inst);
+    if (inst != cur) continue;
+    const InstParams &p = P[cur];
+    const int nx = p.cg_n[0], ny = p.cg_n[1], nz = p.cg_n[2];
+    const int64_t local = (int64_t)key - p.cg_base, yz = local / nx;
+    const bool inside = local >= 0 && local < (int64_t)nx * ny * nz;
+    const int x = (int)(local - yz * nx), y = (int)(yz % ny), z = (int)(yz / ny);
+    const int xl = x > 0 ? x - 1 : 0, xh = x + 2 < nx ? x + 2 : nx;
+    // the 9 ranges (cells outside the grid are empty), all loaded together
+    uint32_t b9[9], e9[9];
+#pragma unroll
+    for (int r9 = 0; r9 < 9; r9++) {
+      const int yy = y + r9 % 3 - 1, zz = z + r9 / 3 - 1;
+      b9[r9] = e9[r9] = 0u;
+      if (inside && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
+        const int64_t row = p.cg_base + (int64_t)nx * (yy + (int64_t)ny * zz);
+        b9[r9] = starts[row + xl];
+        e9[r9] = starts[row + xh];
+      }
+    }
+#pragma unroll
+    for (int r9 = 0; r9 < 9; r9++) {  // within the sorted clumps whatever happens
+      rb[r9][tid] = b9[r9] < (uint64_t)nbin ? b9[r9] : (uint32_t)nbin;
+      re[r9][tid] = e9[r9] < (uint64_t)nbin ? e9[r9] : (uint32_t)nbin;
+    }
+    const float qs[12] = {q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+    const float xq = rec_extent(qa, q1, q2, q3);
+    const float mh0 = (float)p.cg_mh, bfac = (float)p.cg_bfac;
+    int cnt = 0, nq = 0;
+    auto drain = [&]() {
+      for (int t = 0; t < nq; t++) {
+        const uint32_t j = q[t][tid];
+        const float4 *r = scs + 3 * (int64_t)j;
+        const float4 oa = sca[j], os[3] = {r[0], r[1], r[2]};
+        const uint32_t mask =
+            cand_mask(qs, os, qa.w + oa.w + pair_margin(mh0, xq, oa, bfac));
+        if (!mask) continue;
+        if (cnt < kVMax) {
+          vl_m[cnt * cap + me] = (int32_t)csid[j];
+          vl_mask[cnt * cap + me] = (uint16_t)mask;
+        }
+        cnt++;
+      }
+      nq = 0;
+    };
+    // bounding-sphere test of candidate j (first record quarter oa); queues it when it passes
+    auto test = [&](uint32_t j, float4 oa) {
+      const float dx = oa.x - qa.x, dy = oa.y - qa.y, dz = oa.z - qa.z;
+      const float bb = __fmaf_rn(qa.w + oa.w, bfac, pair_margin(mh0, xq, oa, bfac));
+      if (j == (uint32_t)k || !(__fmaf_rn(dx, dx, __fmaf_rn(dy, dy, dz * dz)) < bb * bb)) return;
+      if (nq == kCq) drain();
+      q[nq++][tid] = j;
+    };
+#pragma unroll 1
+    for (int r9 = 0; r9 < 9; r9++) {
+      uint32_t j = rb[r9][tid];
+      const uint32_t e = re[r9][tid];
+      for (; j + 1 < e; j += 2) {  // two candidates' records in flight
+        const float4 oa = sca[j], ob = sca[j + 1];
+        test(j, oa);
+        test(j + 1, ob);
+      }
+      if (j < e) test(j, sca[j]);
+    }
+    drain();
+    if (cnt > kVMax) {
+      atomicAdd(&diag->neigh_overflow, 1ull);
+      cnt = kVMax;
+    }
+    vl_cnt[me] = cnt;
+    break;
   }
 }
 
@@ -2716,12 +2574,6 @@ __global__ void k_sgrid_finalize(InstParams *__restrict__ P, long long *__restri
       }
       cg_out[0] = ninst;
     }
-    int64_t cb = 0;
-    for (int i = 0; i < ninst; i++) {
-      P[i].cg_bbase = cb;
-      cb += cbricks_of(P[i]);
-    }
-    cg_out[1] = cb;
   }
   for (int i = 0; i < ninst; i++) {
     long long *a = acc + 8 * i;

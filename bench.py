@@ -64,11 +64,12 @@ SWEEP_BYTES_PER_CLUMP = 4 * (32.0 + 4.0 + 8.0 + 4.0)
 # (8 B), and writes the next stage record (192 B), the angular acceleration (24 B), the next
 # clump-cell key and the clump's rank in its cell (8 B) and the sweep's float record (centre,
 # radius, four sphere centres as floats: 64 B): 544 B; list entries, the neighbours' records
-# and the cell counters (atomics in L2) not counted.  k_csweep reads
-# each binned clump's float record (64 B) and its index in cell order (4 B) once and two
-# cell starts (8 B), and writes its list length (4 B): 80 B; list entries not counted.
+# and the cell counters (atomics in L2) not counted.  k_csweep reads each binned clump's
+# float record in cell order (64 B), its storage index and key (8 B) and two cell starts
+# (8 B), and writes its list length (4 B): 84 B; the candidates' records (read again by
+# neighbouring clumps, from cache) and the list entries not counted.
 CLUMP_FORCE_BYTES = 544.0
-CSWEEP_BYTES_PER_CLUMP = 80.0
+CSWEEP_BYTES_PER_CLUMP = 84.0
 KERNEL_NAMES = {"sphere": {"force": "k_force", "detect": "k_sweep"},
                 "clump": {"force": "k_force", "detect": "k_csweep"}}
 
