@@ -68,8 +68,7 @@ struct avf_ctx {
   bool clump_bins = false;
   uint32_t *d_ckey = nullptr, *d_crank = nullptr, *d_csid = nullptr;
   float4 *d_frec = nullptr;  // float records of the clump sweep, in storage order
-  float4 *d_sca = nullptr, *d_scs = nullptr;  // the same records in cell order (centre and
-                                              // radius; the sphere centres)
+  float4 *d_sca = nullptr;      // their centres and radii in cell order
   uint32_t *d_cskey = nullptr;  // clump keys in cell order
   int64_t *d_cgrid = nullptr;  // [0] clump cells over all instances
   // candidate lists (verlet_skin > 0): per clump up to kVMax neighbour clumps with their
@@ -457,15 +456,15 @@ void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bo
   mark(AVF_K_SCAN_C);
   mark(AVF_K_SCATTER);
   if (cb)
-    k_cscatter<<<std::max(1u, grid_for(4 * c->n, 256)), 256, 0, c->stream>>>(
-        c->d_ckey, c->d_crank, c->d_starts, c->d_frec, c->d_sca, c->d_scs, c->d_csid, c->d_cskey,
+    k_cscatter<<<std::max(1u, grid_for(c->n, 256)), 256, 0, c->stream>>>(
+        c->d_ckey, c->d_crank, c->d_starts, c->d_frec, c->d_sca, c->d_csid, c->d_cskey,
         c->d_vl_cnt, c->d_diag, c->n);
   else
     launch_scatter(c, c->d_skey, c->d_srank, c->d_starts, cur, c->d_spos, c->d_sid, c->d_ccnt);
   mark(AVF_K_DETECT);
   if (cb)
     k_csweep<<<std::max(1u, grid_for(c->n, kCsBlock)), kCsBlock, 0, c->stream>>>(
-        c->d_P, c->ninst, c->d_cgrid, c->d_starts, c->d_sca, c->d_scs, c->d_csid, c->d_cskey,
+        c->d_P, c->ninst, c->d_cgrid, c->d_starts, c->d_sca, c->d_frec, c->d_csid, c->d_cskey,
         c->d_vl_m, c->d_vl_mask, c->d_vl_cnt, c->d_diag, c->cap, c->n);
   else
     launch_detect(c);
@@ -1036,8 +1035,7 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
   if (c->clump_bins &&
       ((rc = dalloc(c, &c->d_ckey, N)) || (rc = dalloc(c, &c->d_crank, N)) ||
        (rc = dalloc(c, &c->d_csid, N)) || (rc = dalloc(c, &c->d_frec, 4 * N)) ||
-       (rc = dalloc(c, &c->d_sca, N)) || (rc = dalloc(c, &c->d_scs, 3 * N)) ||
-       (rc = dalloc(c, &c->d_cskey, N)) ||
+       (rc = dalloc(c, &c->d_sca, N)) || (rc = dalloc(c, &c->d_cskey, N)) ||
        (rc = dalloc(c, &c->d_cgrid, 2))))
     return bail(rc);
   if (c->opt.verlet_skin > 0.0 &&
@@ -1156,7 +1154,7 @@ int avf_destroy(avf_ctx *c) {
                   c->d_vl_m,   c->d_vl_cnt, c->d_vl_mask, c->d_dsp, c->d_wreach,
                   c->d_ck_st,  c->d_ck_rad, c->d_ck_P, c->d_ck_diag, c->d_umax,
                   c->d_stage2, c->d_ckey, c->d_crank, c->d_csid, c->d_frec,
-                  c->d_sca,    c->d_scs,   c->d_cskey, c->d_cgrid};
+                  c->d_sca,    c->d_cskey, c->d_cgrid};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);

@@ -2216,9 +2216,9 @@ __global__ __launch_bounds__(kSwBlock) void k_sweep(
 // neighbouring cells), counted and scanned like the sphere keys (a quarter of the keys, a
 // fifth of the cells).  The predict writes each binned clump's float record in storage order
 // (its centre, radius and four sphere centres as floats relative to the grid origin: 64 B);
-// k_cscatter moves the records into cell order, split in two arrays (centre and radius,
-// 16 B per clump, read by every bounding-sphere test; the sphere centres, 48 B, read for the
-// few candidates that pass it), with each clump's storage index and key beside them.  k_csweep then lists, for every binned clump, the clumps of its 27 neighbour
+// k_cscatter moves each record's centre and radius (16 B, read by every bounding-sphere test)
+// into cell order, with the clump's storage index and key beside them; the sphere centres
+// (48 B, read for the few candidates that pass that test) stay in storage order.  k_csweep then lists, for every binned clump, the clumps of its 27 neighbour
 // cells that may touch it, with the pairings that may: a float test with a margin far above
 // its rounding error (centre distance against the bounding radii, then the 16 sphere pairs),
 // so the listed pairings are a superset of the touching ones.  The force kernel re-tests
@@ -2229,42 +2229,32 @@ __global__ __launch_bounds__(kSwBlock) void k_sweep(
 constexpr int kCsBlock = 256;
 constexpr int kCq = 8;  // candidates queued per lane between the sweep's two passes
 
-// One lane per quarter of a clump's float record (storage order): the quarter to the clump's
-// place in cell order (its cell's start + its rank; the first quarter to sca, the others to
-// scs), and, from the record's first lane, the clump's storage index and key to the same
-// place.  A clump that is not binned (past side.x)
-// gets an empty list.
+// One lane per clump (storage order): the centre and radius of its float record, its storage
+// index and its key to its place in cell order (its cell's start + its rank).  A clump that
+// is not binned (past side.x) gets an empty list.
 __global__ __launch_bounds__(256) void k_cscatter(const uint32_t *__restrict__ ckey,
                                                   const uint32_t *__restrict__ crank,
                                                   const uint32_t *__restrict__ starts,
                                                   const float4 *__restrict__ frec,
                                                   float4 *__restrict__ sca,
-                                                  float4 *__restrict__ scs,
                                                   uint32_t *__restrict__ csid,
                                                   uint32_t *__restrict__ cskey,
                                                   int32_t *__restrict__ vl_cnt,
                                                   DevDiag *__restrict__ diag, int64_t n) {
-  const int64_t u = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-  const int64_t c = u >> 2;
-  const int qq = (int)(u & 3);
+  const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (c >= n) return;
-  const uint32_t key = ckey[c];
-  const float4 v = frec[u];
+  const uint32_t key = ckey[c], rank = crank[c];
+  const float4 v = frec[4 * c];
   if (key == kInvalidKey) {
-    if (qq == 0) vl_cnt[c] = 0;
+    vl_cnt[c] = 0;
     return;
   }
-  const uint32_t dst = starts[key] + crank[c];
+  const uint32_t dst = starts[key] + rank;
   if (dst < (uint64_t)n) {
-    if (qq == 0)
-      sca[dst] = v;
-    else
-      scs[3 * (int64_t)dst + qq - 1] = v;
-    if (qq == 0) {
-      csid[dst] = (uint32_t)c;
-      cskey[dst] = key;
-    }
-  } else if (qq == 0) {  // counts and starts disagree: never expected; avf_step reports it
+    sca[dst] = v;
+    csid[dst] = (uint32_t)c;
+    cskey[dst] = key;
+  } else {  // counts and starts disagree: never expected; avf_step reports it
     atomicAdd(&diag->bin_broken, 1ull);
   }
 }
@@ -2319,7 +2309,7 @@ __device__ __forceinline__ float pair_margin(float mh0, float xq, float4 oa, flo
 __global__ __launch_bounds__(kCsBlock) void k_csweep(
     const InstParams *__restrict__ P, int32_t ninst, const int64_t *__restrict__ ncells_p,
     const uint32_t *__restrict__ starts, const float4 *__restrict__ sca,
-    const float4 *__restrict__ scs, const uint32_t *__restrict__ csid,
+    const float4 *__restrict__ frec, const uint32_t *__restrict__ csid,
     const uint32_t *__restrict__ cskey,
     int32_t *__restrict__ vl_m, uint16_t *__restrict__ vl_mask, int32_t *__restrict__ vl_cnt,
     DevDiag *__restrict__ diag, int64_t cap, int64_t n) {
@@ -2331,7 +2321,9 @@ __global__ __launch_bounds__(kCsBlock) void k_csweep(
   const int64_t nbin = sk < (uint64_t)n ? (int64_t)sk : n;  // clumps binned this step
   if (k >= nbin) return;
   const uint32_t key = cskey[k], me = csid[k];
-  const float4 qa = sca[k], q1 = scs[3 * k], q2 = scs[3 * k + 1], q3 = scs[3 * k + 2];
+  const float4 qa = sca[k];
+  const float4 *qr = frec + 4 * (int64_t)me;
+  const float4 q1 = qr[1], q2 = qr[2], q3 = qr[3];
   int inst = 0;
   if (ninst > 1) {  // last instance whose first cell is <= key
     int lo = 0, hi = ninst - 1;
@@ -2376,14 +2368,14 @@ __global__ __launch_bounds__(kCsBlock) void k_csweep(
     int cnt = 0, nq = 0;
     auto drain = [&]() {
       for (int t = 0; t < nq; t++) {
-        const uint32_t j = q[t][tid];
-        const float4 *r = scs + 3 * (int64_t)j;
-        const float4 oa = sca[j], os[3] = {r[0], r[1], r[2]};
+        const uint32_t j = q[t][tid], m = csid[j];
+        const float4 *r = frec + 4 * (int64_t)m;
+        const float4 oa = sca[j], os[3] = {r[1], r[2], r[3]};
         const uint32_t mask =
             cand_mask(qs, os, qa.w + oa.w + pair_margin(mh0, xq, oa, bfac));
         if (!mask) continue;
         if (cnt < kVMax) {
-          vl_m[cnt * cap + me] = (int32_t)csid[j];
+          vl_m[cnt * cap + me] = (int32_t)m;
           vl_mask[cnt * cap + me] = (uint16_t)mask;
         }
         cnt++;

@@ -65,9 +65,10 @@ SWEEP_BYTES_PER_CLUMP = 4 * (32.0 + 4.0 + 8.0 + 4.0)
 # clump-cell key and the clump's rank in its cell (8 B) and the sweep's float record (centre,
 # radius, four sphere centres as floats: 64 B): 544 B; list entries, the neighbours' records
 # and the cell counters (atomics in L2) not counted.  k_csweep reads each binned clump's
-# float record in cell order (64 B), its storage index and key (8 B) and two cell starts
-# (8 B), and writes its list length (4 B): 84 B; the candidates' records (read again by
-# neighbouring clumps, from cache) and the list entries not counted.
+# centre and radius in cell order (16 B), its storage index and key (8 B), its sphere
+# centres (48 B) and two cell starts (8 B), and writes its list length (4 B): 84 B; the
+# candidates' records (read again by neighbouring clumps, from cache) and the list entries
+# not counted.
 CLUMP_FORCE_BYTES = 544.0
 CSWEEP_BYTES_PER_CLUMP = 84.0
 KERNEL_NAMES = {"sphere": {"force": "k_force", "detect": "k_sweep"},
