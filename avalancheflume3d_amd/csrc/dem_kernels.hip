@@ -32,14 +32,29 @@ __device__ __forceinline__ uint32_t sphere_key(const InstParams &p, d3 q) {
   return (uint32_t)(p.sg_base + ix + (int64_t)p.sg_n[0] * (iy + (int64_t)p.sg_n[1] * iz));
 }
 
+// The clump-cell grid of an instance, in registers: a caller loads it before the work that
+// precedes the key (its loads then overlap that work).
+struct ClumpGrid {
+  double lo[3], inv_h, side_x, div;
+  int32_t n[3];
+  int64_t base;
+};
+__device__ __forceinline__ ClumpGrid clump_grid(const InstParams &p) {
+  return ClumpGrid{{p.cg_lo[0], p.cg_lo[1], p.cg_lo[2]}, p.cg_inv_h, p.side_x, p.sphere_div,
+                   {p.cg_n[0], p.cg_n[1], p.cg_n[2]}, p.cg_base};
+}
+
 // Clump-cell key of a clump centre for per-step binning by clump (kInvalidKey for a clump
 // past side.x at t_new, which is not binned: cells.cu:65).
+__device__ __forceinline__ uint32_t clump_key(const ClumpGrid &g, d3 rr, double t_new) {
+  if (frozen(t_new, rr.x, g.side_x)) return kInvalidKey;  // not binned (cells.cu:65)
+  const int ix = grid_index(rr.x, g.lo[0], g.inv_h, g.n[0]);
+  const int iy = grid_index(rr.y, g.lo[1], g.inv_h, g.n[1]);
+  const int iz = grid_index(rr.z, g.lo[2], g.inv_h, g.n[2]);
+  return (uint32_t)(g.base + ix + (int64_t)g.n[0] * (iy + (int64_t)g.n[1] * iz));
+}
 __device__ __forceinline__ uint32_t clump_key(const InstParams &p, d3 rr, double t_new) {
-  if (frozen(t_new, rr.x, p.side_x)) return kInvalidKey;  // not binned (cells.cu:65)
-  const int ix = grid_index(rr.x, p.cg_lo[0], p.cg_inv_h, p.cg_n[0]);
-  const int iy = grid_index(rr.y, p.cg_lo[1], p.cg_inv_h, p.cg_n[1]);
-  const int iz = grid_index(rr.z, p.cg_lo[2], p.cg_inv_h, p.cg_n[2]);
-  return (uint32_t)(p.cg_base + ix + (int64_t)p.cg_n[0] * (iy + (int64_t)p.cg_n[1] * iz));
+  return clump_key(clump_grid(p), rr, t_new);
 }
 
 // Sphere centres rr + (rad/div) R (+-1,+-1,+-1), init_Verlet.cu:130-142.
@@ -69,7 +84,7 @@ __device__ __forceinline__ d3 sphere_of(d3 rr, const double *R, double pos, int 
 // radius), then the four sphere centres (sphere_centres on the record's centre and rotation:
 // the predict's FP64 centres) packed in three float4, coordinates relative to the
 // instance's clump-grid origin.  The radius is the record's (a binned clump is not frozen).
-__device__ __forceinline__ void clump_frec(const double *g, const InstParams &p,
+__device__ __forceinline__ void clump_frec(const double *g, const ClumpGrid &p,
                                            float4 *__restrict__ out) {
   const d3 rr = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
   const double rad = g[G_RAD];
@@ -77,8 +92,8 @@ __device__ __forceinline__ void clump_frec(const double *g, const InstParams &p,
 #pragma unroll
   for (int k = 0; k < 9; k++) R[k] = g[G_R + k];
   d3 s[4];
-  sphere_centres(rr, R, rad / p.sphere_div, s);
-  const double ox = p.cg_lo[0], oy = p.cg_lo[1], oz = p.cg_lo[2];
+  sphere_centres(rr, R, rad / p.div, s);
+  const double ox = p.lo[0], oy = p.lo[1], oz = p.lo[2];
   float f[15];
   f[0] = (float)(rr.x - ox);
   f[1] = (float)(rr.y - oy);
@@ -379,7 +394,7 @@ __global__ __launch_bounds__(kPredictBlock) void k_predict(
   }
   __syncthreads();
   if (frec && c < n && keys[0] != kInvalidKey)  // the clump sweep's record (clump keys)
-    clump_frec(rec + threadIdx.x * kStagePad, P[inst_of[c]], frec + 4 * c);
+    clump_frec(rec + threadIdx.x * kStagePad, clump_grid(P[inst_of[c]]), frec + 4 * c);
   const int64_t rows = n - c0 < kPredictBlock ? n - c0 : kPredictBlock;
   double *dst = stage + c0 * kStageDoubles;
   for (int64_t k = threadIdx.x; k < rows * kStageDoubles; k += kPredictBlock)
@@ -1098,6 +1113,11 @@ __device__ __forceinline__ void force_clump(
   // the rounding of the centres, for the boundary modules' early outs
   const double rb = 1.7320508075688772 * pos0 * (1.0 + 1e-12) + 1e-9;
 
+  // the candidate list's length and first entry, loaded with the own record (the entry is
+  // read whatever the length, so the loads do not wait for each other)
+  const int kc0 = VERLET ? vl_cnt[c] : 0;
+  const int32_t m00 = VERLET ? vl_m[c] : 0;
+  const uint16_t b00 = VERLET ? vl_mask[c] : (uint16_t)0;
   const bool frz = frozen(time, rr.x, p.side_x);
   // the initial force pass (avalanche_hst.cu:365-384) runs force_part and then force_hop in
   // either phase, as the filling step does; the other boundary modules join on the steps
@@ -1178,7 +1198,7 @@ __device__ __forceinline__ void force_clump(
       cnt++;
     };
     if constexpr (VERLET) {
-      const int kc = vl_cnt[c];
+      const int kc = kc0;
       // one candidate-list entry: the touching test of k_sweep on the same operands for each
       // candidate pairing, then its place
       auto visit = [&](int64_t cm, uint32_t bits, d3 rm, double radm, double idm,
@@ -1207,8 +1227,8 @@ __device__ __forceinline__ void force_clump(
 #pragma unroll
         for (int k = 0; k < 9; k++) Rm[k] = g[G_R + k];
       };
-      int64_t cm_next = kc > 0 ? vl_m[c] : 0;
-      uint32_t bits_next = kc > 0 ? vl_mask[c] : 0u;
+      int64_t cm_next = kc > 0 ? m00 : 0;
+      uint32_t bits_next = kc > 0 ? b00 : 0u;
       if constexpr (!DEEP) {
         // the next entry is fetched one iteration ahead and a candidate's centre, radius, id
         // and sphere centres are loaded together, so an entry waits for one gather
@@ -1580,6 +1600,9 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
     __syncthreads();  // the pair scratch is free: it now holds the workgroup's records
     double *rec = reinterpret_cast<double *>(&sh);
     uint32_t keys[4] = {kInvalidKey, kInvalidKey, kInvalidKey, kInvalidKey};
+    uint32_t rank = 0u;   // the clump's rank in its cell, stored last (the atomic's latency
+    bool ranked = false;  // overlaps the record writes)
+    ClumpGrid cg{};       // loaded before the predict, which then hides its latency
     if (c < n) {
       while (true) {
         const int cur = __builtin_amdgcn_readfirstlane(inst);
@@ -1588,6 +1611,7 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
           // the next step starts at this step's force-stage time (P.time on per-step binning,
           // the chunk's step time + dt on candidate-list steps)
           const double t_old = CELLS ? q.time : chunk_time(q, si) + q.dt;
+          cg = clump_grid(q);
           const double rad0 = own.rad != 0.0 ? own.rad : sta[C_RAD0 * cap + c];
           if constexpr (LEAN)
             predict_regs(cr, sta, rec + lane * kStagePad, q, own.id, own.rad, rad0, own.rho, c,
@@ -1596,7 +1620,7 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
             predict_clump(st, sta, rec + lane * kStagePad, q, orig, nullptr, c, cap, 2, lt,
                           t_old);
           if constexpr (CELLS && VERLET)
-            keys[0] = clump_key(q, cr.rr, t_old + q.dt);
+            keys[0] = clump_key(cg, cr.rr, t_old + q.dt);
           else if constexpr (CELLS)
             clump_keys(q, cr.rr, cr.R, rad0, t_old + q.dt, keys);
           break;
@@ -1612,10 +1636,11 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
       if (c < n) {
         skey[c] = keys[0];
         if (keys[0] != kInvalidKey) {
-          srank[c] = atomicAdd(&counts[keys[0]], 1u);
+          rank = atomicAdd(&counts[keys[0]], 1u);
           todo = true;
         }
       }
+      ranked = todo;
       const int64_t tiles = (*ncells_p + kScanTile - 1) / kScanTile;
       const int64_t span = ((tiles + nb - 1) / nb) * kScanTile;
       const uint32_t w = todo ? (uint32_t)((int64_t)keys[0] / span) : 0u;
@@ -1635,11 +1660,12 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
     }
     __syncthreads();
     if constexpr (CELLS && VERLET)  // the clump sweep's float record of the next step
-      if (c < n && keys[0] != kInvalidKey) clump_frec(rec + lane * kStagePad, P[inst], frec + 4 * c);
+      if (c < n && keys[0] != kInvalidKey) clump_frec(rec + lane * kStagePad, cg, frec + 4 * c);
     const int64_t rows = n - c0 < BS ? n - c0 : BS;
     double *dst = stage_out + c0 * kStageDoubles;
     for (int64_t k = lane; k < rows * kStageDoubles; k += BS)
       dst[k] = rec[(k / kStageDoubles) * kStagePad + k % kStageDoubles];
+    if (ranked) srank[c] = rank;
   }
 }
 
