@@ -104,6 +104,7 @@ struct avf_ctx {
   bool binned = false;   // sorted sphere arrays describe the current positions
   bool stepped = false;  // st holds the state after steps (the exports may bin it)
   int64_t steps_since_reorder = 0;
+  int64_t buf_set = 0;  // which of the two state buffer sets is current (swapped by reorders)
   int64_t steps_total = 0;
   int64_t list_builds = 0, steps_undone = 0;  // avf_diag
   std::map<int64_t, hipGraphExec_t> graphs;
@@ -560,10 +561,12 @@ int rebuild_grid(avf_ctx *c, bool reorder) {
     k_reorder_move<<<grid_for(c->n, kBlock), kBlock, 0, c->stream>>>(
         c->d_skey, c->d_srank, c->d_starts, c->d_st, c->d_st2, c->d_sta, c->d_sta2, c->d_orig,
         c->d_orig2, c->d_inst, c->d_inst2, c->d_P, c->d_inv, c->n, c->cap);
-    copy_set(c, {{{c->d_st2, c->d_st, (int64_t)c->cap * S_NFIELDS * 2},
-                  {c->d_sta2, c->d_sta, (int64_t)c->cap * C_NFIELDS * 2},
-                  {c->d_orig2, c->d_orig, c->cap},
-                  {c->d_inst2, c->d_inst, c->cap}}});
+    // the moved arrays become the current ones (captured graphs are kept per buffer set)
+    std::swap(c->d_st, c->d_st2);
+    std::swap(c->d_sta, c->d_sta2);
+    std::swap(c->d_orig, c->d_orig2);
+    std::swap(c->d_inst, c->d_inst2);
+    c->buf_set ^= 1;
   }
   HIPCHK(c, hipGetLastError());
   c->sgrid_valid = true;
@@ -755,8 +758,9 @@ int run_verlet(avf_ctx *c, int64_t nsteps, StepProf *prof) {
       // cached by their first step alone.  Plain chunks use positive keys.
       for (int64_t off = 0; off < S; off += kGraphSteps) {
         const int64_t len = std::min(kGraphSteps, S - off);
-        const int64_t key = off + len < S ? -(int64_t{1} << 40) - off
-                                          : -(S * (kMaxChunk + 1) + off + 1);
+        const int64_t key = (off + len < S ? -(int64_t{1} << 40) - off
+                                           : -(S * (kMaxChunk + 1) + off + 1)) ^
+                            (c->buf_set << 56);
         auto it = c->graphs.find(key);
         if (it == c->graphs.end()) {
           hipGraph_t g;
@@ -869,7 +873,7 @@ int run_cells(avf_ctx *c, int64_t nsteps, StepProf *prof) {
     } else if (c->opt.use_graph) {
       for (int64_t off = 0; off < S; off += kGraphSteps) {
         const int64_t len = std::min(kGraphSteps, S - off);
-        const int64_t key = (int64_t{1} << 50) + S * 65536 + off;
+        const int64_t key = ((int64_t{1} << 50) + S * 65536 + off) ^ (c->buf_set << 56);
         auto it = c->graphs.find(key);
         if (it == c->graphs.end()) {
           hipGraph_t g;
