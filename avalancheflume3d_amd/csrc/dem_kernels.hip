@@ -2285,15 +2285,6 @@ __global__ __launch_bounds__(256) void k_cscatter(const uint32_t *__restrict__ c
   }
 }
 
-// Largest coordinate magnitude of a record (the float test's rounding error scales with it).
-__device__ __forceinline__ float rec_extent(float4 a, float4 b, float4 c, float4 d) {
-  const float m0 = fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(b.x)));
-  const float m1 = fmaxf(fmaxf(fabsf(b.y), fabsf(b.z)), fmaxf(fabsf(b.w), fabsf(c.x)));
-  const float m2 = fmaxf(fmaxf(fabsf(c.y), fabsf(c.z)), fmaxf(fabsf(c.w), fabsf(d.x)));
-  const float m3 = fmaxf(fmaxf(fabsf(d.y), fabsf(d.z)), fabsf(d.w));
-  return fmaxf(fmaxf(m0, m1), fmaxf(m2, m3));
-}
-
 // Candidate pairings of two clumps whose bounding spheres overlap: bit (sm * 4 + sl) for
 // sphere sm of o and sl of q when their centres are closer than lim (r_q + r_o + margin).
 __device__ __forceinline__ uint32_t cand_mask(const float *qs, const float4 *os, float lim) {
@@ -2312,10 +2303,11 @@ __device__ __forceinline__ uint32_t cand_mask(const float *qs, const float4 *os,
   return mask;
 }
 
-// The float test's margin for the own clump (largest coordinate magnitude xq) and a
-// candidate with first record quarter oa: above the test's rounding error for the two
-// records.  Every sphere centre of the candidate lies within r (1 + sqrt(3) / div) of its
-// centre, so |centre| + r * bfac bounds the candidate's coordinates.
+// The float test's margin for the own clump (xq: a bound on its coordinates' magnitude) and
+// a candidate with first record quarter oa: above the test's rounding error for the two
+// records, which scales with the largest coordinate magnitude.  Every sphere centre of a
+// clump lies within r (1 + sqrt(3) / div) of its centre, so |centre| + r * bfac bounds the
+// magnitude of every coordinate of its record.
 __device__ __forceinline__ float pair_margin(float mh0, float xq, float4 oa, float bfac) {
   const float eo = fmaxf(fmaxf(fabsf(oa.x), fabsf(oa.y)), fabsf(oa.z)) + oa.w * bfac;
   return fmaxf(mh0, (xq + eo) * 0x1p-21f);
@@ -2348,8 +2340,6 @@ __global__ __launch_bounds__(kCsBlock) void k_csweep(
   if (k >= nbin) return;
   const uint32_t key = cskey[k], me = csid[k];
   const float4 qa = sca[k];
-  const float4 *qr = frec + 4 * (int64_t)me;
-  const float4 q1 = qr[1], q2 = qr[2], q3 = qr[3];
   int inst = 0;
   if (ninst > 1) {  // last instance whose first cell is <= key
     int lo = 0, hi = ninst - 1;
@@ -2388,11 +2378,23 @@ __global__ __launch_bounds__(kCsBlock) void k_csweep(
       rb[r9][tid] = b9[r9] < (uint64_t)nbin ? b9[r9] : (uint32_t)nbin;
       re[r9][tid] = e9[r9] < (uint64_t)nbin ? e9[r9] : (uint32_t)nbin;
     }
-    const float qs[12] = {q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
-    const float xq = rec_extent(qa, q1, q2, q3);
     const float mh0 = (float)p.cg_mh, bfac = (float)p.cg_bfac;
+    // a bound on the own record's coordinates (as pair_margin bounds the candidate's)
+    const float xq = fmaxf(fmaxf(fabsf(qa.x), fabsf(qa.y)), fabsf(qa.z)) + qa.w * bfac;
+    // the own sphere centres, read when the first candidate is queued (most clumps of a
+    // loose packing have none)
+    float qs[12];
+    bool own = false;
     int cnt = 0, nq = 0;
     auto drain = [&]() {
+      if (nq > 0 && !own) {
+        const float4 *qr = frec + 4 * (int64_t)me;
+        const float4 q1 = qr[1], q2 = qr[2], q3 = qr[3];
+        qs[0] = q1.x; qs[1] = q1.y; qs[2] = q1.z; qs[3] = q1.w;
+        qs[4] = q2.x; qs[5] = q2.y; qs[6] = q2.z; qs[7] = q2.w;
+        qs[8] = q3.x; qs[9] = q3.y; qs[10] = q3.z; qs[11] = q3.w;
+        own = true;
+      }
       for (int t = 0; t < nq; t++) {
         const uint32_t j = q[t][tid], m = csid[j];
         const float4 *r = frec + 4 * (int64_t)m;
