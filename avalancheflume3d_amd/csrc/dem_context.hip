@@ -67,7 +67,8 @@ struct avf_ctx {
   // coordinate; the lists go to the candidate-list arrays below
   bool clump_bins = false;
   uint32_t *d_ckey = nullptr, *d_crank = nullptr, *d_csid = nullptr;
-  float4 *d_frec = nullptr;  // float records of the clump sweep, in storage order
+  float4 *d_frec = nullptr;  // float records of the clump sweep, in storage order: centres
+                             // and radii (cap float4), then sphere centres (3 cap float4)
   float4 *d_sca = nullptr;      // their centres and radii in cell order
   uint32_t *d_cskey = nullptr;  // clump keys in cell order
   int64_t *d_cgrid = nullptr;  // [0] clump cells over all instances

@@ -80,12 +80,12 @@ __device__ __forceinline__ d3 sphere_of(d3 rr, const double *R, double pos, int 
                  rr.z + pos * (sx * R[6] + sy * R[7] + sz * R[8]));
 }
 
-// The clump sweep's 64-B float record of a binned clump from its stage record g: (centre,
-// radius), then the four sphere centres (sphere_centres on the record's centre and rotation:
-// the predict's FP64 centres) packed in three float4, coordinates relative to the
+// The clump sweep's float record of a binned clump from its stage record g: (centre, radius)
+// to ca, the four sphere centres (sphere_centres on the record's centre and rotation: the
+// predict's FP64 centres) packed in three float4 to cs, coordinates relative to the
 // instance's clump-grid origin.  The radius is the record's (a binned clump is not frozen).
 __device__ __forceinline__ void clump_frec(const double *g, const ClumpGrid &p,
-                                           float4 *__restrict__ out) {
+                                           float4 *__restrict__ ca, float4 *__restrict__ cs) {
   const d3 rr = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
   const double rad = g[G_RAD];
   double R[9];
@@ -104,10 +104,10 @@ __device__ __forceinline__ void clump_frec(const double *g, const ClumpGrid &p,
     f[4 + 3 * k] = (float)(s[k].y - oy);
     f[5 + 3 * k] = (float)(s[k].z - oz);
   }
-  out[0] = make_float4(f[0], f[1], f[2], (float)rad);
-  out[1] = make_float4(f[3], f[4], f[5], f[6]);
-  out[2] = make_float4(f[7], f[8], f[9], f[10]);
-  out[3] = make_float4(f[11], f[12], f[13], f[14]);
+  ca[0] = make_float4(f[0], f[1], f[2], (float)rad);
+  cs[0] = make_float4(f[3], f[4], f[5], f[6]);
+  cs[1] = make_float4(f[7], f[8], f[9], f[10]);
+  cs[2] = make_float4(f[11], f[12], f[13], f[14]);
 }
 
 // Mass and moment of inertia (isotropic) of a clump of four spheres of radius rad and density
@@ -394,7 +394,8 @@ __global__ __launch_bounds__(kPredictBlock) void k_predict(
   }
   __syncthreads();
   if (frec && c < n && keys[0] != kInvalidKey)  // the clump sweep's record (clump keys)
-    clump_frec(rec + threadIdx.x * kStagePad, clump_grid(P[inst_of[c]]), frec + 4 * c);
+    clump_frec(rec + threadIdx.x * kStagePad, clump_grid(P[inst_of[c]]), frec + c,
+               frec + cap + 3 * c);
   const int64_t rows = n - c0 < kPredictBlock ? n - c0 : kPredictBlock;
   double *dst = stage + c0 * kStageDoubles;
   for (int64_t k = threadIdx.x; k < rows * kStageDoubles; k += kPredictBlock)
@@ -1660,7 +1661,8 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
     }
     __syncthreads();
     if constexpr (CELLS && VERLET)  // the clump sweep's float record of the next step
-      if (c < n && keys[0] != kInvalidKey) clump_frec(rec + lane * kStagePad, cg, frec + 4 * c);
+      if (c < n && keys[0] != kInvalidKey)
+        clump_frec(rec + lane * kStagePad, cg, frec + c, frec + cap + 3 * c);
     const int64_t rows = n - c0 < BS ? n - c0 : BS;
     double *dst = stage_out + c0 * kStageDoubles;
     for (int64_t k = lane; k < rows * kStageDoubles; k += BS)
@@ -2270,7 +2272,7 @@ __global__ __launch_bounds__(256) void k_cscatter(const uint32_t *__restrict__ c
   const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (c >= n) return;
   const uint32_t key = ckey[c], rank = crank[c];
-  const float4 v = frec[4 * c];
+  const float4 v = frec[c];
   if (key == kInvalidKey) {
     vl_cnt[c] = 0;
     return;
@@ -2388,8 +2390,8 @@ __global__ __launch_bounds__(kCsBlock) void k_csweep(
     int cnt = 0, nq = 0;
     auto drain = [&]() {
       if (nq > 0 && !own) {
-        const float4 *qr = frec + 4 * (int64_t)me;
-        const float4 q1 = qr[1], q2 = qr[2], q3 = qr[3];
+        const float4 *qr = frec + cap + 3 * (int64_t)me;
+        const float4 q1 = qr[0], q2 = qr[1], q3 = qr[2];
         qs[0] = q1.x; qs[1] = q1.y; qs[2] = q1.z; qs[3] = q1.w;
         qs[4] = q2.x; qs[5] = q2.y; qs[6] = q2.z; qs[7] = q2.w;
         qs[8] = q3.x; qs[9] = q3.y; qs[10] = q3.z; qs[11] = q3.w;
@@ -2397,8 +2399,8 @@ __global__ __launch_bounds__(kCsBlock) void k_csweep(
       }
       for (int t = 0; t < nq; t++) {
         const uint32_t j = q[t][tid], m = csid[j];
-        const float4 *r = frec + 4 * (int64_t)m;
-        const float4 oa = sca[j], os[3] = {r[1], r[2], r[3]};
+        const float4 *r = frec + cap + 3 * (int64_t)m;
+        const float4 oa = sca[j], os[3] = {r[0], r[1], r[2]};
         const uint32_t mask =
             cand_mask(qs, os, qa.w + oa.w + pair_margin(mh0, xq, oa, bfac));
         if (!mask) continue;
