@@ -66,11 +66,11 @@ struct avf_ctx {
   // record per clump, the clump grid's cell and brick counts, the records' largest
   // coordinate; the lists go to the candidate-list arrays below
   bool clump_bins = false;
-  uint32_t *d_ckey = nullptr, *d_crank = nullptr, *d_csid = nullptr;
+  uint32_t *d_ckey = nullptr, *d_crank = nullptr;
+  uint2 *d_cik = nullptr;  // storage index and key of each clump in cell order
   float4 *d_frec = nullptr;  // float records of the clump sweep, in storage order: centres
                              // and radii (cap float4), then sphere centres (3 cap float4)
   float4 *d_sca = nullptr;      // their centres and radii in cell order
-  uint32_t *d_cskey = nullptr;  // clump keys in cell order
   int64_t *d_cgrid = nullptr;  // [0] clump cells over all instances
   // candidate lists (verlet_skin > 0): per clump up to kVMax neighbour clumps with their
   // candidate sphere pairings (entry-major), displacement tracking, chunk checkpoint
@@ -437,7 +437,6 @@ void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bo
   // four per clump in the sphere grid
   const bool cb = c->clump_bins;
   uint32_t *key = cb ? c->d_ckey : c->d_skey, *rank = cb ? c->d_crank : c->d_srank;
-  uint32_t *sid = cb ? c->d_csid : c->d_sid;
   int64_t *ncells = cb ? c->d_cgrid : c->d_ncells;
   mark(AVF_K_PREDICT);
   if (i == 0)
@@ -459,14 +458,14 @@ void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bo
   mark(AVF_K_SCATTER);
   if (cb)
     k_cscatter<<<std::max(1u, grid_for(c->n, 256)), 256, 0, c->stream>>>(
-        c->d_ckey, c->d_crank, c->d_starts, c->d_frec, c->d_sca, c->d_csid, c->d_cskey,
+        c->d_ckey, c->d_crank, c->d_starts, c->d_frec, c->d_sca, c->d_cik,
         c->d_vl_cnt, c->d_diag, c->n);
   else
     launch_scatter(c, c->d_skey, c->d_srank, c->d_starts, cur, c->d_spos, c->d_sid, c->d_ccnt);
   mark(AVF_K_DETECT);
   if (cb)
     k_csweep<<<std::max(1u, grid_for(c->n, kCsBlock)), kCsBlock, 0, c->stream>>>(
-        c->d_P, c->ninst, c->d_cgrid, c->d_starts, c->d_sca, c->d_frec, c->d_csid, c->d_cskey,
+        c->d_P, c->ninst, c->d_cgrid, c->d_starts, c->d_sca, c->d_frec, c->d_cik,
         c->d_vl_m, c->d_vl_mask, c->d_vl_cnt, c->d_diag, c->cap, c->n);
   else
     launch_detect(c);
@@ -481,7 +480,7 @@ void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bo
   if (i + 1 < S && !cb)  // the next step's counts and ranks, in this step's sorted order (by
                          // clump, the force kernel counts them)
     k_count_ranks<<<std::max(1u, grid_for(4 * c->n, 256)), 256, 0, c->stream>>>(
-        key, sid, c->d_starts, ncells, c->d_counts, rank, bs_next, (int32_t)G, 4 * c->n);
+        key, c->d_sid, c->d_starts, ncells, c->d_counts, rank, bs_next, (int32_t)G, 4 * c->n);
   mark(AVF_NKERNELS);
 }
 
@@ -1039,8 +1038,8 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
     return bail(rc);
   if (c->clump_bins &&
       ((rc = dalloc(c, &c->d_ckey, N)) || (rc = dalloc(c, &c->d_crank, N)) ||
-       (rc = dalloc(c, &c->d_csid, N)) || (rc = dalloc(c, &c->d_frec, 4 * N)) ||
-       (rc = dalloc(c, &c->d_sca, N)) || (rc = dalloc(c, &c->d_cskey, N)) ||
+       (rc = dalloc(c, &c->d_cik, N)) || (rc = dalloc(c, &c->d_frec, 4 * N)) ||
+       (rc = dalloc(c, &c->d_sca, N)) ||
        (rc = dalloc(c, &c->d_cgrid, 2))))
     return bail(rc);
   if (c->opt.verlet_skin > 0.0 &&
@@ -1090,8 +1089,7 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
       (he = hipMemset(c->d_ncells, 0, sizeof(int64_t))) != hipSuccess ||
       (he = hipMemset(c->d_nbricks, 0, sizeof(int64_t))) != hipSuccess ||
       (c->d_cgrid && (he = hipMemset(c->d_cgrid, 0, 2 * sizeof(int64_t))) != hipSuccess) ||
-      (c->d_csid && (he = hipMemset(c->d_csid, 0, N * sizeof(uint32_t))) != hipSuccess) ||
-      (c->d_cskey && (he = hipMemset(c->d_cskey, 0, N * sizeof(uint32_t))) != hipSuccess)) {
+      (c->d_cik && (he = hipMemset(c->d_cik, 0, N * sizeof(uint2))) != hipSuccess)) {
     fail(c, AVF_ERR_HIP, "hipMemset: %s", hipGetErrorString(he));
     return bail(AVF_ERR_HIP);
   }
@@ -1158,8 +1156,8 @@ int avf_destroy(avf_ctx *c) {
                   c->d_acc_sg, c->d_diag,  c->d_count, c->d_clist, c->d_ccnt,
                   c->d_vl_m,   c->d_vl_cnt, c->d_vl_mask, c->d_dsp, c->d_wreach,
                   c->d_ck_st,  c->d_ck_rad, c->d_ck_P, c->d_ck_diag, c->d_umax,
-                  c->d_stage2, c->d_ckey, c->d_crank, c->d_csid, c->d_frec,
-                  c->d_sca,    c->d_cskey, c->d_cgrid};
+                  c->d_stage2, c->d_ckey, c->d_crank, c->d_cik, c->d_frec,
+                  c->d_sca,    c->d_cgrid};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);

@@ -2245,8 +2245,9 @@ __global__ __launch_bounds__(kSwBlock) void k_sweep(
 // fifth of the cells).  The predict writes each binned clump's float record in storage order
 // (its centre, radius and four sphere centres as floats relative to the grid origin: 64 B);
 // k_cscatter moves each record's centre and radius (16 B, read by every bounding-sphere test)
-// into cell order, with the clump's storage index and key beside them; the sphere centres
-// (48 B, read for the few candidates that pass that test) stay in storage order.  k_csweep then lists, for every binned clump, the clumps of its 27 neighbour
+// into cell order, with the clump's storage index and key beside them (one 8-B pair); the
+// sphere centres (48 B, read for the few candidates that pass that test) stay in storage
+// order.  k_csweep then lists, for every binned clump, the clumps of its 27 neighbour
 // cells that may touch it, with the pairings that may: a float test with a margin far above
 // its rounding error (centre distance against the bounding radii, then the 16 sphere pairs),
 // so the listed pairings are a superset of the touching ones.  The force kernel re-tests
@@ -2265,8 +2266,7 @@ __global__ __launch_bounds__(256) void k_cscatter(const uint32_t *__restrict__ c
                                                   const uint32_t *__restrict__ starts,
                                                   const float4 *__restrict__ frec,
                                                   float4 *__restrict__ sca,
-                                                  uint32_t *__restrict__ csid,
-                                                  uint32_t *__restrict__ cskey,
+                                                  uint2 *__restrict__ cik,
                                                   int32_t *__restrict__ vl_cnt,
                                                   DevDiag *__restrict__ diag, int64_t n) {
   const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
@@ -2280,8 +2280,7 @@ __global__ __launch_bounds__(256) void k_cscatter(const uint32_t *__restrict__ c
   const uint32_t dst = starts[key] + rank;
   if (dst < (uint64_t)n) {
     sca[dst] = v;
-    csid[dst] = (uint32_t)c;
-    cskey[dst] = key;
+    cik[dst] = make_uint2((uint32_t)c, key);
   } else {  // counts and starts disagree: never expected; avf_step reports it
     atomicAdd(&diag->bin_broken, 1ull);
   }
@@ -2329,8 +2328,7 @@ __device__ __forceinline__ float pair_margin(float mh0, float xq, float4 oa, flo
 __global__ __launch_bounds__(kCsBlock) void k_csweep(
     const InstParams *__restrict__ P, int32_t ninst, const int64_t *__restrict__ ncells_p,
     const uint32_t *__restrict__ starts, const float4 *__restrict__ sca,
-    const float4 *__restrict__ frec, const uint32_t *__restrict__ csid,
-    const uint32_t *__restrict__ cskey,
+    const float4 *__restrict__ frec, const uint2 *__restrict__ cik,
     int32_t *__restrict__ vl_m, uint16_t *__restrict__ vl_mask, int32_t *__restrict__ vl_cnt,
     DevDiag *__restrict__ diag, int64_t cap, int64_t n) {
   __shared__ uint32_t q[kCq][kCsBlock];
@@ -2340,7 +2338,8 @@ __global__ __launch_bounds__(kCsBlock) void k_csweep(
   const uint32_t sk = starts[*ncells_p];
   const int64_t nbin = sk < (uint64_t)n ? (int64_t)sk : n;  // clumps binned this step
   if (k >= nbin) return;
-  const uint32_t key = cskey[k], me = csid[k];
+  const uint2 ik = cik[k];
+  const uint32_t key = ik.y, me = ik.x;
   const float4 qa = sca[k];
   int inst = 0;
   if (ninst > 1) {  // last instance whose first cell is <= key
@@ -2398,7 +2397,7 @@ __global__ __launch_bounds__(kCsBlock) void k_csweep(
         own = true;
       }
       for (int t = 0; t < nq; t++) {
-        const uint32_t j = q[t][tid], m = csid[j];
+        const uint32_t j = q[t][tid], m = cik[j].x;
         const float4 *r = frec + cap + 3 * (int64_t)m;
         const float4 oa = sca[j], os[3] = {r[0], r[1], r[2]};
         const uint32_t mask =
@@ -2462,14 +2461,27 @@ __global__ __launch_bounds__(256) void k_sgrid_reduce(const double *__restrict__
                     (long long)0x8000000000000000ull, (long long)0x8000000000000000ull,
                     0x7fffffffffffffffll};
   bool active = false;
+  // a thread's clumps are all loaded before any is reduced
+  double x[kSgItems], y[kSgItems], z[kSgItems], w2[kSgItems];
+  int ins[kSgItems];
+#pragma unroll
   for (int k = 0; k < kSgItems; k++) {
     const int64_t c = c0 + threadIdx.x + (int64_t)k * blockDim.x;
-    if (c >= n) break;
-    const double x = st[S_RX * cap + c], y = st[S_RY * cap + c], z = st[S_RZ * cap + c];
-    const double vx = st[S_VX * cap + c], vy = st[S_VY * cap + c], vz = st[S_VZ * cap + c];
-    const long long kx = order_key(x), ky = order_key(y), kz = order_key(z);
-    const long long kv = order_key(vx * vx + vy * vy + vz * vz);
-    const int inst = inst_of[c];
+    const bool in = c < n;
+    x[k] = in ? st[S_RX * cap + c] : 0.0;
+    y[k] = in ? st[S_RY * cap + c] : 0.0;
+    z[k] = in ? st[S_RZ * cap + c] : 0.0;
+    const double vx = in ? st[S_VX * cap + c] : 0.0, vy = in ? st[S_VY * cap + c] : 0.0,
+                 vz = in ? st[S_VZ * cap + c] : 0.0;
+    w2[k] = vx * vx + vy * vy + vz * vz;
+    ins[k] = in ? inst_of[c] : -1;
+  }
+#pragma unroll
+  for (int k = 0; k < kSgItems; k++) {
+    if (ins[k] < 0) continue;
+    const long long kx = order_key(x[k]), ky = order_key(y[k]), kz = order_key(z[k]);
+    const long long kv = order_key(w2[k]);
+    const int inst = ins[k];
     if (inst == inst0) {
       v[0] = kx < v[0] ? kx : v[0];
       v[1] = ky < v[1] ? ky : v[1];
