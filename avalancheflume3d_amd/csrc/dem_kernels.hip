@@ -2461,27 +2461,14 @@ __global__ __launch_bounds__(256) void k_sgrid_reduce(const double *__restrict__
                     (long long)0x8000000000000000ull, (long long)0x8000000000000000ull,
                     0x7fffffffffffffffll};
   bool active = false;
-  // a thread's clumps are all loaded before any is reduced
-  double x[kSgItems], y[kSgItems], z[kSgItems], w2[kSgItems];
-  int ins[kSgItems];
-#pragma unroll
   for (int k = 0; k < kSgItems; k++) {
     const int64_t c = c0 + threadIdx.x + (int64_t)k * blockDim.x;
-    const bool in = c < n;
-    x[k] = in ? st[S_RX * cap + c] : 0.0;
-    y[k] = in ? st[S_RY * cap + c] : 0.0;
-    z[k] = in ? st[S_RZ * cap + c] : 0.0;
-    const double vx = in ? st[S_VX * cap + c] : 0.0, vy = in ? st[S_VY * cap + c] : 0.0,
-                 vz = in ? st[S_VZ * cap + c] : 0.0;
-    w2[k] = vx * vx + vy * vy + vz * vz;
-    ins[k] = in ? inst_of[c] : -1;
-  }
-#pragma unroll
-  for (int k = 0; k < kSgItems; k++) {
-    if (ins[k] < 0) continue;
-    const long long kx = order_key(x[k]), ky = order_key(y[k]), kz = order_key(z[k]);
-    const long long kv = order_key(w2[k]);
-    const int inst = ins[k];
+    if (c >= n) break;
+    const double x = st[S_RX * cap + c], y = st[S_RY * cap + c], z = st[S_RZ * cap + c];
+    const double vx = st[S_VX * cap + c], vy = st[S_VY * cap + c], vz = st[S_VZ * cap + c];
+    const long long kx = order_key(x), ky = order_key(y), kz = order_key(z);
+    const long long kv = order_key(vx * vx + vy * vy + vz * vz);
+    const int inst = inst_of[c];
     if (inst == inst0) {
       v[0] = kx < v[0] ? kx : v[0];
       v[1] = ky < v[1] ? ky : v[1];

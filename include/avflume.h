@@ -188,8 +188,9 @@ typedef struct avf_diag {
   int64_t closed_instances; /* instances whose list steps recompute clump mass and     */
                             /* inertia from radius and density (avf_options.closed_forms */
                             /* and the upload check); the others load them              */
-  int64_t sweep_global_bricks; /* contact-sweep bricks whose halo did not fit LDS (swept */
-                            /* from global memory in FP64)                               */
+  int64_t sweep_global_bricks; /* sphere-cell sweep bricks (initial force pass,         */
+                            /* step_binning = 1, exports) whose halo did not fit LDS     */
+                            /* (swept from global memory)                                */
   int64_t sweep_fp64_pairs; /* sphere pairs within the float test's margin of sep,       */
                             /* decided by the FP64 test                                  */
 } avf_diag;
