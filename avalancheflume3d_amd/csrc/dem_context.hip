@@ -442,7 +442,7 @@ void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bo
   if (i == 0)
     k_predict<<<grid_for(c->n, kPredictBlock), kPredictBlock, 0, c->stream>>>(
         c->d_st, c->d_sta, c->d_stage, c->d_inst, c->d_orig, c->d_P, c->d_counts, key, rank,
-        c->n, c->cap, 1, ListTrack{nullptr, nullptr}, cb ? 1 : 0, cb ? c->d_frec : nullptr);
+        c->n, c->cap, 3, ListTrack{nullptr, nullptr}, cb ? 1 : 0, cb ? c->d_frec : nullptr);
   mark(AVF_K_SCAN_A);
   // the share sums of the scan: k_count_sum on a chunk's first step, after that added up by
   // the previous step's count pass into the step's half of d_bsum2 (the scan clears the

@@ -267,7 +267,9 @@ __device__ __forceinline__ void clump_keys(const InstParams &p, d3 rr, const dou
   for (int s = 0; s < 4; s++) keys[s] = out ? kInvalidKey : sphere_key(p, sph[s]);
 }
 
-// mode 1: predict (init_Verlet) then stage and sphere-cell keys (into keys[4]); mode 2: the
+// mode 1: predict (init_Verlet) then stage and sphere-cell keys (into keys[4]); mode 3: the
+// same without writing the predicted state back to st (per-step binning: a chunk's steps
+// carry the state in the stage records, and its last step writes it back); mode 2: the
 // same predict and stage without keys (candidate-list step), updating the displacement
 // bound; mode 0: stage the current state only (initial force pass, avalanche_hst.cu:365-384,
 // whose force stage sees the unchanged time), with keys.  t_old: the time at the start of the step (P.time, or
@@ -299,17 +301,19 @@ __device__ __forceinline__ void predict_clump(double *__restrict__ st, double *_
     } else {
       const d3 ang = make_d3(st[S_GX * cap + c], st[S_GY * cap + c], st[S_GZ * cap + c]);
       rot_step = verlet_advance(rr, vv, ww, R, aa, ang, p.dt, p.hdt);
-      st[S_RX * cap + c] = rr.x;
-      st[S_RY * cap + c] = rr.y;
-      st[S_RZ * cap + c] = rr.z;
-      st[S_VX * cap + c] = vv.x;
-      st[S_VY * cap + c] = vv.y;
-      st[S_VZ * cap + c] = vv.z;
-      st[S_WX * cap + c] = ww.x;
-      st[S_WY * cap + c] = ww.y;
-      st[S_WZ * cap + c] = ww.z;
+      if (mode != 3) {
+        st[S_RX * cap + c] = rr.x;
+        st[S_RY * cap + c] = rr.y;
+        st[S_RZ * cap + c] = rr.z;
+        st[S_VX * cap + c] = vv.x;
+        st[S_VY * cap + c] = vv.y;
+        st[S_VZ * cap + c] = vv.z;
+        st[S_WX * cap + c] = ww.x;
+        st[S_WY * cap + c] = ww.y;
+        st[S_WZ * cap + c] = ww.z;
 #pragma unroll
-      for (int k = 0; k < 9; k++) st[(S_R00 + k) * cap + c] = R[k];
+        for (int k = 0; k < 9; k++) st[(S_R00 + k) * cap + c] = R[k];
+      }
     }
   }
 
