@@ -1120,7 +1120,7 @@ __device__ __forceinline__ void force_clump(
 
   // the candidate list's length and first entry, loaded with the own record (the entry is
   // read whatever the length, so the loads do not wait for each other)
-  const int kc0 = VERLET ? vl_cnt[c] : 0;
+  const int kc0 = VERLET ? min(vl_cnt[c], kVMax) : 0;  // (the clump sweep counts past kVMax)
   const int32_t m00 = VERLET ? vl_m[c] : 0;
   const uint16_t b00 = VERLET ? vl_mask[c] : (uint16_t)0;
   const bool frz = frozen(time, rr.x, p.side_x);
@@ -2263,8 +2263,9 @@ constexpr int kCsBlock = 256;
 constexpr int kCq = 8;  // candidates queued per lane between the sweep's two passes
 
 // One lane per clump (storage order): the centre and radius of its float record, its storage
-// index and its key to its place in cell order (its cell's start + its rank).  A clump that
-// is not binned (past side.x) gets an empty list.
+// index and its key to its place in cell order (its cell's start + its rank); every clump's
+// list is emptied (the sweep appends to it).  A clump that is not binned (past side.x)
+// keeps an empty list.
 __global__ __launch_bounds__(256) void k_cscatter(const uint32_t *__restrict__ ckey,
                                                   const uint32_t *__restrict__ crank,
                                                   const uint32_t *__restrict__ starts,
@@ -2277,10 +2278,8 @@ __global__ __launch_bounds__(256) void k_cscatter(const uint32_t *__restrict__ c
   if (c >= n) return;
   const uint32_t key = ckey[c], rank = crank[c];
   const float4 v = frec[c];
-  if (key == kInvalidKey) {
-    vl_cnt[c] = 0;
-    return;
-  }
+  vl_cnt[c] = 0;
+  if (key == kInvalidKey) return;
   const uint32_t dst = starts[key] + rank;
   if (dst < (uint64_t)n) {
     sca[dst] = v;
@@ -2318,17 +2317,24 @@ __device__ __forceinline__ float pair_margin(float mh0, float xq, float4 oa, flo
   return fmaxf(mh0, (xq + eo) * 0x1p-21f);
 }
 
-// The candidate lists of every binned clump, one lane per clump in cell order.  A lane reads
-// its clump's record, storage index and key (consecutive lanes: consecutive addresses), the
-// first and last clump of the three cells x - 1 .. x + 1 of each of its 9 neighbour rows
-// (each a contiguous range of the cell order), then walks the 9 ranges reading the records
-// from global memory (lanes of a wavefront sit in the same or neighbouring cells, so they
-// read the same ranges).  Two passes, so that the lanes of a wavefront test sphere pairs
-// together instead of each whenever one of them meets a candidate: the bounding-sphere test
-// queues each candidate that passes (kCq per lane in LDS; a full queue is emptied on the
-// spot), then the 16 sphere pairs of the queued candidates.  Entries go to vl_m / vl_mask
-// (entry-major, as the candidate lists of k_vmerge), at most kVMax (the rest are counted in
-// diag.neigh_overflow).
+// The candidate lists of every binned clump, one lane per clump in cell order.  Every pair of
+// neighbouring clumps is tested once, by the lane of its clump earlier in cell order: its
+// partner lies in the same row after it, in the row above or in the three rows of the next
+// layer, so a lane reads its clump's record, storage index and key (consecutive lanes:
+// consecutive addresses), the first and last clump of the three cells x - 1 .. x + 1 of
+// those 5 rows (each a contiguous range of the cell order), then walks the 5 ranges reading
+// the records from global memory (lanes of a wavefront sit in the same or neighbouring
+// cells, so they read the same ranges).  Two passes, so that the lanes of a wavefront test
+// sphere pairs together instead of each whenever one of them meets a candidate: the
+// bounding-sphere test queues each candidate that passes (kCq per lane in LDS; a full queue
+// is emptied on the spot), then the 16 sphere pairs of the queued candidates.  A pair that
+// may touch is entered in both clumps' lists (vl_m / vl_mask, entry-major as the candidate
+// lists of k_vmerge; slots taken by atomicAdd on the list lengths, which the scatter
+// emptied), the partner's entry with the transposed pairing mask: both tests are symmetric
+// (float subtraction is, and the margin is built from both records alike), so a clump's
+// list holds the same pairings as a test from its own side.  At most kVMax entries per
+// clump are kept (the rest are counted in diag.neigh_overflow; the length may count past
+// kVMax).
 __global__ __launch_bounds__(kCsBlock) void k_csweep(
     const InstParams *__restrict__ P, int32_t ninst, const int64_t *__restrict__ ncells_p,
     const uint32_t *__restrict__ starts, const float4 *__restrict__ sca,
@@ -2372,7 +2378,7 @@ __global__ __launch_bounds__(kCsBlock) void k_csweep(
     for (int r9 = 0; r9 < 9; r9++) {
       const int yy = y + r9 % 3 - 1, zz = z + r9 / 3 - 1;
       b9[r9] = e9[r9] = 0u;
-      if (inside && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
+      if (r9 >= 4 && inside && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
         const int64_t row = p.cg_base + (int64_t)nx * (yy + (int64_t)ny * zz);
         b9[r9] = starts[row + xl];
         e9[r9] = starts[row + xh];
@@ -2390,7 +2396,7 @@ __global__ __launch_bounds__(kCsBlock) void k_csweep(
     // loose packing have none)
     float qs[12];
     bool own = false;
-    int cnt = 0, nq = 0;
+    int nq = 0;
     auto drain = [&]() {
       if (nq > 0 && !own) {
         const float4 *qr = frec + cap + 3 * (int64_t)me;
@@ -2407,11 +2413,26 @@ __global__ __launch_bounds__(kCsBlock) void k_csweep(
         const uint32_t mask =
             cand_mask(qs, os, qa.w + oa.w + pair_margin(mh0, xq, oa, bfac));
         if (!mask) continue;
-        if (cnt < kVMax) {
-          vl_m[cnt * cap + me] = (int32_t)m;
-          vl_mask[cnt * cap + me] = (uint16_t)mask;
+        // the pair's entry in both lists: the own one with this mask, the candidate's with
+        // its transpose (its spheres first); slots from the lists' lengths
+        uint32_t mt = 0u;
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+          for (int b = 0; b < 4; b++) mt |= ((mask >> (a * 4 + b)) & 1u) << (b * 4 + a);
+        const int s0 = atomicAdd(&vl_cnt[me], 1), s1 = atomicAdd(&vl_cnt[m], 1);
+        if (s0 < kVMax) {
+          vl_m[s0 * cap + me] = (int32_t)m;
+          vl_mask[s0 * cap + me] = (uint16_t)mask;
+        } else {
+          atomicAdd(&diag->neigh_overflow, 1ull);
         }
-        cnt++;
+        if (s1 < kVMax) {
+          vl_m[s1 * cap + m] = (int32_t)me;
+          vl_mask[s1 * cap + m] = (uint16_t)mt;
+        } else {
+          atomicAdd(&diag->neigh_overflow, 1ull);
+        }
       }
       nq = 0;
     };
@@ -2423,9 +2444,13 @@ __global__ __launch_bounds__(kCsBlock) void k_csweep(
       if (nq == kCq) drain();
       q[nq++][tid] = j;
     };
+    // each pair is found once, from its clump earlier in cell order: the candidates after k,
+    // which lie in the own row (from k + 1), the row above and the three rows of the next
+    // layer (rows 4 .. 8)
 #pragma unroll 1
-    for (int r9 = 0; r9 < 9; r9++) {
+    for (int r9 = 4; r9 < 9; r9++) {
       uint32_t j = rb[r9][tid];
+      if (r9 == 4 && j <= (uint32_t)k) j = (uint32_t)k + 1u;
       const uint32_t e = re[r9][tid];
       for (; j + 1 < e; j += 2) {  // two candidates' records in flight
         const float4 oa = sca[j], ob = sca[j + 1];
@@ -2435,11 +2460,6 @@ __global__ __launch_bounds__(kCsBlock) void k_csweep(
       if (j < e) test(j, sca[j]);
     }
     drain();
-    if (cnt > kVMax) {
-      atomicAdd(&diag->neigh_overflow, 1ull);
-      cnt = kVMax;
-    }
-    vl_cnt[me] = cnt;
     break;
   }
 }
