@@ -68,8 +68,7 @@ struct avf_ctx {
   bool clump_bins = false;
   uint32_t *d_ckey = nullptr, *d_crank = nullptr;
   uint2 *d_cik = nullptr;  // storage index and key of each clump in cell order
-  float4 *d_frec = nullptr;  // float records of the clump sweep, in storage order: centres
-                             // and radii (cap float4), then sphere centres (3 cap float4)
+  float4 *d_frec = nullptr;  // float records of the clump sweep (centre, radius), storage order
   float4 *d_sca = nullptr;      // their centres and radii in cell order
   int64_t *d_cgrid = nullptr;  // [0] clump cells over all instances
   // candidate lists (verlet_skin > 0): per clump up to kVMax neighbour clumps with their
@@ -465,7 +464,7 @@ void launch_cstep(avf_ctx *c, int64_t i, int64_t S, hipEvent_t *ev = nullptr, bo
   mark(AVF_K_DETECT);
   if (cb)
     k_csweep<<<std::max(1u, grid_for(c->n, kCsBlock)), kCsBlock, 0, c->stream>>>(
-        c->d_P, c->ninst, c->d_cgrid, c->d_starts, c->d_sca, c->d_frec, c->d_cik,
+        c->d_P, c->ninst, c->d_cgrid, c->d_starts, c->d_sca, cur, c->d_cik,
         c->d_vl_m, c->d_vl_mask, c->d_vl_cnt, c->d_diag, c->cap, c->n);
   else
     launch_detect(c);
@@ -1038,7 +1037,7 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
     return bail(rc);
   if (c->clump_bins &&
       ((rc = dalloc(c, &c->d_ckey, N)) || (rc = dalloc(c, &c->d_crank, N)) ||
-       (rc = dalloc(c, &c->d_cik, N)) || (rc = dalloc(c, &c->d_frec, 4 * N)) ||
+       (rc = dalloc(c, &c->d_cik, N)) || (rc = dalloc(c, &c->d_frec, N)) ||
        (rc = dalloc(c, &c->d_sca, N)) ||
        (rc = dalloc(c, &c->d_cgrid, 2))))
     return bail(rc);

@@ -80,34 +80,32 @@ __device__ __forceinline__ d3 sphere_of(d3 rr, const double *R, double pos, int 
                  rr.z + pos * (sx * R[6] + sy * R[7] + sz * R[8]));
 }
 
-// The clump sweep's float record of a binned clump from its stage record g: (centre, radius)
-// to ca, the four sphere centres (sphere_centres on the record's centre and rotation: the
-// predict's FP64 centres) packed in three float4 to cs, coordinates relative to the
-// instance's clump-grid origin.  The radius is the record's (a binned clump is not frozen).
-__device__ __forceinline__ void clump_frec(const double *g, const ClumpGrid &p,
-                                           float4 *__restrict__ ca, float4 *__restrict__ cs) {
+// The clump sweep's float record of a binned clump from its stage record g: its centre,
+// relative to the instance's clump-grid origin, and radius (the record's: a binned clump is
+// not frozen).
+__device__ __forceinline__ void clump_fca(const double *g, const ClumpGrid &p,
+                                          float4 *__restrict__ ca) {
+  ca[0] = make_float4((float)(g[G_RR] - p.lo[0]), (float)(g[G_RR + 1] - p.lo[1]),
+                      (float)(g[G_RR + 2] - p.lo[2]), (float)g[G_RAD]);
+}
+
+// The four sphere centres of a clump as floats relative to the clump-grid origin, from its
+// stage record g (sphere_centres on the record's centre and rotation: the predict's FP64
+// centres): sphere k at f[3k .. 3k + 2].
+__device__ __forceinline__ void clump_fcs(const double *__restrict__ g, const ClumpGrid &p,
+                                          float *f) {
   const d3 rr = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
-  const double rad = g[G_RAD];
   double R[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) R[k] = g[G_R + k];
   d3 s[4];
-  sphere_centres(rr, R, rad / p.div, s);
-  const double ox = p.lo[0], oy = p.lo[1], oz = p.lo[2];
-  float f[15];
-  f[0] = (float)(rr.x - ox);
-  f[1] = (float)(rr.y - oy);
-  f[2] = (float)(rr.z - oz);
+  sphere_centres(rr, R, g[G_RAD] / p.div, s);
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    f[3 + 3 * k] = (float)(s[k].x - ox);
-    f[4 + 3 * k] = (float)(s[k].y - oy);
-    f[5 + 3 * k] = (float)(s[k].z - oz);
+    f[3 * k] = (float)(s[k].x - p.lo[0]);
+    f[3 * k + 1] = (float)(s[k].y - p.lo[1]);
+    f[3 * k + 2] = (float)(s[k].z - p.lo[2]);
   }
-  ca[0] = make_float4(f[0], f[1], f[2], (float)rad);
-  cs[0] = make_float4(f[3], f[4], f[5], f[6]);
-  cs[1] = make_float4(f[7], f[8], f[9], f[10]);
-  cs[2] = make_float4(f[11], f[12], f[13], f[14]);
 }
 
 // Mass and moment of inertia (isotropic) of a clump of four spheres of radius rad and density
@@ -398,8 +396,7 @@ __global__ __launch_bounds__(kPredictBlock) void k_predict(
   }
   __syncthreads();
   if (frec && c < n && keys[0] != kInvalidKey)  // the clump sweep's record (clump keys)
-    clump_frec(rec + threadIdx.x * kStagePad, clump_grid(P[inst_of[c]]), frec + c,
-               frec + cap + 3 * c);
+    clump_fca(rec + threadIdx.x * kStagePad, clump_grid(P[inst_of[c]]), frec + c);
   const int64_t rows = n - c0 < kPredictBlock ? n - c0 : kPredictBlock;
   double *dst = stage + c0 * kStageDoubles;
   for (int64_t k = threadIdx.x; k < rows * kStageDoubles; k += kPredictBlock)
@@ -1666,7 +1663,7 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
     __syncthreads();
     if constexpr (CELLS && VERLET)  // the clump sweep's float record of the next step
       if (c < n && keys[0] != kInvalidKey)
-        clump_frec(rec + lane * kStagePad, cg, frec + c, frec + cap + 3 * c);
+        clump_fca(rec + lane * kStagePad, cg, frec + c);
     const int64_t rows = n - c0 < BS ? n - c0 : BS;
     double *dst = stage_out + c0 * kStageDoubles;
     for (int64_t k = lane; k < rows * kStageDoubles; k += BS)
@@ -2247,12 +2244,13 @@ __global__ __launch_bounds__(kSwBlock) void k_sweep(
 // side >= the largest clump's bounding diameter, so the clumps of any touching pair sit in
 // neighbouring cells), counted and scanned like the sphere keys (a quarter of the keys, a
 // fifth of the cells).  The predict writes each binned clump's float record in storage order
-// (its centre, radius and four sphere centres as floats relative to the grid origin: 64 B);
-// k_cscatter moves each record's centre and radius (16 B, read by every bounding-sphere test)
-// into cell order, with the clump's storage index and key beside them (one 8-B pair); the
-// sphere centres (48 B, read for the few candidates that pass that test) stay in storage
-// order.  k_csweep then lists, for every binned clump, the clumps of its 27 neighbour
-// cells that may touch it, with the pairings that may: a float test with a margin far above
+// (its centre relative to the grid origin and its radius, as floats: 16 B, read by every
+// bounding-sphere test); k_cscatter moves it into cell order, with the clump's storage index
+// and key beside it (one 8-B pair).  The sphere centres, needed only for the few candidates
+// that pass that test, are computed then from the clump's stage record, as floats relative
+// to the grid origin.  k_csweep then lists, for every binned clump, the clumps of its 27
+// neighbour cells that may touch it, with the pairings that may: a float test with a margin
+// far above
 // its rounding error (centre distance against the bounding radii, then the 16 sphere pairs),
 // so the listed pairings are a superset of the touching ones.  The force kernel re-tests
 // every listed pairing with force_part.cu:140-157's FP64 test on the stage records' centres
@@ -2291,9 +2289,7 @@ __global__ __launch_bounds__(256) void k_cscatter(const uint32_t *__restrict__ c
 
 // Candidate pairings of two clumps whose bounding spheres overlap: bit (sm * 4 + sl) for
 // sphere sm of o and sl of q when their centres are closer than lim (r_q + r_o + margin).
-__device__ __forceinline__ uint32_t cand_mask(const float *qs, const float4 *os, float lim) {
-  const float o[12] = {os[0].x, os[0].y, os[0].z, os[0].w, os[1].x, os[1].y,
-                       os[1].z, os[1].w, os[2].x, os[2].y, os[2].z, os[2].w};
+__device__ __forceinline__ uint32_t cand_mask(const float *qs, const float *o, float lim) {
   const float lim2 = lim * lim;
   uint32_t mask = 0u;
 #pragma unroll
@@ -2338,7 +2334,7 @@ __device__ __forceinline__ float pair_margin(float mh0, float xq, float4 oa, flo
 __global__ __launch_bounds__(kCsBlock) void k_csweep(
     const InstParams *__restrict__ P, int32_t ninst, const int64_t *__restrict__ ncells_p,
     const uint32_t *__restrict__ starts, const float4 *__restrict__ sca,
-    const float4 *__restrict__ frec, const uint2 *__restrict__ cik,
+    const double *__restrict__ stage, const uint2 *__restrict__ cik,
     int32_t *__restrict__ vl_m, uint16_t *__restrict__ vl_mask, int32_t *__restrict__ vl_cnt,
     DevDiag *__restrict__ diag, int64_t cap, int64_t n) {
   __shared__ uint32_t q[kCq][kCsBlock];
@@ -2392,24 +2388,22 @@ __global__ __launch_bounds__(kCsBlock) void k_csweep(
     const float mh0 = (float)p.cg_mh, bfac = (float)p.cg_bfac;
     // a bound on the own record's coordinates (as pair_margin bounds the candidate's)
     const float xq = fmaxf(fmaxf(fabsf(qa.x), fabsf(qa.y)), fabsf(qa.z)) + qa.w * bfac;
-    // the own sphere centres, read when the first candidate is queued (most clumps of a
-    // loose packing have none)
+    // the own sphere centres, from the own stage record when the first candidate is queued
+    // (most clumps of a loose packing have none)
+    const ClumpGrid cgp = clump_grid(p);
     float qs[12];
     bool own = false;
     int nq = 0;
     auto drain = [&]() {
       if (nq > 0 && !own) {
-        const float4 *qr = frec + cap + 3 * (int64_t)me;
-        const float4 q1 = qr[0], q2 = qr[1], q3 = qr[2];
-        qs[0] = q1.x; qs[1] = q1.y; qs[2] = q1.z; qs[3] = q1.w;
-        qs[4] = q2.x; qs[5] = q2.y; qs[6] = q2.z; qs[7] = q2.w;
-        qs[8] = q3.x; qs[9] = q3.y; qs[10] = q3.z; qs[11] = q3.w;
+        clump_fcs(stage + (int64_t)me * kStageDoubles, cgp, qs);
         own = true;
       }
       for (int t = 0; t < nq; t++) {
         const uint32_t j = q[t][tid], m = cik[j].x;
-        const float4 *r = frec + cap + 3 * (int64_t)m;
-        const float4 oa = sca[j], os[3] = {r[0], r[1], r[2]};
+        const float4 oa = sca[j];
+        float os[12];
+        clump_fcs(stage + (int64_t)m * kStageDoubles, cgp, os);
         const uint32_t mask =
             cand_mask(qs, os, qa.w + oa.w + pair_margin(mh0, xq, oa, bfac));
         if (!mask) continue;

@@ -62,15 +62,14 @@ SWEEP_BYTES_PER_CLUMP = 4 * (32.0 + 4.0 + 8.0 + 4.0)
 # Per-step binning by clump (the default): k_force reads the stage record (192 B), the
 # angular acceleration (24 B), mass and inertia (32 B), the instance index and list length
 # (8 B), and writes the next stage record (192 B), the angular acceleration (24 B), the next
-# clump-cell key and the clump's rank in its cell (8 B) and the sweep's float record (centre,
-# radius, four sphere centres as floats: 64 B): 544 B; list entries, the neighbours' records
-# and the cell counters (atomics in L2) not counted.  k_csweep reads each binned clump's
-# centre and radius in cell order (16 B), its storage index and key (8 B), its sphere
-# centres (48 B) and two cell starts (8 B), and writes its list length (4 B): 84 B; the
-# candidates' records (read again by neighbouring clumps, from cache) and the list entries
-# not counted.
-CLUMP_FORCE_BYTES = 544.0
-CSWEEP_BYTES_PER_CLUMP = 84.0
+# clump-cell key and the clump's rank in its cell (8 B) and the sweep's float record (centre
+# and radius as floats: 16 B): 496 B; list entries, the neighbours' records and the cell
+# counters (atomics in L2) not counted.  k_csweep reads each binned clump's centre and radius
+# in cell order (16 B), its storage index and key (8 B) and two cell starts (8 B): 32 B; the
+# candidates' centres (read again by neighbouring clumps, from cache), the stage records of
+# clumps with candidates that pass the bounding test, and the list entries not counted.
+CLUMP_FORCE_BYTES = 496.0
+CSWEEP_BYTES_PER_CLUMP = 32.0
 KERNEL_NAMES = {"sphere": {"force": "k_force", "detect": "k_sweep"},
                 "clump": {"force": "k_force", "detect": "k_csweep"}}
 
