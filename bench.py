@@ -33,8 +33,9 @@ HBM_PEAK_GBS = 8000.0
 # bound (8 B).  The list entries (6 B each) and the neighbours' stage records (L2 hits) are
 # not counted.
 FORCE_BYTES_PER_CLUMP = 488.0
-# Mass and inertia are not read where the list steps recompute them from radius and density
-# (working sets beyond the last-level cache; avf_diag.closed_instances says so): 456 B there.
+# Mass and inertia are not read where the steps recompute them from radius and density
+# (per-step binning, and candidate lists for working sets beyond the last-level cache;
+# avf_diag.closed_instances says so): 32 B less there.
 MASS_INERTIA_BYTES = 32.0
 # Algorithmic bytes per clump-step of the whole step (480 B per clump-step).
 STEP_BYTES_PER_CLUMP = 480.0
