@@ -521,7 +521,7 @@ void launch_vstep(avf_ctx *c, bool rebuild, hipEvent_t *ev = nullptr, bool ft = 
   if (rebuild) {
     launch_detect(c, c->opt.verlet_skin);
     k_vmerge<kForceBlock><<<grid_for(c->n, kForceBlock), kForceBlock, 0, c->stream>>>(
-        c->d_stage, c->d_P, c->d_inst, c->d_clist, c->d_ccnt, c->d_vl_m, c->d_vl_mask,
+        c->d_stage, c->d_P, c->d_inst, c->d_orig, c->d_clist, c->d_ccnt, c->d_vl_m, c->d_vl_mask,
         c->d_vl_cnt, list_track(c), c->d_diag, c->n, c->cap, c->opt.verlet_skin);
   }
   mark(AVF_K_FORCE);
@@ -1961,7 +1961,7 @@ extern "C" int avf_window_run(avf_ctx *c, const avf_window_spec *spec, const dou
   HIPCHK(c, hipMemsetAsync(w->umax, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(c, hipMemsetAsync(w->missed, 0, sizeof(unsigned long long), c->stream));
   k_win_list<<<grid_for(n, 64), 64, 0, c->stream>>>(
-      c->d_st, c->d_sta, w->bstage, c->d_P, c->d_inst, w->bstarts, w->bspos, w->bsid,
+      c->d_st, c->d_sta, w->bstage, c->d_P, c->d_inst, c->d_orig, w->bstarts, w->bspos, w->bsid,
       spec->skin, w->list, w->cnt, w->sph0, w->dmax2, w->wd, n, cap);
   k_win_in_fill<<<grid_for(n, kBlock), kBlock, 0, c->stream>>>(w->list, w->cnt, w->in, w->icnt,
                                                                w->wd, n);

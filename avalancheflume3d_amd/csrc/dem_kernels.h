@@ -57,15 +57,15 @@ enum StaticField { C_RAD, C_RAD0, C_RHO, C_MASS, C_IX, C_IY, C_IZ, C_NFIELDS };
 // Force / torque of the last force stage.
 enum ForceField { F_FX, F_FY, F_FZ, F_TX, F_TY, F_TZ, F_NFIELDS };
 
-// 192-B record per clump written by the predict pass and gathered by neighbours:
-// rr[3], rad, vv[3], rho, ww[3], (orig | flags), R rows[9], a[3] (the acceleration the
-// step's finish uses).  With R and a it holds the whole carried state except the angular
-// acceleration, so inside a chunk of list steps a clump can live in its record alone.
+// 176-B record per clump written by the predict pass and gathered by neighbours:
+// rr[3], rad, vv[3], ww[3], R rows[9], a[3] (the acceleration the step's finish uses).
+// With R and a it holds the whole carried state except the angular acceleration, so inside
+// a chunk of list steps a clump can live in its record alone.  A clump's density and
+// reference index, which no step changes, are read from sta and orig.
 // Sphere centres are not stored: whoever needs one recomputes rr + pos R (+-1, +-1, +-1)
 // with sphere_of, bit-identical to the predict's, pos = (radius before freezing) / div.
-constexpr int kStageDoubles = 24;
-enum StageSlot { G_RR = 0, G_RAD = 3, G_VV = 4, G_RHO = 7, G_WW = 8, G_ID = 11, G_R = 12,
-                 G_AA = 21 };
+constexpr int kStageDoubles = 22;
+enum StageSlot { G_RR = 0, G_RAD = 3, G_VV = 4, G_WW = 7, G_R = 10, G_AA = 19 };
 
 constexpr uint32_t kInvalidKey = 0xffffffffu;
 constexpr int kScanBlock = 256;

@@ -1,17 +1,21 @@
 // dem_kernels.hip -- gfx950 kernels of the DEM step for tetrahedral 4-sphere clumps.
 //
-// One step (reference: avalanche_hst.cu:668-710, avalanche; :438-467, hopper filling):
-//   k_predict   init_Verlet.cu:11-152 fused with the sphere-centre update, the 192-B stage
-//               record and the sphere-cell keys (integer atomic ranks: counting sort);
-//   k_scan_*    exclusive scan of the sphere-cell counts;
-//   k_scatter   places sphere records (x, y, z, r) + id in cell order;
-//   k_detect    the touching (or, with a skin, candidate) spheres of every sphere from the
-//               27 sphere cells around it; k_vmerge turns them into per-clump lists;
-//   k_force     boundary modules + force_part (contacts folded in the reference's summation
-//               order) + get_accel + finish_Verlet + the cell_dim reduction when the regrid
-//               is due; on candidate-list steps also the next step's predict;
-//   k_finalize  the regrid arithmetic and cc++ (per step, or once per chunk of list steps).
-// No float atomics anywhere: per-clump sums are sequential in the reference order.
+// One step with the cell list rebuilt every step, binned by clump (reference:
+// avalanche_hst.cu:668-710, avalanche; :438-467, hopper filling):
+//   k_count_scan  exclusive scan of the clump-cell counts (the step's time and counter);
+//   k_cscatter    each clump's centre and radius, storage index and key into cell order;
+//   k_csweep      the candidate pairings of every clump from its 27 clump cells (a float
+//                 test with a margin: a superset of the touching ones);
+//   k_force       boundary modules + force_part on the listed pairings (re-tested in FP64,
+//                 contacts folded in the reference's summation order) + get_accel +
+//                 finish_Verlet + the cell_dim reduction when the regrid is due, then the
+//                 next step's init_Verlet.cu:11-152 predict, 176-B stage record, clump key
+//                 and count (integer atomic ranks: counting sort) and float record;
+//   k_finalize    the regrid arithmetic and cc++ at the end of a chunk.
+// A chunk's first step starts with k_predict and k_count_sum.  Binned by sphere instead,
+// k_scatter and k_sweep take the place of k_cscatter and k_csweep; on candidate-list steps
+// k_force alone makes the step (k_vmerge builds the lists).  No float atomics anywhere:
+// per-clump sums are sequential in the reference order.
 #include "dem_kernels.h"
 
 namespace avf {
@@ -199,11 +203,10 @@ __device__ __forceinline__ double verlet_advance(d3 &rr, d3 &vv, d3 &ww, double 
   return mag;
 }
 
-// The 24-double stage record of a clump (what neighbours gather): centre, radius, velocity,
-// density, angular velocity, reference index, rotation rows, acceleration.
+// The 22-double stage record of a clump (what neighbours gather): centre, radius, velocity,
+// angular velocity, rotation rows, acceleration.
 __device__ __forceinline__ void fill_stage(double *__restrict__ g, d3 rr, double rad, d3 vv,
-                                           double rho, d3 ww, uint32_t id, const double *R,
-                                           d3 aa) {
+                                           d3 ww, const double *R, d3 aa) {
   g[G_RR + 0] = rr.x;
   g[G_RR + 1] = rr.y;
   g[G_RR + 2] = rr.z;
@@ -211,11 +214,9 @@ __device__ __forceinline__ void fill_stage(double *__restrict__ g, d3 rr, double
   g[G_VV + 0] = vv.x;
   g[G_VV + 1] = vv.y;
   g[G_VV + 2] = vv.z;
-  g[G_RHO] = rho;
   g[G_WW + 0] = ww.x;
   g[G_WW + 1] = ww.y;
   g[G_WW + 2] = ww.z;
-  g[G_ID] = __longlong_as_double((long long)id);
 #pragma unroll
   for (int k = 0; k < 9; k++) g[G_R + k] = R[k];
   g[G_AA + 0] = aa.x;
@@ -315,7 +316,7 @@ __device__ __forceinline__ void predict_clump(double *__restrict__ st, double *_
     }
   }
 
-  fill_stage(g, rr, rad, vv, sta[C_RHO * cap + c], ww, (uint32_t)orig[c], R, aa);
+  fill_stage(g, rr, rad, vv, ww, R, aa);
 
   if (mode == 2) {
     track_update(lt, c, r0, rr, rad0 / p.sphere_div, rot_step);
@@ -345,7 +346,7 @@ struct ClumpRegs {
 // record (the angular acceleration in st), and the chunk's last step writes them back to st.
 __device__ __forceinline__ void predict_regs(ClumpRegs &s, double *__restrict__ sta,
                                              double *__restrict__ g, const InstParams &p,
-                                             uint32_t id, double rad, double rad0, double rho,
+                                             double rad, double rad0,
                                              int64_t c, int64_t cap, ListTrack lt,
                                              double t_old) {
   double rot_step = 0.0;
@@ -356,13 +357,13 @@ __device__ __forceinline__ void predict_regs(ClumpRegs &s, double *__restrict__ 
   } else {
     rot_step = verlet_advance(s.rr, s.vv, s.ww, s.R, s.aa, s.ang, p.dt, p.hdt);
   }
-  fill_stage(g, s.rr, rad, s.vv, rho, s.ww, id, s.R, s.aa);
+  fill_stage(g, s.rr, rad, s.vv, s.ww, s.R, s.aa);
   track_update(lt, c, r0, s.rr, rad0 / p.sphere_div, rot_step);
 }
 
-// The stage records of a workgroup's 256 clumps are assembled in LDS (rows padded to 25
-// doubles against bank conflicts) and written as one contiguous span, instead of 24
-// scattered 192-B-strided stores per lane.
+// The stage records of a workgroup's 256 clumps are assembled in LDS (rows padded to 23
+// doubles against bank conflicts) and written as one contiguous span, instead of 22
+// scattered 176-B-strided stores per lane.
 constexpr int kPredictBlock = 256;
 constexpr int kStagePad = kStageDoubles + 1;
 __global__ __launch_bounds__(kPredictBlock) void k_predict(
@@ -899,7 +900,8 @@ constexpr int kSphListV = 24; // candidate spheres per sphere
 template <int BS>
 __global__ __launch_bounds__(BS) void k_vmerge(
     const double *__restrict__ stage, const InstParams *__restrict__ P,
-    const int32_t *__restrict__ inst_of, const uint32_t *__restrict__ clist,
+    const int32_t *__restrict__ inst_of, const int32_t *__restrict__ orig,
+    const uint32_t *__restrict__ clist,
     const int32_t *__restrict__ ccnt, int32_t *__restrict__ vl_m, uint16_t *__restrict__ vl_mask,
     int32_t *__restrict__ vl_cnt, ListTrack lt, DevDiag *__restrict__ diag, int64_t n,
     int64_t cap, double skin) {
@@ -950,7 +952,7 @@ __global__ __launch_bounds__(BS) void k_vmerge(
         if (di >= -1 && di <= 1 && dj >= -1 && dj <= 1 && dk >= -1 && dk <= 1)
           off = (uint32_t)((di + 1) * 9 + (dj + 1) * 3 + (dk + 1));
       }
-      const uint32_t key = (off << 27) | (uint32_t)__double_as_longlong(g[G_ID]);
+      const uint32_t key = (off << 27) | (uint32_t)orig[cm];
       if (cnt == kVMax) {
         overflow = true;
         continue;
@@ -1091,8 +1093,8 @@ __device__ __forceinline__ void force_clump(
     const uint32_t *__restrict__ clist, const int32_t *__restrict__ ccnt,
     const int32_t *__restrict__ vl_m, const uint16_t *__restrict__ vl_mask,
     const int32_t *__restrict__ vl_cnt, const uint8_t *__restrict__ wreach,
-    double *__restrict__ fout, DevDiag *__restrict__ diag, int64_t cap, int initial,
-    double time, const ClumpRegs &own, ClumpRegs &out) {
+    double *__restrict__ fout, DevDiag *__restrict__ diag, const int32_t *__restrict__ orig,
+    int64_t cap, int initial, double time, const ClumpRegs &own, ClumpRegs &out) {
   static_assert(!NEXT || VERLET || CELLS, "the fused predict needs the record-resident state");
   static_assert(!CELLS || LEAN, "per-step binning keeps the state in the records");
   constexpr bool kRecState = VERLET || CELLS;  // the state lives in the stage records
@@ -1225,7 +1227,7 @@ __device__ __forceinline__ void force_clump(
         const double *g = stage + cm * kStageDoubles;
         rm = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
         radm = g[G_RAD];
-        idm = g[G_ID];
+        idm = __longlong_as_double((long long)(uint32_t)orig[cm]);
 #pragma unroll
         for (int k = 0; k < 9; k++) Rm[k] = g[G_R + k];
       };
@@ -1294,8 +1296,7 @@ __device__ __forceinline__ void force_clump(
             continue;
           }
           const double *g = stage + cm * kStageDoubles;
-          place(cm, make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]),
-                (uint32_t)__double_as_longlong(g[G_ID]), bit);
+          place(cm, make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]), (uint32_t)orig[cm], bit);
         }
       }
     }
@@ -1341,7 +1342,7 @@ __device__ __forceinline__ void force_clump(
         const int64_t cm = sh.m[e][lane];
         const double *g = stage + cm * kStageDoubles;
         const d3 rrm = make_d3(g[G_RR], g[G_RR + 1], g[G_RR + 2]);
-        const double rad_2 = g[G_RAD], rhom = g[G_RHO];
+        const double rad_2 = g[G_RAD], rhom = sta[C_RHO * cap + cm];
         const d3 vvm = make_d3(g[G_VV], g[G_VV + 1], g[G_VV + 2]);
         const d3 wwm = make_d3(g[G_WW], g[G_WW + 1], g[G_WW + 2]);
         const d3 r2 = rec_sphere(g, rad_2 / p.sphere_div, sdiv);
@@ -1544,14 +1545,14 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
   constexpr bool kRegsOut = NEXT && LEAN;
   if (kOwnStage && c < n) {  // the own record, whole, in 16-byte loads
     const double2 *o = reinterpret_cast<const double2 *>(stage + c * kStageDoubles);
-    const double2 a = o[0], b = o[1], v0 = o[2], v1 = o[3], w0 = o[4], w1 = o[5];
-    const double2 r0 = o[6], r1 = o[7], r2 = o[8], r3 = o[9], r4 = o[10], r5 = o[11];
+    const double2 a = o[0], b = o[1], v0 = o[2], v1 = o[3], w1 = o[4];
+    const double2 r0 = o[5], r1 = o[6], r2 = o[7], r3 = o[8], r4 = o[9], r5 = o[10];
     own.rr = make_d3(a.x, a.y, b.x);
     own.rad = b.y;
     own.vv = make_d3(v0.x, v0.y, v1.x);
-    own.rho = v1.y;
-    own.ww = make_d3(w0.x, w0.y, w1.x);
-    own.id = (uint32_t)__double_as_longlong(w1.y);
+    own.ww = make_d3(v1.y, w1.x, w1.y);
+    own.rho = sta[C_RHO * cap + c];
+    own.id = (uint32_t)orig[c];
     own.R[0] = r0.x;
     own.R[1] = r0.y;
     own.R[2] = r1.x;
@@ -1571,7 +1572,7 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
         const double t = chunk_time(q, si);
         force_clump<MAXN, BS, VERLET, NEXT, LEAN, CLOSED, DEEP, CELLS>(
             q, sh, lane, c, st, sta, stage, clist, ccnt, vl_m, vl_mask, vl_cnt, lt.wreach, fout,
-            diag, cap, initial, time_ahead ? t + q.dt : t, own, cr);
+            diag, orig, cap, initial, time_ahead ? t + q.dt : t, own, cr);
         break;
       }
     }
@@ -1616,7 +1617,7 @@ __attribute__((amdgpu_waves_per_eu(1, DEEP ? 1 : 8))) void k_force(
           cg = clump_grid(q);
           const double rad0 = own.rad != 0.0 ? own.rad : sta[C_RAD0 * cap + c];
           if constexpr (LEAN)
-            predict_regs(cr, sta, rec + lane * kStagePad, q, own.id, own.rad, rad0, own.rho, c,
+            predict_regs(cr, sta, rec + lane * kStagePad, q, own.rad, rad0, c,
                          cap, lt, t_old);
           else
             predict_clump(st, sta, rec + lane * kStagePad, q, orig, nullptr, c, cap, 2, lt,
@@ -2800,7 +2801,7 @@ __global__ void k_export_contacts(const double *__restrict__ st, const double *_
   const double *gl = stage + c * kStageDoubles;
   const d3 rr = make_d3(gl[G_RR], gl[G_RR + 1], gl[G_RR + 2]);
   if (frozen(p.time, rr.x, p.side_x)) return;
-  const double rad = gl[G_RAD], rho = gl[G_RHO];
+  const double rad = gl[G_RAD], rho = sta[C_RHO * cap + c];
   const d3 vv = make_d3(gl[G_VV], gl[G_VV + 1], gl[G_VV + 2]);
   const d3 ww = make_d3(gl[G_WW], gl[G_WW + 1], gl[G_WW + 2]);
   const double massl = kSphFact * rho * rad * rad * rad;
@@ -2826,7 +2827,7 @@ __global__ void k_export_contacts(const double *__restrict__ st, const double *_
           const double d2 = dx * dx + dy * dy + dz * dz;
           if (!(d2 < sep2)) continue;
           const double *g = stage + cm * kStageDoubles;
-          const double rad_2 = g[G_RAD], rhom = g[G_RHO];
+          const double rad_2 = g[G_RAD], rhom = sta[C_RHO * cap + cm];
           const double massm = kSphFact * rhom * rad_2 * rad_2 * rad_2;
           const double meff = 0.5 * (massm + massl);
           const double km = 0.5 * p.kappa_pp * meff, gm = 0.5 * p.gamma_pp * meff;
@@ -2844,7 +2845,7 @@ __global__ void k_export_contacts(const double *__restrict__ st, const double *_
           const unsigned long long k = atomicAdd(count, 1ull);
           if ((int64_t)k < capn) {
             out[4 * k + 0] = orig[c];
-            out[4 * k + 1] = (int64_t)(uint32_t)__double_as_longlong(g[G_ID]);
+            out[4 * k + 1] = orig[cm];
             out[4 * k + 2] = ((int64_t)s) | ((int64_t)(id & 3) << 8);
             out[4 * k + 3] = rep ? 1 : 0;
           }

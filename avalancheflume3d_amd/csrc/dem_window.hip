@@ -96,7 +96,8 @@ __global__ void k_win_finalize(const InstParams *__restrict__ P_in, InstParams *
 __global__ __launch_bounds__(64) void k_win_list(
     const double *__restrict__ st, const double *__restrict__ sta,
     const double *__restrict__ stage, const InstParams *__restrict__ P,
-    const int32_t *__restrict__ inst_of, const uint32_t *__restrict__ starts,
+    const int32_t *__restrict__ inst_of, const int32_t *__restrict__ orig,
+    const uint32_t *__restrict__ starts,
     const double4 *__restrict__ spos, const uint32_t *__restrict__ sid, double skin,
     int32_t *__restrict__ list, int32_t *__restrict__ cnt_out, double *__restrict__ sph0,
     double *__restrict__ dmax2, WinDiag *__restrict__ wd, int64_t n, int64_t cap) {
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(64) void k_win_list(
             if (di >= -1 && di <= 1 && dj >= -1 && dj <= 1 && dk >= -1 && dk <= 1)
               off = (uint32_t)((di + 1) * 9 + (dj + 1) * 3 + (dk + 1));
           }
-          const uint32_t key = (off << 27) | (uint32_t)__double_as_longlong(g[G_ID]);
+          const uint32_t key = (off << 27) | (uint32_t)orig[cm];
           if (cnt == kWinMaxNeigh) {
             overflow = true;
             continue;

@@ -26,13 +26,13 @@ sys.path.insert(0, ROOT)
 METRIC = "clump-steps/s + HBM GB/s at 250k clumps; batched vs plain; adjoint/forward cost"
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per clump and launch of k_force on a candidate-list step of a lean chunk
-# (which also predicts the next step): reads the clump's stage record (rr, radius, v,
-# density, w, reference index, R, a: 192 B), the angular acceleration (24 B), mass and
-# inertia (32 B), the instance index and list length (8 B) and the displacement bound
-# (8 B); writes the next stage record (192 B), the angular acceleration (24 B) and the
+# (which also predicts the next step): reads the clump's stage record (rr, radius, v, w, R,
+# a: 176 B), its density and reference index (12 B), the angular acceleration (24 B), mass
+# and inertia (32 B), the instance index and list length (8 B) and the displacement bound
+# (8 B); writes the next stage record (176 B), the angular acceleration (24 B) and the
 # bound (8 B).  The list entries (6 B each) and the neighbours' stage records (L2 hits) are
 # not counted.
-FORCE_BYTES_PER_CLUMP = 488.0
+FORCE_BYTES_PER_CLUMP = 468.0
 # Mass and inertia are not read where the steps recompute them from radius and density
 # (per-step binning, and candidate lists for working sets beyond the last-level cache;
 # avf_diag.closed_instances says so): 32 B less there.
@@ -49,27 +49,28 @@ TIMED_EVERY = 8
 # Spatial reorder interval (steps) of the candidate-list comparison runs.
 DEFAULT_REORDER = 400
 # Per-step binning by sphere (--binning sphere): algorithmic bytes per clump and launch of
-# k_force (a chunk's steps but its last): the clump's stage record (192 B), the angular acceleration (24 B),
-# mass and inertia (32 B), the instance index (4 B), the four per-sphere contact counts
-# (16 B) read; the next stage record (192 B), the angular acceleration (24 B) and the four
-# sphere-cell keys (16 B) written.  The contact entries (4 B each) and the neighbours'
-# records (gathered, cache hits) are not counted; the keys are counted by their own kernel.
-CELLS_FORCE_BYTES = 500.0
+# k_force (a chunk's steps but its last): the clump's stage record (176 B), its density and
+# reference index (12 B), the angular acceleration (24 B), mass and inertia (32 B), the
+# instance index (4 B), the four per-sphere contact counts (16 B) read; the next stage
+# record (176 B), the angular acceleration (24 B) and the four sphere-cell keys (16 B)
+# written.  The contact entries (4 B each) and the neighbours' records (gathered, cache
+# hits) are not counted; the keys are counted by their own kernel.
+CELLS_FORCE_BYTES = 480.0
 # Per-step binning by sphere: algorithmic bytes per clump of the contact sweep (k_sweep):
 # each binned sphere's sorted record
 # (x, y, z, r: 32 B) and id (4 B) read once, two cell starts per sphere (the grid holds about
 # two cells per sphere: 8 B), its contact count (4 B) written; contact entries not counted.
 SWEEP_BYTES_PER_CLUMP = 4 * (32.0 + 4.0 + 8.0 + 4.0)
-# Per-step binning by clump (the default): k_force reads the stage record (192 B), the
-# angular acceleration (24 B), mass and inertia (32 B), the instance index and list length
-# (8 B), and writes the next stage record (192 B), the angular acceleration (24 B), the next
-# clump-cell key and the clump's rank in its cell (8 B) and the sweep's float record (centre
-# and radius as floats: 16 B): 496 B; list entries, the neighbours' records and the cell
+# Per-step binning by clump (the default): k_force reads the stage record (176 B), the
+# density and reference index (12 B), the angular acceleration (24 B), mass and inertia
+# (32 B), the instance index and list length (8 B), and writes the next stage record
+# (176 B), the angular acceleration (24 B), the next clump-cell key and the clump's rank in
+# its cell (8 B) and the sweep's float record (centre and radius as floats: 16 B): 476 B; list entries, the neighbours' records and the cell
 # counters (atomics in L2) not counted.  k_csweep reads each binned clump's centre and radius
 # in cell order (16 B), its storage index and key (8 B) and two cell starts (8 B): 32 B; the
 # candidates' centres (read again by neighbouring clumps, from cache), the stage records of
 # clumps with candidates that pass the bounding test, and the list entries not counted.
-CLUMP_FORCE_BYTES = 496.0
+CLUMP_FORCE_BYTES = 476.0
 CSWEEP_BYTES_PER_CLUMP = 32.0
 KERNEL_NAMES = {"sphere": {"force": "k_force", "detect": "k_sweep"},
                 "clump": {"force": "k_force", "detect": "k_csweep"}}
