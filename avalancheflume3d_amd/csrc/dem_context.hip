@@ -972,16 +972,12 @@ int avf_create(avf_ctx **out, const avf_instance_desc *inst, int32_t ninst,
   // the angular acceleration, so a list step reads and writes each clump's state once
   c->lean = c->opt.lean_state >= 0;
   // the steps recompute clump mass and inertia from radius and density (where they hold
-  // the closed forms) unless asked otherwise: always with per-step binning (1 % faster at
-  // 250k clumps), on candidate lists for working sets beyond the last-level cache, where
-  // the bytes saved outweigh the arithmetic (+2.7 % at 2M clumps; at 250k clumps the loads
-  // are cache hits and the arithmetic would cost 1 %)
+  // the closed forms) unless asked otherwise: the bytes saved outweigh the arithmetic
+  // (per-step binning at 250k clumps 1 % faster, candidate lists 0.5 %, at 2M clumps 2.7 %)
   // contexts of at most one wavefront per SIMD (256 CUs x 4 SIMDs x 64 clumps) are bound by
   // the latency of the candidate gathers, not by occupancy
   c->deep = total <= 256 * 4 * kForceBlock;
-  c->closed = c->opt.closed_forms > 0 ||
-              (c->opt.closed_forms == 0 && (c->opt.verlet_skin <= 0.0 ||
-                                            (double)total * 488.0 > 256.0 * 1024.0 * 1024.0));
+  c->closed = c->opt.closed_forms >= 0;
   c->cell_cap = std::max<int64_t>(8 * 4 * total, 1 << 16);
   if (c->cell_cap > (int64_t)0xfffffff0) c->cell_cap = 0xfffffff0;
 

@@ -34,8 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # not counted.
 FORCE_BYTES_PER_CLUMP = 468.0
 # Mass and inertia are not read where the steps recompute them from radius and density
-# (per-step binning, and candidate lists for working sets beyond the last-level cache;
-# avf_diag.closed_instances says so): 32 B less there.
+# (avf_diag.closed_instances says so): 32 B less there.
 MASS_INERTIA_BYTES = 32.0
 # Algorithmic bytes per clump-step of the whole step (480 B per clump-step).
 STEP_BYTES_PER_CLUMP = 480.0

@@ -149,12 +149,10 @@ typedef struct avf_options {
   double verlet_skin;       /* > 0: candidate lists of sphere pairs within sep + skin, */
                             /* rebuilt only when some sphere has moved skin / 2 since  */
                             /* the build (results unchanged); 0: rebuild every step    */
-  int32_t closed_forms;     /* clump mass and inertia in the step: 1 = recomputed from  */
-                            /* radius and density when every clump of an instance holds */
-                            /* the closed forms bit for bit (fewer loads), -1 = always  */
-                            /* loaded, 0 = recomputed with per-step binning, and on     */
-                            /* candidate lists for working sets beyond the 256 MiB      */
-                            /* last-level cache; same results                          */
+  int32_t closed_forms;     /* clump mass and inertia in the step: 0 or 1 = recomputed  */
+                            /* from radius and density when every clump of an instance  */
+                            /* holds the closed forms bit for bit (fewer loads), -1 =   */
+                            /* always loaded; same results                              */
   int32_t step_binning;     /* per-step binning (verlet_skin 0): 0 = one key per clump   */
                             /* in cells of the largest clump's bounding diameter and a  */
                             /* list of candidate sphere pairings per clump, re-tested  */

@@ -347,10 +347,10 @@ def test_closed_form_mass_and_inertia_change_no_bit(gpu, skin):
 
 
 def test_large_configuration_default_path(gpu):
-    """2,000,000 polydisperse clumps (lean chunks, and by default for this working set the
-    mass and inertia recomputed from radius and density) against the same run with them
-    loaded: 20 candidate-list steps, bit for bit, with no clump outside the clump-cell grid
-    and no neighbour-list overflow."""
+    """2,000,000 polydisperse clumps (lean chunks, and by default the mass and inertia
+    recomputed from radius and density) against the same run with them loaded: 20
+    candidate-list steps, bit for bit, with no clump outside the clump-cell grid and no
+    neighbour-list overflow."""
     from avalancheflume3d_amd import workloads
     inst = workloads.large()
     outs = []
