@@ -2342,12 +2342,17 @@ __global__ __launch_bounds__(kCsBlock) void k_csweep(
   __shared__ uint32_t rb[9][kCsBlock], re[9][kCsBlock];
   const int tid = threadIdx.x;
   const int64_t k = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kCsBlock + tid;
+  if (k >= n) return;
+  // the lane's clump is loaded together with the binned count instead of after it, so that
+  // the two latencies overlap.  A lane past the binned clumps (its entry left by an earlier
+  // step, or never written) walks no candidate: any key gives rows inside the instance's
+  // grid (the inside test below), and its ranges are emptied by the binned count.
+  const uint2 ik = cik[k];
+  const float4 qa = sca[k];
   const uint32_t sk = starts[*ncells_p];
   const int64_t nbin = sk < (uint64_t)n ? (int64_t)sk : n;  // clumps binned this step
-  if (k >= nbin) return;
-  const uint2 ik = cik[k];
+  const uint32_t lim = k < nbin ? (uint32_t)nbin : 0u;
   const uint32_t key = ik.y, me = ik.x;
-  const float4 qa = sca[k];
   int inst = 0;
   if (ninst > 1) {  // last instance whose first cell is <= key
     int lo = 0, hi = ninst - 1;
@@ -2383,8 +2388,8 @@ __global__ __launch_bounds__(kCsBlock) void k_csweep(
     }
 #pragma unroll
     for (int r9 = 0; r9 < 9; r9++) {  // within the sorted clumps whatever happens
-      rb[r9][tid] = b9[r9] < (uint64_t)nbin ? b9[r9] : (uint32_t)nbin;
-      re[r9][tid] = e9[r9] < (uint64_t)nbin ? e9[r9] : (uint32_t)nbin;
+      rb[r9][tid] = b9[r9] < lim ? b9[r9] : lim;
+      re[r9][tid] = e9[r9] < lim ? e9[r9] : lim;
     }
     const float mh0 = (float)p.cg_mh, bfac = (float)p.cg_bfac;
     // a bound on the own record's coordinates (as pair_margin bounds the candidate's)
